@@ -1,0 +1,367 @@
+"""ctypes bindings to the CPU checkers. TEST INFRASTRUCTURE ONLY.
+
+Two libraries, both built by ``oracle/Makefile``:
+
+* ``build/libo2d_oracle.so`` -- the fp64 C restatement (``o2d_oracle.c``), the parity oracle.
+* ``_ref/libo2d_ref.so``     -- the reference's own templates compiled from its sources
+  (``ref_harness.cpp``); used to write golden fixtures, to validate the restatement, and as
+  the "reference" CPU baseline. Absent where the reference tree was never available.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
+module, always as the checker or the timed CPU baseline, never as the thing measured.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "build", "libo2d_oracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libo2d_ref.so")
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+
+CORES = {"rm": 1, "gauss": 2, "wl": 3, "v2": 4, "v3": 5}
+RESULTS = {"vel": 1, "vort": 4, "shear": 5}
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        L = ctypes.CDLL(ORACLE_SO)
+        L.oc_points_on_points.argtypes = [_int, _int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                          _vp, _vp, _vp, _vp, _vp]
+        L.oc_panels_on_points.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp,
+                                          _vp, _vp]
+        L.oc_points_on_panels.argtypes = [_i64, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp,
+                                          _vp]
+        L.oc_panel_geometry.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.oc_body_volume.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp]
+        L.oc_unit_rot_strengths.argtypes = [_i64, _vp, _vp, _i64, _vp, _dbl, _dbl, _vp, _vp]
+        L.oc_bem_assemble.argtypes = [_i64, _vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _dbl,
+                                      _vp, _i64]
+        L.oc_vels_to_rhs.argtypes = [_i64, _int, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.oc_finalize.argtypes = [_i64, _vp, _vp, _dbl, _dbl]
+        L.oc_rk_combine.argtypes = [_i64, _int, _vp, _vp, _vp, _dbl, _vp, _vp]
+        L.oc_points_on_points_adjoint.argtypes = [_int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                                  _vp, _vp, _vp, _vp, _vp, _vp]
+        L.oc_morton_keys.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _dbl, _vp]
+        _lib = L
+    return _lib
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise RuntimeError(f"reference build missing: {REF_SO}")
+        R = ctypes.CDLL(REF_SO)
+        pp = [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp]
+        R.ref_points_on_points_d.argtypes = pp
+        R.ref_points_on_points_f.argtypes = pp
+        pa = [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]
+        R.ref_panels_on_points_d.argtypes = pa
+        R.ref_panels_on_points_f.argtypes = pa
+        pb = [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp]
+        R.ref_points_on_panels_d.argtypes = pb
+        R.ref_points_on_panels_f.argtypes = pb
+        pc = [_i64, _vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        R.ref_bem_coeffs_d.argtypes = pc
+        R.ref_bem_coeffs_f.argtypes = pc
+        R.ref_rhs_panels_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _int, _vp, _vp, _vp, _vp]
+        R.ref_points_move2_d.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl]
+        R.ref_points_move3_d.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
+                                         _dbl, _dbl, _dbl]
+        R.ref_finalize_d.argtypes = [_i64, _vp, _vp, _dbl, _dbl]
+        _ref = R
+    return _ref
+
+
+class _Quiet:
+    """Silence fd 1 while the reference (which logs every call to stdout) runs."""
+
+    def __enter__(self):
+        import sys
+        sys.stdout.flush()
+        self._saved = os.dup(1)
+        self._null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self._null, 1)
+        return self
+
+    def __exit__(self, *exc):
+        os.dup2(self._saved, 1)
+        os.close(self._saved)
+        os.close(self._null)
+        return False
+
+
+quiet = _Quiet
+
+
+# ----------------------------------------------------------------------------- restatement
+
+def points_on_points(core, sx, sy, ss, sr, tx, ty, tr=None, restype="vel"):
+    """fp64 unscaled sums (u, v[, w|e]) at targets; tr=None means point (inert) targets."""
+    sx, sy, ss, sr, tx, ty = map(_f64, (sx, sy, ss, sr, tx, ty))
+    tr = None if tr is None else _f64(tr)
+    nt = len(tx)
+    tu, tv = np.zeros(nt), np.zeros(nt)
+    tw = np.zeros(nt) if restype == "vort" else None
+    te = np.zeros(nt) if restype == "shear" else None
+    rc = lib().oc_points_on_points(CORES[core], RESULTS[restype], len(sx), _p(sx), _p(sy), _p(ss),
+                                   _p(sr), nt, _p(tx), _p(ty), _p(tr), _p(tu), _p(tv), _p(tw),
+                                   _p(te))
+    assert rc == 0, rc
+    if restype == "vort":
+        return tu, tv, tw
+    if restype == "shear":
+        return tu, tv, te
+    return tu, tv
+
+
+def panels_on_points(nx, ny, idx, vs, ss, tx, ty):
+    nx, ny, vs, tx, ty = map(_f64, (nx, ny, vs, tx, ty))
+    ss = None if ss is None else _f64(ss)
+    idx = _u32(idx)
+    nt = len(tx)
+    tu, tv = np.zeros(nt), np.zeros(nt)
+    rc = lib().oc_panels_on_points(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), _p(vs), _p(ss),
+                                   nt, _p(tx), _p(ty), _p(tu), _p(tv))
+    assert rc == 0, rc
+    return tu, tv
+
+
+def points_on_panels(sx, sy, ss, nx, ny, idx):
+    sx, sy, ss, nx, ny = map(_f64, (sx, sy, ss, nx, ny))
+    idx = _u32(idx)
+    np_ = len(idx) // 2
+    pu, pv = np.zeros(np_), np.zeros(np_)
+    rc = lib().oc_points_on_panels(len(sx), _p(sx), _p(sy), _p(ss), len(nx), _p(nx), _p(ny), np_,
+                                   _p(idx), _p(pu), _p(pv))
+    assert rc == 0, rc
+    return pu, pv
+
+
+def panel_geometry(nx, ny, idx):
+    nx, ny = _f64(nx), _f64(ny)
+    idx = _u32(idx)
+    np_ = len(idx) // 2
+    out = [np.zeros(np_) for _ in range(5)]
+    rc = lib().oc_panel_geometry(len(nx), _p(nx), _p(ny), np_, _p(idx), *[_p(o) for o in out])
+    assert rc == 0, rc
+    return tuple(out)  # tang_x, tang_y, norm_x, norm_y, area
+
+
+def body_volume(nx, ny, idx):
+    nx, ny = _f64(nx), _f64(ny)
+    idx = _u32(idx)
+    vol, cx, cy = _dbl(), _dbl(), _dbl()
+    rc = lib().oc_body_volume(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), ctypes.byref(vol),
+                              ctypes.byref(cx), ctypes.byref(cy))
+    assert rc == 0, rc
+    return vol.value, cx.value, cy.value
+
+
+def unit_rot_strengths(nx, ny, idx, cx, cy):
+    nx, ny = _f64(nx), _f64(ny)
+    idx = _u32(idx)
+    np_ = len(idx) // 2
+    vs, ss = np.zeros(np_), np.zeros(np_)
+    rc = lib().oc_unit_rot_strengths(len(nx), _p(nx), _p(ny), np_, _p(idx), cx, cy, _p(vs), _p(ss))
+    assert rc == 0, rc
+    return vs, ss
+
+
+def bem_assemble(nx, ny, idx, nunk, augmented, urot_u=None, urot_v=None, vol=0.0):
+    nx, ny = _f64(nx), _f64(ny)
+    idx = _u32(idx)
+    np_ = len(idx) // 2
+    n = np_ * nunk + (1 if augmented else 0)
+    A = np.zeros(n * n)
+    uu = None if urot_u is None else _f64(urot_u)
+    uv = None if urot_v is None else _f64(urot_v)
+    rc = lib().oc_bem_assemble(len(nx), _p(nx), _p(ny), np_, _p(idx), nunk, int(augmented),
+                               _p(uu), _p(uv), vol, _p(A), n)
+    assert rc == 0, rc
+    return A.reshape(n, n).T  # column-major storage -> A[row, col]
+
+
+def vels_to_rhs(pu, pv, tang_x, tang_y, nunk, norm_bc=None):
+    pu, pv, tang_x, tang_y = map(_f64, (pu, pv, tang_x, tang_y))
+    nb = None if norm_bc is None else _f64(norm_bc)
+    rhs = np.zeros(len(pu) * nunk)
+    rc = lib().oc_vels_to_rhs(len(pu), nunk, _p(pu), _p(pv), _p(tang_x), _p(tang_y), _p(nb), _p(rhs))
+    assert rc == 0, rc
+    return rhs
+
+
+def finalize(u, v, fs=(0.0, 0.0)):
+    u, v = _f64(u).copy(), _f64(v).copy()
+    lib().oc_finalize(len(u), _p(u), _p(v), float(fs[0]), float(fs[1]))
+    return u, v
+
+
+def rk_combine(x, y, us, vs, w, dt):
+    x, y = _f64(x).copy(), _f64(y).copy()
+    us = [_f64(a) for a in us]
+    vs = [_f64(a) for a in vs]
+    w = _f64(w)
+    up = (_vp * len(us))(*[a.ctypes.data for a in us])
+    vp = (_vp * len(vs))(*[a.ctypes.data for a in vs])
+    rc = lib().oc_rk_combine(len(x), len(us), _p(w), up, vp, float(dt), _p(x), _p(y))
+    assert rc == 0, rc
+    return x, y
+
+
+def points_on_points_adjoint(core, sx, sy, ss, sr, tx, ty, gu, gv):
+    sx, sy, ss, sr, tx, ty, gu, gv = map(_f64, (sx, sy, ss, sr, tx, ty, gu, gv))
+    ns = len(sx)
+    out = [np.zeros(ns) for _ in range(4)]
+    rc = lib().oc_points_on_points_adjoint(CORES[core], ns, _p(sx), _p(sy), _p(ss), _p(sr), len(tx),
+                                           _p(tx), _p(ty), _p(gu), _p(gv), *[_p(o) for o in out])
+    assert rc == 0, rc
+    return tuple(out)  # gx, gy, gs, gr
+
+
+def morton_keys(x, y, x0, y0, inv_extent):
+    x, y = _f64(x), _f64(y)
+    keys = np.zeros(len(x), dtype=np.uint32)
+    lib().oc_morton_keys(len(x), _p(x), _p(y), float(x0), float(y0), float(inv_extent), _p(keys))
+    return keys
+
+
+# ----------------------------------------------------------------------------- reference build
+
+def ref_points_on_points(sx, sy, ss, sr, tx, ty, tr=None, restype="vel", dtype=np.float64):
+    """Reference points_affect_points at S=dtype (A=double), V2 core (the compiled default)."""
+    c = lambda a: np.ascontiguousarray(a, dtype=dtype)
+    sx, sy, ss, sr, tx, ty = map(c, (sx, sy, ss, sr, tx, ty))
+    tr = None if tr is None else c(tr)
+    nt = len(tx)
+    tu, tv = np.zeros(nt, dtype), np.zeros(nt, dtype)
+    tw = np.zeros(nt, dtype) if restype == "vort" else None
+    te = np.zeros(nt, dtype) if restype == "shear" else None
+    fn = ref().ref_points_on_points_d if dtype == np.float64 else ref().ref_points_on_points_f
+    with quiet():
+        rc = fn(len(sx), _p(sx), _p(sy), _p(ss), _p(sr), nt, _p(tx), _p(ty), _p(tr),
+                RESULTS[restype], _p(tu), _p(tv), _p(tw), _p(te))
+    assert rc == 0, rc
+    if restype == "vort":
+        return tu, tv, tw
+    if restype == "shear":
+        return tu, tv, te
+    return tu, tv
+
+
+def ref_panels_on_points(nx, ny, idx, vs, ss, tx, ty, dtype=np.float64):
+    c = lambda a: np.ascontiguousarray(a, dtype=dtype)
+    nx, ny = np.ascontiguousarray(nx, np.float32), np.ascontiguousarray(ny, np.float32)
+    vs, tx, ty = map(c, (vs, tx, ty))
+    ss = None if ss is None else c(ss)
+    idx = _u32(idx)
+    nt = len(tx)
+    tu, tv = np.zeros(nt, dtype), np.zeros(nt, dtype)
+    fn = ref().ref_panels_on_points_d if dtype == np.float64 else ref().ref_panels_on_points_f
+    with quiet():
+        rc = fn(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), _p(vs), _p(ss), nt, _p(tx), _p(ty),
+                _p(tu), _p(tv))
+    assert rc == 0, rc
+    return tu, tv
+
+
+def ref_points_on_panels(sx, sy, ss, sr, nx, ny, idx, dtype=np.float64):
+    c = lambda a: np.ascontiguousarray(a, dtype=dtype)
+    sx, sy, ss, sr = map(c, (sx, sy, ss, sr))
+    nx, ny = np.ascontiguousarray(nx, np.float32), np.ascontiguousarray(ny, np.float32)
+    idx = _u32(idx)
+    np_ = len(idx) // 2
+    pu, pv = np.zeros(np_, dtype), np.zeros(np_, dtype)
+    fn = ref().ref_points_on_panels_d if dtype == np.float64 else ref().ref_points_on_panels_f
+    with quiet():
+        rc = fn(len(sx), _p(sx), _p(sy), _p(ss), _p(sr), len(nx), _p(nx), _p(ny), np_, _p(idx),
+                _p(pu), _p(pv))
+    assert rc == 0, rc
+    return pu, pv
+
+
+def ref_bem_coeffs(nx, ny, idx, nunk, body_kind, dtype=np.float64):
+    """Reference BEM self-block (augmented when the body calls for it), plus the inputs the
+    restatement needs: tangents, lengths, volume, unit-rotation panel velocities."""
+    nx, ny = np.ascontiguousarray(nx, np.float32), np.ascontiguousarray(ny, np.float32)
+    idx = _u32(idx)
+    np_ = len(idx) // 2
+    nmax = np_ * nunk + 1
+    A = np.zeros(nmax * nmax, dtype)
+    nr, nc = _i64(), _i64()
+    tx, ty, ar, uu, uv = (np.zeros(np_, dtype) for _ in range(5))
+    vol = np.zeros(1, dtype)
+    fn = ref().ref_bem_coeffs_d if dtype == np.float64 else ref().ref_bem_coeffs_f
+    with quiet():
+        rc = fn(len(nx), _p(nx), _p(ny), np_, _p(idx), nunk, body_kind, _p(A), ctypes.byref(nr),
+                ctypes.byref(nc), _p(tx), _p(ty), _p(ar), _p(vol), _p(uu), _p(uv))
+    assert rc == 0, rc
+    n = nr.value
+    return dict(A=A[: n * n].reshape(n, n).T.copy(), tang_x=tx, tang_y=ty, area=ar,
+                vol=float(vol[0]), urot_u=uu, urot_v=uv)
+
+
+def ref_rhs_panels(nx, ny, idx, nunk, pu, pv, norm_bc):
+    nx, ny = np.ascontiguousarray(nx, np.float32), np.ascontiguousarray(ny, np.float32)
+    idx = _u32(idx)
+    pu, pv = _f64(pu), _f64(pv)
+    nb = np.ascontiguousarray(norm_bc, np.float32)
+    np_ = len(idx) // 2
+    rhs = np.zeros(np_ * nunk)
+    with quiet():
+        rc = ref().ref_rhs_panels_d(len(nx), _p(nx), _p(ny), np_, _p(idx), nunk, _p(pu), _p(pv),
+                                    _p(nb), _p(rhs))
+    assert rc == 0, rc
+    return rhs
+
+
+def ref_move(x, y, us, vs, w, dt):
+    x, y = _f64(x).copy(), _f64(y).copy()
+    us = [_f64(a) for a in us]
+    vs = [_f64(a) for a in vs]
+    with quiet():
+        if len(us) == 2:
+            rc = ref().ref_points_move2_d(len(x), _p(x), _p(y), _p(us[0]), _p(vs[0]), _p(us[1]),
+                                          _p(vs[1]), float(dt), float(w[0]), float(w[1]))
+        else:
+            rc = ref().ref_points_move3_d(len(x), _p(x), _p(y), _p(us[0]), _p(vs[0]), _p(us[1]),
+                                          _p(vs[1]), _p(us[2]), _p(vs[2]), float(dt), float(w[0]),
+                                          float(w[1]), float(w[2]))
+    assert rc == 0, rc
+    return x, y
+
+
+def ref_finalize(u, v, fs):
+    u, v = _f64(u).copy(), _f64(v).copy()
+    with quiet():
+        ref().ref_finalize_d(len(u), _p(u), _p(v), float(fs[0]), float(fs[1]))
+    return u, v

@@ -1,0 +1,291 @@
+// ref_harness.cpp - a C-ABI shim over the Omega2D reference's own templates.
+//
+// TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile against the reference headers where
+// they lie (src/Influence.h, src/Coefficients.h, src/RHS.h, src/Points.h, src/Surfaces.h),
+// never copied. The shared library lands in oracle/_ref/ and is used only to write the golden
+// fixtures under tests/golden/, to validate the C restatement in oracle/o2d_oracle.c, and as
+// the "reference" CPU baseline timed beside the GPU numbers. Nothing in the product links it.
+//
+// The reference core function is a compile-time choice (src/CoreFunc.h:35-39, Vatistas n=2 is
+// hard-wired); this shim therefore pins the V2 core only. Every other core is pinned by closed
+// forms in the tests.
+
+#include "Collection.h"
+#include "Influence.h"
+#include "Coefficients.h"
+#include "RHS.h"
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+namespace {
+
+// Surfaces with the protected unknown-count switch exposed (src/Surfaces.h:69,260,1411)
+template <class S>
+struct HarnessSurfaces : public Surfaces<S> {
+  using Surfaces<S>::Surfaces;
+  void set_vortex_only() { this->source_str_is_unknown = false; }
+};
+
+template <class S>
+Points<S> make_points(int64_t n, const S* x, const S* y, const S* str, const S* rad, bool inert_pts) {
+  std::vector<S> xs(2 * n);
+  for (int64_t i = 0; i < n; ++i) { xs[2 * i] = x[i]; xs[2 * i + 1] = y[i]; }
+  std::vector<S> vals;
+  if (!inert_pts) vals.assign(str, str + n);
+  ElementPacket<S> pk(xs, std::vector<Int>(), vals, (size_t)n, 0);
+  Points<S> p(pk, inert_pts ? inert : active, lagrangian, nullptr, 0.01f);
+  if (!inert_pts) {
+    Vector<S>& r = p.get_rad();
+    for (int64_t i = 0; i < n; ++i) r[i] = rad[i];
+  }
+  return p;
+}
+
+// Panels built from float node coordinates (the reference Surfaces ctor takes ElementPacket<float>)
+template <class S>
+HarnessSurfaces<S> make_surfaces(int64_t nnodes, const float* x, const float* y, int64_t np,
+                                 const uint32_t* idx, elem_t e, move_t m,
+                                 std::shared_ptr<Body> body) {
+  std::vector<float> xs(2 * nnodes);
+  for (int64_t i = 0; i < nnodes; ++i) { xs[2 * i] = x[i]; xs[2 * i + 1] = y[i]; }
+  std::vector<Int> id(idx, idx + 2 * np);
+  std::vector<float> vals(np, 0.0f);
+  ElementPacket<float> pk(xs, id, vals, (size_t)np, 1);
+  return HarnessSurfaces<S>(pk, e, m, body);
+}
+
+template <class S>
+void copy_out(const Vector<S>& v, S* out) { std::memcpy(out, v.data(), v.size() * sizeof(S)); }
+
+template <class S, class A>
+int pts_on_pts(int64_t ns, const S* sx, const S* sy, const S* ss, const S* sr,
+               int64_t nt, const S* tx, const S* ty, const S* tr, int restype,
+               S* tu, S* tv, S* tw, S* te) {
+  Points<S> src = make_points<S>(ns, sx, sy, ss, sr, false);
+  const bool targ_inert = (tr == nullptr);
+  std::vector<S> dummy(nt, S(0));
+  Points<S> targ = make_points<S>(nt, tx, ty, targ_inert ? nullptr : dummy.data(), tr, targ_inert);
+  targ.zero_vels();
+  if (restype == velandvort) (void)targ.get_vort();
+  if (restype == velandshear) (void)targ.get_shear();
+  points_affect_points<S, A>(src, targ, ResultsType((results_t)restype), ExecEnv(true, direct, cpu_x86));
+  copy_out<S>(targ.get_vel()[0], tu);
+  copy_out<S>(targ.get_vel()[1], tv);
+  if (restype == velandvort && tw) copy_out<S>(targ.get_vort(), tw);
+  if (restype == velandshear && te) copy_out<S>(targ.get_shear(), te);
+  return 0;
+}
+
+template <class S, class A>
+int pan_on_pts(int64_t nnodes, const float* nx, const float* ny, int64_t np, const uint32_t* idx,
+               const S* vs, const S* ss, int64_t nt, const S* tx, const S* ty, S* tu, S* tv) {
+  HarnessSurfaces<S> src = make_surfaces<S>(nnodes, nx, ny, np, idx, active, fixed, nullptr);
+  for (int64_t j = 0; j < np; ++j) {
+    src.get_str()[j] = vs[j];
+    src.get_src_str()[j] = ss ? ss[j] : S(0);
+  }
+  Points<S> targ = make_points<S>(nt, tx, ty, nullptr, nullptr, true);
+  targ.zero_vels();
+  panels_affect_points<S, A>(src, targ, ResultsType(velonly), ExecEnv(true, direct, cpu_x86));
+  copy_out<S>(targ.get_vel()[0], tu);
+  copy_out<S>(targ.get_vel()[1], tv);
+  return 0;
+}
+
+template <class S, class A>
+int pts_on_pan(int64_t ns, const S* sx, const S* sy, const S* ss, const S* sr,
+               int64_t nnodes, const float* nx, const float* ny, int64_t np, const uint32_t* idx,
+               S* pu, S* pv) {
+  Points<S> src = make_points<S>(ns, sx, sy, ss, sr, false);
+  HarnessSurfaces<S> targ = make_surfaces<S>(nnodes, nx, ny, np, idx, reactive, fixed, nullptr);
+  targ.zero_vels();
+  points_affect_panels<S, A>(src, targ, ResultsType(velonly), ExecEnv(true, direct, cpu_x86));
+  copy_out<S>(targ.get_vel()[0], pu);
+  copy_out<S>(targ.get_vel()[1], pv);
+  return 0;
+}
+
+// Full BEM block for one reactive body acting on itself, exactly as solve_bem builds it
+// (src/BEMHelper.h:202-216): unit-rotation velocities for the augmented column first, then
+// panels_on_panels_coeff. body_kind: 0 = no Body (never augmented), 1 = "ground" body,
+// 2 = a named moving body. Also returns the panel bases/areas, the body volume and the
+// unit-rotation panel velocities so the restatement can be fed the same inputs.
+template <class S, class A>
+int bem_coeffs(int64_t nnodes, const float* nx, const float* ny, int64_t np, const uint32_t* idx,
+               int nunk, int body_kind, S* coeffs, int64_t* nrows_out, int64_t* ncols_out,
+               S* tang_x, S* tang_y, S* area, S* vol_out, S* urot_u, S* urot_v) {
+  std::shared_ptr<Body> body;
+  if (body_kind == 1) { body = std::make_shared<Body>(); body->set_name("ground"); }
+  if (body_kind == 2) { body = std::make_shared<Body>(); body->set_name("rotor"); }
+  HarnessSurfaces<S> surf = make_surfaces<S>(nnodes, nx, ny, np, idx, reactive,
+                                             body ? bodybound : fixed, body);
+  if (nunk == 1) surf.set_vortex_only();
+  surf.transform(0.0);
+  if (surf.is_augmented()) {
+    surf.zero_vels();
+    surf.zero_strengths();
+    surf.add_unit_rot_strengths();
+    // panels_affect_panels (src/Influence.h:774-795) only compiles at S=float, because it builds
+    // Points<S> from an ElementPacket<float>; the same steps are spelled out here with the
+    // packet converted to S, calling the reference's own represent_as_particles and
+    // panels_affect_points.
+    ElementPacket<float> fp = surf.represent_as_particles(0.0001, -1.0);
+    ElementPacket<S> sp(std::vector<S>(fp.x.begin(), fp.x.end()), fp.idx,
+                        std::vector<S>(fp.val.begin(), fp.val.end()), fp.nelem, fp.ndim);
+    Points<S> temppts(sp, active, lagrangian, nullptr, 0.0001f);
+    temppts.zero_vels();
+    const Surfaces<S>& csrc = surf;
+    panels_affect_points<S, A>(csrc, temppts, ResultsType(velonly), ExecEnv(true, direct, cpu_x86));
+    for (size_t d = 0; d < Dimensions; ++d)
+      for (int64_t i = 0; i < np; ++i) surf.get_vel()[d][i] += temppts.get_vel()[d][i];
+    surf.zero_strengths();
+    surf.finalize_vels(std::array<double, Dimensions>({0.0, 0.0}));
+  }
+  for (int64_t i = 0; i < np; ++i) {
+    urot_u[i] = surf.get_vel()[0][i];
+    urot_v[i] = surf.get_vel()[1][i];
+  }
+  const Surfaces<S>& csurf = surf;
+  Vector<S> c = panels_on_panels_coeff<S>(csurf, static_cast<Surfaces<S>&>(surf));
+  const int64_t nr = surf.get_num_rows();
+  *nrows_out = nr;
+  *ncols_out = nr;
+  if (coeffs) std::memcpy(coeffs, c.data(), c.size() * sizeof(S));
+  copy_out<S>(surf.get_tang()[0], tang_x);
+  copy_out<S>(surf.get_tang()[1], tang_y);
+  copy_out<S>(surf.get_area(), area);
+  *vol_out = surf.get_vol();
+  return (int64_t)c.size() == nr * nr ? 0 : -1;
+}
+
+// RHS from panel-centre velocities (src/RHS.h:54-115) for a reactive surface with given normal BCs
+template <class S>
+int rhs_panels(int64_t nnodes, const float* nx, const float* ny, int64_t np, const uint32_t* idx,
+               int nunk, const S* pu, const S* pv, const float* norm_bc, S* rhs) {
+  std::vector<float> xs(2 * nnodes);
+  for (int64_t i = 0; i < nnodes; ++i) { xs[2 * i] = nx[i]; xs[2 * i + 1] = ny[i]; }
+  std::vector<Int> id(idx, idx + 2 * np);
+  std::vector<float> vals(2 * np);
+  for (int64_t i = 0; i < np; ++i) { vals[2 * i] = 0.0f; vals[2 * i + 1] = norm_bc[i]; }
+  ElementPacket<float> pk(xs, id, vals, (size_t)np, 1);
+  HarnessSurfaces<S> surf(pk, reactive, fixed, nullptr);
+  if (nunk == 1) surf.set_vortex_only();
+  for (int64_t i = 0; i < np; ++i) { surf.get_vel()[0][i] = pu[i]; surf.get_vel()[1][i] = pv[i]; }
+  std::vector<S> r = vels_to_rhs_panels<S>(surf);
+  std::memcpy(rhs, r.data(), r.size() * sizeof(S));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_points_on_points_d(int64_t ns, const double* sx, const double* sy, const double* ss,
+                           const double* sr, int64_t nt, const double* tx, const double* ty,
+                           const double* tr, int restype, double* tu, double* tv, double* tw,
+                           double* te) {
+  return pts_on_pts<double, double>(ns, sx, sy, ss, sr, nt, tx, ty, tr, restype, tu, tv, tw, te);
+}
+
+int ref_points_on_points_f(int64_t ns, const float* sx, const float* sy, const float* ss,
+                           const float* sr, int64_t nt, const float* tx, const float* ty,
+                           const float* tr, int restype, float* tu, float* tv, float* tw,
+                           float* te) {
+  return pts_on_pts<float, double>(ns, sx, sy, ss, sr, nt, tx, ty, tr, restype, tu, tv, tw, te);
+}
+
+int ref_panels_on_points_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                           const uint32_t* idx, const double* vs, const double* ss, int64_t nt,
+                           const double* tx, const double* ty, double* tu, double* tv) {
+  return pan_on_pts<double, double>(nnodes, nx, ny, np, idx, vs, ss, nt, tx, ty, tu, tv);
+}
+
+int ref_panels_on_points_f(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                           const uint32_t* idx, const float* vs, const float* ss, int64_t nt,
+                           const float* tx, const float* ty, float* tu, float* tv) {
+  return pan_on_pts<float, double>(nnodes, nx, ny, np, idx, vs, ss, nt, tx, ty, tu, tv);
+}
+
+int ref_points_on_panels_d(int64_t ns, const double* sx, const double* sy, const double* ss,
+                           const double* sr, int64_t nnodes, const float* nx, const float* ny,
+                           int64_t np, const uint32_t* idx, double* pu, double* pv) {
+  return pts_on_pan<double, double>(ns, sx, sy, ss, sr, nnodes, nx, ny, np, idx, pu, pv);
+}
+
+int ref_points_on_panels_f(int64_t ns, const float* sx, const float* sy, const float* ss,
+                           const float* sr, int64_t nnodes, const float* nx, const float* ny,
+                           int64_t np, const uint32_t* idx, float* pu, float* pv) {
+  return pts_on_pan<float, double>(ns, sx, sy, ss, sr, nnodes, nx, ny, np, idx, pu, pv);
+}
+
+int ref_bem_coeffs_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                     const uint32_t* idx, int nunk, int body_kind, double* coeffs,
+                     int64_t* nrows, int64_t* ncols, double* tang_x, double* tang_y,
+                     double* area, double* vol, double* urot_u, double* urot_v) {
+  return bem_coeffs<double, double>(nnodes, nx, ny, np, idx, nunk, body_kind, coeffs, nrows,
+                                    ncols, tang_x, tang_y, area, vol, urot_u, urot_v);
+}
+
+int ref_bem_coeffs_f(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                     const uint32_t* idx, int nunk, int body_kind, float* coeffs,
+                     int64_t* nrows, int64_t* ncols, float* tang_x, float* tang_y,
+                     float* area, float* vol, float* urot_u, float* urot_v) {
+  return bem_coeffs<float, double>(nnodes, nx, ny, np, idx, nunk, body_kind, coeffs, nrows,
+                                   ncols, tang_x, tang_y, area, vol, urot_u, urot_v);
+}
+
+int ref_rhs_panels_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                     const uint32_t* idx, int nunk, const double* pu, const double* pv,
+                     const float* norm_bc, double* rhs) {
+  return rhs_panels<double>(nnodes, nx, ny, np, idx, nunk, pu, pv, norm_bc, rhs);
+}
+
+// finalize_vels (src/ElementBase.h:215-234) followed by one Ralston RK2 position update
+// (src/ElementBase.h:299-327 with weights 1/4, 3/4 from src/Convection.h:442)
+int ref_points_move2_d(int64_t n, double* x, double* y, const double* u1, const double* v1,
+                       const double* u2, const double* v2, double dt, double w1, double w2) {
+  std::vector<double> s(n, 1.0), r(n, 0.01);
+  Points<double> p0 = make_points<double>(n, x, y, s.data(), r.data(), false);
+  Points<double> p1 = make_points<double>(n, x, y, s.data(), r.data(), false);
+  for (int64_t i = 0; i < n; ++i) {
+    p0.get_vel()[0][i] = u1[i]; p0.get_vel()[1][i] = v1[i];
+    p1.get_vel()[0][i] = u2[i]; p1.get_vel()[1][i] = v2[i];
+  }
+  p0.move(0.0, dt, w1, p0, w2, p1);
+  copy_out<double>(p0.get_pos()[0], x);
+  copy_out<double>(p0.get_pos()[1], y);
+  return 0;
+}
+
+int ref_points_move3_d(int64_t n, double* x, double* y, const double* u1, const double* v1,
+                       const double* u2, const double* v2, const double* u3, const double* v3,
+                       double dt, double w1, double w2, double w3) {
+  std::vector<double> s(n, 1.0), r(n, 0.01);
+  Points<double> p0 = make_points<double>(n, x, y, s.data(), r.data(), false);
+  Points<double> p1 = make_points<double>(n, x, y, s.data(), r.data(), false);
+  Points<double> p2 = make_points<double>(n, x, y, s.data(), r.data(), false);
+  for (int64_t i = 0; i < n; ++i) {
+    p0.get_vel()[0][i] = u1[i]; p0.get_vel()[1][i] = v1[i];
+    p1.get_vel()[0][i] = u2[i]; p1.get_vel()[1][i] = v2[i];
+    p2.get_vel()[0][i] = u3[i]; p2.get_vel()[1][i] = v3[i];
+  }
+  p0.move(0.0, dt, w1, p0, w2, p1, w3, p2);
+  copy_out<double>(p0.get_pos()[0], x);
+  copy_out<double>(p0.get_pos()[1], y);
+  return 0;
+}
+
+int ref_finalize_d(int64_t n, double* u, double* v, double fsx, double fsy) {
+  std::vector<double> x(n, 0.0), s(n, 1.0), r(n, 0.01);
+  Points<double> p = make_points<double>(n, x.data(), x.data(), s.data(), r.data(), false);
+  for (int64_t i = 0; i < n; ++i) { p.get_vel()[0][i] = u[i]; p.get_vel()[1][i] = v[i]; }
+  p.finalize_vels(std::array<double, Dimensions>({fsx, fsy}));
+  copy_out<double>(p.get_vel()[0], u);
+  copy_out<double>(p.get_vel()[1], v);
+  return 0;
+}
+
+}  // extern "C"
