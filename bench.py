@@ -1,0 +1,292 @@
+"""Headline benchmark: G pair-interactions/s of the direct Biot-Savart velocity evaluation.
+
+Workload (one step = one full velocity evaluation): 1,048,576 particles in 64 Lamb-Oseen
+blobs acting on themselves (particle targets, kernelu_0v_0b) plus on 65,536 tracers (point
+targets, kernelu_0v_0p), Gaussian core, fp32 split-precision pair arithmetic with fp64
+accumulation. Inputs are synthetic (seeded) and resident in HBM before timing starts.
+
+Also reported on the same line: the plain fp64 form's rate, the split form's error against
+the fp64 form (all targets) and against the CPU oracle (a target subsample), the
+adjoint/forward time ratio on the 250k-particle / 4,096-probe adjoint workload, the dominant
+kernel's roofline position, and the reference CPU path timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "G pair-interactions/s at N=1M (fp32-split vs fp64, err vs fp64); adjoint/fwd"
+PEAK_FP32 = 157.3e12   # MI355X dense FP32 vector, FLOP/s (includes packed FP32)
+PEAK_FP64 = 78.6e12    # MI355X FP64 vector, FLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the bounded reference-baseline sample")
+    ap.add_argument("--n", type=int, default=1 << 20, help="particles (default 2^20)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "layout (targets sharded, sources all-gathered) is described in DESIGN.md and not "
+              "run here.", file=sys.stderr)
+        return 2
+
+    import torch
+    import omega2d_amd as o2d
+    from omega2d_amd import device as dv
+    from omega2d_amd import workloads as wl
+    from omega2d_amd._cabi import (KT_ADJOINT, KT_NBODY_PLAIN, KT_NBODY_SPLIT, KT_SPLIT_PREP,
+                                   timing_enable, timing_read)
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    w = wl.c2_blobs(n=args.n)
+    n, nt = len(w["x"]), len(w["tx"])
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sx, sy, ss, sr = T(w["x"]), T(w["y"]), T(w["s"]), T(w["r"])
+    tx, ty = T(w["tx"]), T(w["ty"])
+    pairs = float(n) * n + float(nt) * n
+    flops_b = o2d.flops_per_pair("gauss", False)   # 19: particle targets
+    flops_p = o2d.flops_per_pair("gauss", True)    # 17: point targets
+    flops = float(n) * n * flops_b + float(nt) * n * flops_p + 2.0 * (n + nt)
+    log(f"workload {w['name']}: {n} particles + {nt} tracers, {pairs:.4e} pairs per step")
+
+    pu = torch.zeros(n, dtype=torch.float64, device=dev)
+    pv = torch.zeros_like(pu)
+    qu = torch.zeros(nt, dtype=torch.float64, device=dev)
+    qv = torch.zeros_like(qu)
+
+    def step(precision):
+        pu.zero_()
+        pv.zero_()
+        qu.zero_()
+        qv.zero_()
+        dv.points_on_points(sx, sy, ss, sr, sx, sy, sr, core="gauss", precision=precision,
+                            tu=pu, tv=pv)
+        dv.points_on_points(sx, sy, ss, sr, tx, ty, None, core="gauss", precision=precision,
+                            tu=qu, tv=qv)
+
+    # ---- split form: warmup, then exactly K timed steps
+    for i in range(args.warmup):
+        step("split")
+    torch.cuda.synchronize()
+    for c in (KT_NBODY_SPLIT, KT_SPLIT_PREP):
+        timing_read(c)
+    timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step("split")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing_enable(False)
+    k_ms, k_n = timing_read(KT_NBODY_SPLIT)
+    prep_ms, prep_n = timing_read(KT_SPLIT_PREP)
+    ms_split = (t1 - t0) * 1e3 / args.steps
+    value = pairs / (ms_split * 1e-3) / 1e9
+    log(f"split: {ms_split:.2f} ms/step, {value:.1f} G-int/s; main kernel {k_ms / max(k_n, 1):.3f} ms"
+        f" x{k_n}, prep {prep_ms / max(prep_n, 1):.3f} ms x{prep_n}")
+    su, sv_, squ, sqv = (a.cpu().numpy() for a in (pu, pv, qu, qv))
+
+    # ---- plain fp64 form
+    step("fp64")
+    torch.cuda.synchronize()
+    timing_read(KT_NBODY_PLAIN)
+    timing_enable(True)
+    t0 = time.perf_counter()
+    for i in range(args.fp64_steps):
+        step("fp64")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing_enable(False)
+    k64_ms, k64_n = timing_read(KT_NBODY_PLAIN)
+    ms_64 = (t1 - t0) * 1e3 / args.fp64_steps
+    value_64 = pairs / (ms_64 * 1e-3) / 1e9
+    fu, fv, fqu, fqv = (a.cpu().numpy() for a in (pu, pv, qu, qv))
+    err_fp64 = float(np.sqrt((np.sum((su - fu) ** 2) + np.sum((sv_ - fv) ** 2)
+                              + np.sum((squ - fqu) ** 2) + np.sum((sqv - fqv) ** 2))
+                             / (np.sum(fu ** 2) + np.sum(fv ** 2) + np.sum(fqu ** 2)
+                                + np.sum(fqv ** 2))))
+    log(f"fp64: {ms_64:.2f} ms/step, {value_64:.1f} G-int/s; split/fp64 = {ms_64 / ms_split:.2f}x;"
+        f" split err vs fp64 {err_fp64:.3e}")
+
+    # ---- split error against the CPU oracle on a target subsample
+    err_oracle = None
+    try:
+        import oracle as O
+        rng = np.random.default_rng(0)
+        sub = np.sort(rng.choice(n, 256, replace=False))
+        ou, ov = O.points_on_points("gauss", w["x"], w["y"], w["s"], w["r"], w["x"][sub],
+                                    w["y"][sub], w["r"][sub])
+        err_oracle = float(np.sqrt((np.sum((su[sub] - ou) ** 2) + np.sum((sv_[sub] - ov) ** 2))
+                                   / (np.sum(ou ** 2) + np.sum(ov ** 2))))
+        err64_oracle = float(np.sqrt((np.sum((fu[sub] - ou) ** 2) + np.sum((fv[sub] - ov) ** 2))
+                                     / (np.sum(ou ** 2) + np.sum(ov ** 2))))
+        log(f"oracle subsample (256 targets): split err {err_oracle:.3e}, fp64 err {err64_oracle:.3e}")
+    except Exception as e:  # the checker is optional on a box without the oracle build
+        log(f"oracle check skipped: {e}")
+        err64_oracle = None
+
+    # ---- adjoint / forward on the adjoint workload (fp64 both)
+    a = wl.c4_adjoint()
+    ax, ay, as_, ar, atx, aty = (T(a[k]) for k in ("x", "y", "s", "r", "tx", "ty"))
+    au = torch.zeros(len(a["tx"]), dtype=torch.float64, device=dev)
+    av = torch.zeros_like(au)
+    grads = tuple(torch.zeros(len(a["x"]), dtype=torch.float64, device=dev) for _ in range(4))
+
+    def fwd():
+        au.zero_()
+        av.zero_()
+        dv.points_on_points(ax, ay, as_, ar, atx, aty, None, core="gauss", tu=au, tv=av)
+
+    def adj():
+        for g in grads:
+            g.zero_()
+        dv.points_on_points_adjoint(ax, ay, as_, ar, atx, aty, 2 * au, 2 * av, core="gauss",
+                                    out=grads)
+
+    fwd()
+    adj()
+    torch.cuda.synchronize()
+    timing_read(KT_NBODY_PLAIN)
+    timing_read(KT_ADJOINT)
+    timing_enable(True)
+    reps = 5
+    t0 = time.perf_counter()
+    for i in range(reps):
+        fwd()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(reps):
+        adj()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    timing_enable(False)
+    ms_fwd, ms_adj = (t1 - t0) * 1e3 / reps, (t2 - t1) * 1e3 / reps
+    kf_ms, kf_n = timing_read(KT_NBODY_PLAIN)
+    ka_ms, ka_n = timing_read(KT_ADJOINT)
+    log(f"adjoint workload: fwd {ms_fwd:.3f} ms, adjoint {ms_adj:.3f} ms, ratio {ms_adj / ms_fwd:.2f}")
+
+    # ---- roofline of the dominant kernel (split N-body, particle targets), from HIP events
+    kernel_ms = k_ms / max(k_n, 1)
+    flops_per_launch = flops / 2.0   # two launches per step: particles (dominant) and tracers
+    flops_main = float(n) * n * flops_b + 2.0 * n
+    # the two launches have different sizes; apportion the measured per-step kernel time by pairs
+    share = (float(n) * n) / pairs
+    main_ms = kernel_ms * 2.0 * share
+    achieved = flops_main / (main_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32 / 1e12,
+                "unit": "TFLOP/s", "frac": round(achieved * 1e12 / PEAK_FP32, 4), "traffic": None,
+                "kernel": "nbody32_kernel<Gaussian, particle targets>",
+                "flops_per_pair": flops_b, "kernel_ms": round(main_ms, 4)}
+    del flops_per_launch
+
+    # ---- CPU baseline: the reference's own OpenMP path (S=float, A=double), bounded sample
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "G pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_split, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "1,048,576 particles in 64 Lamb-Oseen blobs (self, particle "
+                               "targets) + 65,536 tracers, Gaussian core sigma=7.8125e-5, one "
+                               "velocity evaluation, fp32 split pair math + fp64 accumulation",
+                   "particles": n, "tracers": nt, "pairs_per_step": pairs, "core": "gaussian",
+                   "parallelism": "single GPU"},
+        "fp64_plain": {"value": round(value_64, 2), "ms_per_step": round(ms_64, 3),
+                       "kernel_ms": round(k64_ms / max(k64_n, 1) * 2 * share, 4),
+                       "frac_fp64_peak": round(flops_main / (k64_ms / max(k64_n, 1) * 2 * share
+                                                             * 1e-3) / PEAK_FP64, 4)},
+        "split_over_fp64": round(ms_64 / ms_split, 3),
+        "err_vs_fp64": err_fp64,
+        "err_vs_oracle_subsample": err_oracle,
+        "fp64_err_vs_oracle_subsample": err64_oracle,
+        "adjoint_over_fwd": round(ms_adj / ms_fwd, 3),
+        "adjoint_workload": {"particles": len(a["x"]), "probes": len(a["tx"]),
+                             "fwd_ms": round(ms_fwd, 3), "adjoint_ms": round(ms_adj, 3),
+                             "fwd_kernel_ms": round(kf_ms / max(kf_n, 1), 4),
+                             "adjoint_kernel_ms": round(ka_ms / max(ka_n, 1), 4)},
+        "split_prep_ms": round(prep_ms / max(prep_n, 1), 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(w, seconds):
+    """The reference's points_affect_points<float,double> (OpenMP, compiled from its sources by
+    oracle/Makefile) on a bounded target sample of the same workload, scaled to pairs/s."""
+    try:
+        import oracle as O
+    except Exception as e:
+        log(f"cpu baseline skipped: {e}")
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    n = len(w["x"])
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    sx, sy, ss, sr = f32(w["x"]), f32(w["y"]), f32(w["s"]), f32(w["r"])
+    if O.have_ref():
+        kind, fn = "reference", lambda m: O.ref_points_on_points(
+            sx, sy, ss, sr, sx[:m], sy[:m], sr[:m], "vel", dtype=np.float32)
+        label = ("reference points_affect_points<float,double>, its compiled-in Vatistas n=2 "
+                 "core (the Gaussian core needs a source edit), particle targets")
+    else:
+        kind, fn = "port", lambda m: O.points_on_points(
+            "gauss", w["x"], w["y"], w["s"], w["r"], w["x"][:m], w["y"][:m], w["r"][:m])
+        label = "fp64 C restatement, Gaussian core, particle targets"
+    m = max(threads * 4, 64)
+    t0 = time.perf_counter()
+    fn(m)
+    dt = time.perf_counter() - t0
+    rate = m * float(n) / dt
+    m2 = int(min(n, max(m, seconds * rate / n)))
+    t0 = time.perf_counter()
+    fn(m2)
+    dt = time.perf_counter() - t0
+    rate = m2 * float(n) / dt
+    log(f"cpu baseline ({kind}): {m2} targets x {n} sources in {dt:.2f} s = {rate / 1e9:.3f} "
+        f"G-int/s on {threads} threads")
+    return {"value": round(rate / 1e9, 4), "unit": "G pair-interactions/s", "cores": threads,
+            "kind": kind, "sample": f"{m2} of {n} particle targets x {n} sources, {dt:.1f} s; "
+                                    f"{label}; OpenMP x86 env (the Vc env cannot be built offline)"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

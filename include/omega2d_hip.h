@@ -1,0 +1,218 @@
+/*
+ * omega2d_hip.h - C ABI of the MI355X (gfx950) direct Biot-Savart library.
+ *
+ * Plain pointers and sizes only. Every entry point returns 0 on success or a negative
+ * O2D_ERR_* code; o2d_hip_last_error_string() describes the last failure on the calling thread.
+ *
+ * Conventions kept from the reference (src/Influence.h, src/ElementBase.h:200-234):
+ *  - velocity sums are UNSCALED (no 1/(2 pi), no freestream) and ACCUMULATED (+=) into
+ *    caller-zeroed outputs; the caller finalizes with u = fs + u/(2 pi);
+ *  - sources are const; the self pair (d = 0) contributes exactly 0;
+ *  - one caller thread per device (the reference is single-caller).
+ *
+ * Two families of entry points:
+ *  - o2d_hip_*_d / _f: host arrays, synchronous (results are in the host arrays on return).
+ *    _d takes double storage, _f float storage (the reference application's STORE=float);
+ *    both compute in the precision env->precision selects. Device mirrors are kept by the
+ *    library per device and reused across calls.
+ *  - o2d_hip_*_dev_d: device pointers, enqueued on env->stream (asynchronous).
+ */
+#ifndef OMEGA2D_HIP_H
+#define OMEGA2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O2D_HIP_ABI_VERSION 1
+
+/* status codes */
+#define O2D_OK 0
+#define O2D_ERR_ARG (-1)       /* a required pointer is NULL */
+#define O2D_ERR_SIZE (-2)      /* negative/too large count, or a panel index out of range */
+#define O2D_ERR_CORE (-3)      /* unknown core function */
+#define O2D_ERR_RESULTS (-4)   /* unsupported result type or option */
+#define O2D_ERR_PRECISION (-5) /* unknown precision form */
+#define O2D_ERR_HIP (-6)       /* HIP runtime or library failure */
+#define O2D_ERR_STATE (-7)     /* e.g. solve before factor */
+
+/* regularised velocity cores, in the order of src/CoreFunc.h:35-39 */
+#define O2D_CORE_RM 1       /* Rosenhead-Moore            src/CoreFunc.h:42-82  */
+#define O2D_CORE_GAUSSIAN 2 /* exponential (Gaussian)     src/CoreFunc.h:85-181 */
+#define O2D_CORE_WL 3       /* Winckelmans-Leonard        src/CoreFunc.h:184-230 */
+#define O2D_CORE_V2 4       /* Vatistas n=2 (ref default) src/CoreFunc.h:233-275 */
+#define O2D_CORE_V3 5       /* Vatistas n=3               src/CoreFunc.h:278-322 */
+
+/* result sets: the values of results_t, src/ResultsType.h:26-32 */
+#define O2D_VELONLY 1
+#define O2D_VELANDVORT 4
+#define O2D_VELANDSHEAR 5
+
+/* precision forms */
+#define O2D_PLAIN_FP64 1 /* fp64 pair arithmetic and accumulation */
+#define O2D_SPLIT_FP32 2 /* fp32 pair arithmetic on coordinates rebased to each source tile's
+                            fp64 centroid, fp64 accumulation per target (velocity only) */
+
+/* execution environment: the gpu_hip counterpart of ExecEnv (src/ExecEnv.h:45-108) plus the
+ * run-time choices the reference makes at compile time */
+typedef struct o2d_env {
+  int32_t core;      /* O2D_CORE_* */
+  int32_t precision; /* O2D_PLAIN_FP64 | O2D_SPLIT_FP32 */
+  int32_t results;   /* O2D_VELONLY | O2D_VELANDVORT | O2D_VELANDSHEAR */
+  int32_t device;    /* HIP device ordinal */
+  void* stream;      /* hipStream_t. Device-pointer entry points enqueue on exactly this stream
+                        (NULL = the device's null stream); the synchronous host-array entry
+                        points use it when non-NULL, else a stream of the library's own */
+} o2d_env;
+
+/* ---- runtime -------------------------------------------------------------------------- */
+
+/* Select the device, create the library stream and buffer pool for it. Optional: every entry
+ * point initialises its device on first use. */
+int o2d_hip_init(int32_t device);
+/* Release every device's buffers, LU factors and streams. */
+int o2d_hip_release(void);
+const char* o2d_hip_last_error_string(void);
+int32_t o2d_hip_abi_version(void);
+/* Algorithmic flops per pair interaction in the reference's own convention
+ * (src/Kernels.h:81,95,312,362, src/CoreFunc.h); panel = 0 for point sources, 1 for vortex
+ * panels (kernelu_1v_0p), 2 for vortex+source panels (kernelu_1vs_0p). */
+int32_t o2d_hip_flops_per_pair(int32_t core, int32_t point_targets, int32_t panel);
+
+/* ---- kernel timing ------------------------------------------------------------------------- */
+
+/* Kernel classes timed by o2d_hip_timing_*: the main kernel of each entry point. */
+#define O2D_KT_NBODY_SPLIT 1   /* split fp32 particles-on-points kernel */
+#define O2D_KT_NBODY_PLAIN 2   /* plain fp64 particles-on-points kernel */
+#define O2D_KT_PANELS 3        /* panels on points */
+#define O2D_KT_PANEL_RHS 4     /* points on panels */
+#define O2D_KT_BEM_ASSEMBLE 5  /* influence-matrix assembly */
+#define O2D_KT_ADJOINT 6       /* adjoint kernel */
+#define O2D_KT_SPLIT_PREP 7    /* Morton order + rebasing of the split form */
+#define O2D_KT_COUNT 8
+
+/* enable = 1: from now on record a HIP event pair on the launch stream around every timed
+ * kernel launch (results are unaffected); enable = 0: stop recording. */
+int o2d_hip_timing_enable(int32_t enable);
+/* Total milliseconds and number of launches recorded for a kernel class since the last read
+ * (waits for the recorded events, then clears the class). */
+int o2d_hip_timing_read(int32_t kernel_class, double* total_ms, int64_t* launches);
+
+/* ---- particles on points: points_affect_points (src/Influence.h:56-416) ------------------ */
+
+/* Sources: ns particles (sx, sy, strength sstr, core radius srad).
+ * Targets: nt points (tx, ty); trad = NULL means point targets (inert: kernelu_0v_0p),
+ * otherwise particle targets with core radii (kernelu_0v_0b).
+ * tu, tv accumulate the unscaled velocity; tw (vorticity, results = VELANDVORT) and te (shear
+ * rate, results = VELANDSHEAR) accumulate likewise and may be NULL otherwise.
+ * Replaces the internal CPU loops and the external_vel_solver_f_/_tr_f_/_d_ C ABI
+ * (src/Influence.h:31-38, 74-98) for env->precision's form. */
+int o2d_hip_points_on_points_d(const o2d_env* env, int64_t ns, const double* sx, const double* sy,
+                               const double* sstr, const double* srad, int64_t nt,
+                               const double* tx, const double* ty, const double* trad,
+                               double* tu, double* tv, double* tw, double* te);
+int o2d_hip_points_on_points_f(const o2d_env* env, int64_t ns, const float* sx, const float* sy,
+                               const float* sstr, const float* srad, int64_t nt, const float* tx,
+                               const float* ty, const float* trad, float* tu, float* tv,
+                               float* tw, float* te);
+int o2d_hip_points_on_points_dev_d(const o2d_env* env, int64_t ns, const double* sx,
+                                   const double* sy, const double* sstr, const double* srad,
+                                   int64_t nt, const double* tx, const double* ty,
+                                   const double* trad, double* tu, double* tv, double* tw,
+                                   double* te);
+
+/* ---- panels on points: panels_affect_points (src/Influence.h:422-614) -------------------- */
+
+/* Panels: nnodes node coordinates (nx, ny), np panels whose endpoints are nodes
+ * idx[2j], idx[2j+1]; vortex sheet strength vstr[j], source strength sstr[j] per unit length
+ * (sstr = NULL: vortex-only panels, kernelu_1v_0p; else kernelu_1vs_0p). Unscaled, accumulated.
+ * Velocity only (env->results must be O2D_VELONLY); computed in fp64 whatever env->precision. */
+int o2d_hip_panels_on_points_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                               const double* ny, int64_t np, const uint32_t* idx,
+                               const double* vstr, const double* sstr, int64_t nt,
+                               const double* tx, const double* ty, double* tu, double* tv);
+int o2d_hip_panels_on_points_dev_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                                   const double* ny, int64_t np, const uint32_t* idx,
+                                   const double* vstr, const double* sstr, int64_t nt,
+                                   const double* tx, const double* ty, double* tu, double* tv);
+
+/* ---- particles on panels: points_affect_panels (src/Influence.h:633-771), the BEM RHS ----- */
+
+/* Panel-averaged velocity of singular point vortices (radii are ignored, as in the reference):
+ * pu[i] -= (1/L_i) sum_j kernelu_1v_0p(panel i, sstr[j], particle j). Unscaled, accumulated. */
+int o2d_hip_points_on_panels_d(const o2d_env* env, int64_t ns, const double* sx,
+                               const double* sy, const double* sstr, int64_t nnodes,
+                               const double* nx, const double* ny, int64_t np,
+                               const uint32_t* idx, double* pu, double* pv);
+int o2d_hip_points_on_panels_dev_d(const o2d_env* env, int64_t ns, const double* sx,
+                                   const double* sy, const double* sstr, int64_t nnodes,
+                                   const double* nx, const double* ny, int64_t np,
+                                   const uint32_t* idx, double* pu, double* pv);
+
+/* ---- BEM: panels_on_panels_coeff (src/Coefficients.h:155-573) + solve (src/BEM.h:116-234) */
+
+/* Self-influence block of one body, fp64, written column-major into A (n x n, leading
+ * dimension lda >= n) with n = np*nunk + (augmented ? 1 : 0). nunk = 1: vortex unknowns only;
+ * nunk = 2: interleaved [vortex, source] unknowns per panel. When augmented, urot_u/urot_v are
+ * the finalized panel-centre velocities induced by the body's unit-rotation strengths
+ * (src/BEMHelper.h:203-210) and vol the body's signed area (src/Surfaces.h:623-677). */
+int o2d_hip_bem_assemble_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                           const double* ny, int64_t np, const uint32_t* idx, int32_t nunk,
+                           int32_t augmented, const double* urot_u, const double* urot_v,
+                           double vol, double* A, int64_t lda);
+int o2d_hip_bem_assemble_dev_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                               const double* ny, int64_t np, const uint32_t* idx, int32_t nunk,
+                               int32_t augmented, const double* urot_u, const double* urot_v,
+                               double vol, double* A, int64_t lda);
+/* LU-factor the n x n column-major fp64 matrix on the device and keep the factors for
+ * o2d_hip_bem_solve_d (the reference re-initialises its solver only when A changes,
+ * src/BEM.h:182-195). A is a host array (_d) or device array (_dev_d, not modified). */
+int o2d_hip_bem_factor_d(const o2d_env* env, int64_t n, const double* A, int64_t lda);
+int o2d_hip_bem_factor_dev_d(const o2d_env* env, int64_t n, const double* A, int64_t lda);
+/* x = A^-1 b with the stored factors (x may alias b). */
+int o2d_hip_bem_solve_d(const o2d_env* env, int64_t n, const double* b, double* x);
+int o2d_hip_bem_solve_dev_d(const o2d_env* env, int64_t n, const double* b, double* x);
+
+/* ---- RK position update: ElementBase::move (src/ElementBase.h:276-359) -------------------- */
+
+/* x += dt * sum_s w[s] * u_s (nstage = 1, 2 or 3), fp64. u and v are arrays of nstage
+ * pointers to finalized stage velocities. Host arrays (_d) or device arrays (_dev_d). */
+int o2d_hip_rk_move_d(const o2d_env* env, int64_t n, double dt, int32_t nstage, const double* w,
+                      const double* const* u, const double* const* v, double* x, double* y);
+int o2d_hip_rk_move_dev_d(const o2d_env* env, int64_t n, double dt, int32_t nstage,
+                          const double* w, const double* const* u, const double* const* v,
+                          double* x, double* y);
+
+/* ---- adjoint (no reference counterpart) -------------------------------------------------- */
+
+/* Vector-Jacobian product of the unscaled point-target velocity sum
+ *   tu_k = -sum_j sstr_j K(d_kj^2; srad_j) dy_kj,  tv_k = sum_j sstr_j K dx_kj
+ * given gu = dJ/dtu, gv = dJ/dtv at the nt point targets: accumulates dJ/dsx, dJ/dsy,
+ * dJ/dsstr, dJ/dsrad into gx, gy, gs, gr. The interaction matrix is recomputed, never stored.
+ * fp64 (env->precision must be O2D_PLAIN_FP64). */
+int o2d_hip_points_on_points_adjoint_d(const o2d_env* env, int64_t ns, const double* sx,
+                                       const double* sy, const double* sstr, const double* srad,
+                                       int64_t nt, const double* tx, const double* ty,
+                                       const double* gu, const double* gv, double* gx,
+                                       double* gy, double* gs, double* gr);
+int o2d_hip_points_on_points_adjoint_dev_d(const o2d_env* env, int64_t ns, const double* sx,
+                                           const double* sy, const double* sstr,
+                                           const double* srad, int64_t nt, const double* tx,
+                                           const double* ty, const double* gu, const double* gv,
+                                           double* gx, double* gy, double* gs, double* gr);
+
+/* ---- index work of the split-precision form (exposed for testing) ------------------------ */
+
+/* 32-bit Morton keys of (x, y) on the square box [x0, x0 + 1/inv_extent)^2 (16 bits per axis,
+ * x in the even bits) and the stable key order of 0..n-1. Device arrays. */
+int o2d_hip_morton_order_dev_d(const o2d_env* env, int64_t n, const double* x, const double* y,
+                               double x0, double y0, double inv_extent, uint32_t* keys,
+                               uint32_t* order);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OMEGA2D_HIP_H */

@@ -1,0 +1,611 @@
+// nbody.hip - particles on points, the direct Biot-Savart sum (src/Influence.h:56-416).
+//
+// Plain form (fp64): every lane owns TPL targets and runs an unrolled loop over a 256-source
+// tile staged in LDS from the SoA inputs with coalesced loads; all lanes of a wave read the
+// same source (an LDS broadcast). Sums are fp64 per target.
+//
+// Split form (fp32 pair arithmetic): sources are put in Morton order once per call, cut into
+// groups of kGroup sources, and stored as fp32 offsets from their group's fp64 centroid.
+// Per (target, group) a lane forms its own fp64 offset to that centroid once and rounds it to
+// fp32, so every pair difference dx = (t - c) - (s - c) is an fp32 difference of two small
+// numbers. Each group's fp32 partial sums are folded into fp64 per-target accumulators.
+//
+// Both forms split the source range into chunks when there are too few target workgroups to
+// fill the GPU; chunk partials are then reduced in a fixed order by a second kernel (no float
+// atomics, so results are reproducible run to run).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "nbody.h"
+#include "o2d_math.h"
+#include "o2d_runtime.h"
+
+namespace o2d {
+
+namespace {
+
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+constexpr int kTile = 256;   // sources per LDS stage
+constexpr int kGroup = 64;   // sources sharing one centroid in the split form
+constexpr int kTpl64 = 2;    // targets per lane, plain form
+constexpr int kTpl32 = 2;    // targets per lane, split form
+
+struct alignas(16) Src64 {
+  double x, y, s, w;
+};
+
+// ---------------------------------------------------------------------------- plain fp64
+
+template <int CORE, int RES, bool TR, int TPL>
+__global__ __launch_bounds__(kBlock) void nbody64_kernel(
+    int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ ss, const double* __restrict__ sr, int64_t nt,
+    const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tr,
+    int64_t chunk, double* __restrict__ ou, double* __restrict__ ov, double* __restrict__ ow,
+    int direct) {
+  __shared__ Src64 tile[kTile];
+  const int tid = threadIdx.x;
+  const int64_t tbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
+
+  double px[TPL], py[TPL], pw[TPL], au[TPL], av[TPL], aw[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tbase + (int64_t)t * kBlock;
+    const int64_t ii = i < nt ? i : nt - 1;
+    px[t] = tx[ii];
+    py[t] = ty[ii];
+    pw[t] = TR ? rterm<CORE>(tr[ii]) : 0.0;
+    au[t] = av[t] = aw[t] = 0.0;
+  }
+
+  const int64_t jb = (int64_t)blockIdx.y * chunk;
+  const int64_t je = jb + chunk < ns ? jb + chunk : ns;
+  for (int64_t j0 = jb; j0 < je; j0 += kTile) {
+    const int64_t j = j0 + tid;
+    Src64 s;
+    if (j < je) {
+      s.x = sx[j];
+      s.y = sy[j];
+      s.s = ss[j];
+      s.w = rterm<CORE>(sr[j]);
+    } else {  // zero-strength padding with a positive radius term: contributes exactly 0
+      s.x = 0.0;
+      s.y = 0.0;
+      s.s = 0.0;
+      s.w = 1.0;
+    }
+    __syncthreads();
+    tile[tid] = s;
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kTile; ++k) {
+      const Src64 q = tile[k];
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        const double dx = px[t] - q.x;
+        const double dy = py[t] - q.y;
+        const double d2 = __builtin_fma(dx, dx, dy * dy);
+        const double w = TR ? q.w + pw[t] : q.w;
+        if constexpr (RES == O2D_VELONLY) {
+          const double g = q.s * core_k64<CORE>(d2, w);
+          au[t] = __builtin_fma(-g, dy, au[t]);
+          av[t] = __builtin_fma(g, dx, av[t]);
+        } else {
+          double kk, bbb;
+          core_kg64<CORE>(d2, w, kk, bbb);
+          kk *= q.s;
+          bbb *= q.s;
+          au[t] = __builtin_fma(-kk, dy, au[t]);
+          av[t] = __builtin_fma(kk, dx, av[t]);
+          if constexpr (RES == O2D_VELANDVORT) {
+            // (bbb dx^2 + K) - (-bbb dy^2 - K)   src/Kernels.h:176-178
+            aw[t] += __builtin_fma(bbb, d2, 2.0 * kk);
+          } else {
+            // sqrt(dudx^2 + asym^2/4)           src/Kernels.h:225-230
+            const double dudx = -bbb * dx * dy;
+            const double asym = bbb * (dx * dx - dy * dy);
+            aw[t] += sqrt(__builtin_fma(dudx, dudx, 0.25 * asym * asym));
+          }
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tbase + (int64_t)t * kBlock;
+    if (i >= nt) continue;
+    if (direct) {
+      ou[i] += au[t];
+      ov[i] += av[t];
+      if (RES != O2D_VELONLY) ow[i] += aw[t];
+    } else {
+      const int64_t o = (int64_t)blockIdx.y * nt + i;
+      ou[o] = au[t];
+      ov[o] = av[t];
+      if (RES != O2D_VELONLY) ow[o] = aw[t];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------- split fp32
+
+// sources prepared by split_prep: fp32 (x - c, y - c, strength, radius term) in Morton order,
+// padded to whole tiles; cent[g] is the fp64 centroid of group g
+template <int CORE, bool TR, int TPL, int UNROLL>
+__global__ __launch_bounds__(kBlock) void nbody32_kernel(
+    const float4* __restrict__ src, const double2* __restrict__ cent, int64_t ntiles,
+    int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty,
+    const double* __restrict__ tr, int64_t tiles_per_chunk, double* __restrict__ ou,
+    double* __restrict__ ov, int direct) {
+  constexpr int kGroups = kTile / kGroup;
+  __shared__ float4 tile[kTile];
+  __shared__ double2 tcent[kGroups];
+  const int tid = threadIdx.x;
+  const int64_t tbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
+
+  double px[TPL], py[TPL], au[TPL], av[TPL];
+  float pw[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tbase + (int64_t)t * kBlock;
+    const int64_t ii = i < nt ? i : nt - 1;
+    px[t] = tx[ii];
+    py[t] = ty[ii];
+    pw[t] = TR ? (float)rterm<CORE>(tr[ii]) : 0.0f;
+    au[t] = av[t] = 0.0;
+  }
+
+  const int64_t tb = (int64_t)blockIdx.y * tiles_per_chunk;
+  const int64_t te = tb + tiles_per_chunk < ntiles ? tb + tiles_per_chunk : ntiles;
+  for (int64_t it = tb; it < te; ++it) {
+    const float4 s = src[it * kTile + tid];
+    double2 c;
+    if (tid < kGroups) c = cent[it * kGroups + tid];
+    __syncthreads();
+    tile[tid] = s;
+    if (tid < kGroups) tcent[tid] = c;
+    __syncthreads();
+#pragma unroll 1
+    for (int g = 0; g < kGroups; ++g) {
+      const double2 cg = tcent[g];
+      float ex[TPL], ey[TPL], fu[TPL], fv[TPL];
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        ex[t] = (float)(px[t] - cg.x);
+        ey[t] = (float)(py[t] - cg.y);
+        fu[t] = 0.0f;
+        fv[t] = 0.0f;
+      }
+#pragma unroll UNROLL
+      for (int k = 0; k < kGroup; ++k) {
+        const float4 q = tile[g * kGroup + k];
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          const float dx = ex[t] - q.x;
+          const float dy = ey[t] - q.y;
+          const float w = TR ? q.w + pw[t] : q.w;
+          const float gk = q.z * core_k32<CORE>(dx, dy, w);
+          fu[t] = __builtin_fmaf(-gk, dy, fu[t]);
+          fv[t] = __builtin_fmaf(gk, dx, fv[t]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        au[t] += (double)fu[t];
+        av[t] += (double)fv[t];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int64_t i = tbase + (int64_t)t * kBlock;
+    if (i >= nt) continue;
+    if (direct) {
+      ou[i] += au[t];
+      ov[i] += av[t];
+    } else {
+      const int64_t o = (int64_t)blockIdx.y * nt + i;
+      ou[o] = au[t];
+      ov[o] = av[t];
+    }
+  }
+}
+
+// out[i] += sum over chunks c (in order) of part[c*nt + i]
+__global__ __launch_bounds__(kBlock) void reduce_chunks_kernel(int64_t nchunk, int64_t nt,
+                                                               const double* __restrict__ pu,
+                                                               const double* __restrict__ pv,
+                                                               const double* __restrict__ pw,
+                                                               double* __restrict__ ou,
+                                                               double* __restrict__ ov,
+                                                               double* __restrict__ ow) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nt) return;
+  double su = 0.0, sv = 0.0, sw = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    su += pu[c * nt + i];
+    sv += pv[c * nt + i];
+    if (pw) sw += pw[c * nt + i];
+  }
+  ou[i] += su;
+  ov[i] += sv;
+  if (pw) ow[i] += sw;
+}
+
+// ---------------------------------------------------------------------------- Morton order
+
+struct BBox {
+  double x0, y0, inv_extent, pad;
+};
+
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+__device__ __forceinline__ double warp_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double warp_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// per-block min/max of x and y -> part[4*block + {0,1,2,3}] = xmin, ymin, xmax, ymax
+__global__ __launch_bounds__(kBlock) void bbox_partial_kernel(int64_t n, const double* __restrict__ x,
+                                                              const double* __restrict__ y,
+                                                              double* __restrict__ part) {
+  __shared__ double red[4][kBlock / 64];
+  double xmn = INFINITY, ymn = INFINITY, xmx = -INFINITY, ymx = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    xmn = fmin(xmn, x[i]);
+    xmx = fmax(xmx, x[i]);
+    ymn = fmin(ymn, y[i]);
+    ymx = fmax(ymx, y[i]);
+  }
+  xmn = warp_min(xmn);
+  ymn = warp_min(ymn);
+  xmx = warp_max(xmx);
+  ymx = warp_max(ymx);
+  const int wv = threadIdx.x / 64, ln = threadIdx.x % 64;
+  if (ln == 0) {
+    red[0][wv] = xmn;
+    red[1][wv] = ymn;
+    red[2][wv] = xmx;
+    red[3][wv] = ymx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) {
+      xmn = fmin(xmn, red[0][w]);
+      ymn = fmin(ymn, red[1][w]);
+      xmx = fmax(xmx, red[2][w]);
+      ymx = fmax(ymx, red[3][w]);
+    }
+    part[4 * blockIdx.x + 0] = xmn;
+    part[4 * blockIdx.x + 1] = ymn;
+    part[4 * blockIdx.x + 2] = xmx;
+    part[4 * blockIdx.x + 3] = ymx;
+  }
+}
+
+// one workgroup of kBlock threads folds the per-block partials (min/max are exact, so the
+// order does not matter)
+__global__ __launch_bounds__(kBlock) void bbox_final_kernel(int nblk, const double* __restrict__ part,
+                                                            BBox* __restrict__ out) {
+  __shared__ double red[4][kBlock / 64];
+  double xmn = INFINITY, ymn = INFINITY, xmx = -INFINITY, ymx = -INFINITY;
+  for (int b = threadIdx.x; b < nblk; b += kBlock) {
+    xmn = fmin(xmn, part[4 * b + 0]);
+    ymn = fmin(ymn, part[4 * b + 1]);
+    xmx = fmax(xmx, part[4 * b + 2]);
+    ymx = fmax(ymx, part[4 * b + 3]);
+  }
+  xmn = warp_min(xmn);
+  ymn = warp_min(ymn);
+  xmx = warp_max(xmx);
+  ymx = warp_max(ymx);
+  const int wv = threadIdx.x / 64, ln = threadIdx.x % 64;
+  if (ln == 0) {
+    red[0][wv] = xmn;
+    red[1][wv] = ymn;
+    red[2][wv] = xmx;
+    red[3][wv] = ymx;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < kBlock / 64; ++w) {
+    xmn = fmin(xmn, red[0][w]);
+    ymn = fmin(ymn, red[1][w]);
+    xmx = fmax(xmx, red[2][w]);
+    ymx = fmax(ymx, red[3][w]);
+  }
+  const double ext = fmax(xmx - xmn, ymx - ymn);
+  BBox b;
+  b.x0 = xmn;
+  b.y0 = ymn;
+  b.inv_extent = ext > 0.0 ? 1.0 / ext : 0.0;
+  b.pad = 0.0;
+  *out = b;
+}
+
+// q = clamp(floor(((x - x0) * inv) * 65536), 0, 65535) with unfused IEEE operations
+__device__ __forceinline__ uint32_t quant16(double v, double v0, double inv) {
+#pragma clang fp contract(off)
+  double q = floor(__dmul_rn(__dmul_rn(__dsub_rn(v, v0), inv), 65536.0));
+  q = q < 0.0 ? 0.0 : (q > 65535.0 ? 65535.0 : q);
+  return (uint32_t)q;
+}
+
+__global__ __launch_bounds__(kBlock) void morton_kernel(int64_t n, const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const BBox* __restrict__ bb,
+                                                        double x0, double y0, double inv,
+                                                        uint32_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ order) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (bb) {
+    x0 = bb->x0;
+    y0 = bb->y0;
+    inv = bb->inv_extent;
+  }
+  keys[i] = spread16(quant16(x[i], x0, inv)) | (spread16(quant16(y[i], y0, inv)) << 1);
+  order[i] = (uint32_t)i;
+}
+
+// one thread per group: fp64 centroid of the group's real sources in sorted order
+__global__ __launch_bounds__(kBlock) void centroid_kernel(int64_t ns, int64_t ngroups,
+                                                          const uint32_t* __restrict__ order,
+                                                          const double* __restrict__ sx,
+                                                          const double* __restrict__ sy,
+                                                          double2* __restrict__ cent) {
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t b = g * kGroup;
+  const int64_t e = b + kGroup < ns ? b + kGroup : ns;
+  double cx = 0.0, cy = 0.0;
+  if (e > b) {
+    for (int64_t j = b; j < e; ++j) {
+      const uint32_t o = order[j];
+      cx += sx[o];
+      cy += sy[o];
+    }
+    const double inv = 1.0 / (double)(e - b);
+    cx *= inv;
+    cy *= inv;
+  }
+  cent[g] = make_double2(cx, cy);
+}
+
+template <int CORE>
+__global__ __launch_bounds__(kBlock) void rebase_kernel(int64_t ns, int64_t npad,
+                                                        const uint32_t* __restrict__ order,
+                                                        const double* __restrict__ sx,
+                                                        const double* __restrict__ sy,
+                                                        const double* __restrict__ ss,
+                                                        const double* __restrict__ sr,
+                                                        const double2* __restrict__ cent,
+                                                        float4* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= npad) return;
+  if (j < ns) {
+    const uint32_t o = order[j];
+    const double2 c = cent[j / kGroup];
+    out[j] = make_float4((float)(sx[o] - c.x), (float)(sy[o] - c.y), (float)ss[o],
+                         (float)rterm<CORE>(sr[o]));
+  } else {
+    out[j] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------- launch helpers
+
+struct ChunkPlan {
+  int64_t nchunk;
+  int64_t per_chunk;  // sources (plain) or tiles (split) per chunk
+};
+
+// Enough workgroups for ~8 per CU; never more chunks than tiles.
+ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles) {
+  const int64_t want = 8 * (int64_t)d->num_cu;
+  int64_t nchunk = target_wgs >= want ? 1 : (want + target_wgs - 1) / target_wgs;
+  if (nchunk > ntiles) nchunk = ntiles;
+  if (nchunk < 1) nchunk = 1;
+  const int64_t per = (ntiles + nchunk - 1) / nchunk;
+  nchunk = (ntiles + per - 1) / per;
+  return {nchunk, per};
+}
+
+template <int CORE, int RES, bool TR>
+void launch64(dim3 grid, hipStream_t s, int64_t ns, const double* sx, const double* sy,
+              const double* ss, const double* sr, int64_t nt, const double* tx, const double* ty,
+              const double* tr, int64_t chunk, double* ou, double* ov, double* ow, int direct) {
+  nbody64_kernel<CORE, RES, TR, kTpl64><<<grid, kBlock, 0, s>>>(ns, sx, sy, ss, sr, nt, tx, ty, tr,
+                                                               chunk, ou, ov, ow, direct);
+}
+
+template <int CORE>
+void dispatch64(int res, bool tr_present, dim3 grid, hipStream_t s, int64_t ns, const double* sx,
+                const double* sy, const double* ss, const double* sr, int64_t nt,
+                const double* tx, const double* ty, const double* tr, int64_t chunk, double* ou,
+                double* ov, double* ow, int direct) {
+#define O2D_L64(R, T) \
+  launch64<CORE, R, T>(grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct)
+  if (res == O2D_VELONLY) {
+    if (tr_present) O2D_L64(O2D_VELONLY, true);
+    else O2D_L64(O2D_VELONLY, false);
+  } else if (res == O2D_VELANDVORT) {
+    if (tr_present) O2D_L64(O2D_VELANDVORT, true);
+    else O2D_L64(O2D_VELANDVORT, false);
+  } else {
+    if (tr_present) O2D_L64(O2D_VELANDSHEAR, true);
+    else O2D_L64(O2D_VELANDSHEAR, false);
+  }
+#undef O2D_L64
+}
+
+template <int CORE>
+void dispatch32(bool tr_present, dim3 grid, hipStream_t s, const float4* src, const double2* cent,
+                int64_t ntiles, int64_t nt, const double* tx, const double* ty, const double* tr,
+                int64_t per, double* ou, double* ov, int direct) {
+  // the Gaussian's in-core branch makes a deep unroll expensive in registers
+  constexpr int kUnroll = CORE == O2D_CORE_GAUSSIAN ? 4 : 16;
+  if (tr_present)
+    nbody32_kernel<CORE, true, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
+        src, cent, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
+  else
+    nbody32_kernel<CORE, false, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
+        src, cent, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
+}
+
+template <int CORE>
+void launch_rebase(dim3 grid, hipStream_t s, int64_t ns, int64_t npad, const uint32_t* order,
+                   const double* sx, const double* sy, const double* ss, const double* sr,
+                   const double2* cent, float4* out) {
+  rebase_kernel<CORE><<<grid, kBlock, 0, s>>>(ns, npad, order, sx, sy, ss, sr, cent, out);
+}
+
+inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+}  // namespace
+
+// Morton keys and the stable key order; bbox == true derives the box from the data on device.
+int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const double* y,
+                 bool bbox, double x0, double y0, double inv, uint32_t* keys_out,
+                 uint32_t* order_out) {
+  if (n <= 0) return 0;
+  void *kA, *oA, *bbp = nullptr;
+  O2D_TRY(slot(d, kKeysA, n * sizeof(uint32_t), s, &kA));
+  O2D_TRY(slot(d, kOrderA, n * sizeof(uint32_t), s, &oA));
+  BBox* bb = nullptr;
+  if (bbox) {
+    const int nblk = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 1024);
+    O2D_TRY(slot(d, kBBox, sizeof(double) * (4 * nblk + 4), s, &bbp));
+    double* part = static_cast<double*>(bbp);
+    bb = reinterpret_cast<BBox*>(part + 4 * nblk);
+    bbox_partial_kernel<<<nblk, kBlock, 0, s>>>(n, x, y, part);
+    bbox_final_kernel<<<1, kBlock, 0, s>>>(nblk, part, bb);
+    O2D_LAUNCH_CHECK();
+  }
+  morton_kernel<<<grid1(n), kBlock, 0, s>>>(n, x, y, bb, x0, y0, inv, static_cast<uint32_t*>(kA),
+                                            static_cast<uint32_t*>(oA));
+  O2D_LAUNCH_CHECK();
+  size_t tbytes = 0;
+  O2D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, static_cast<uint32_t*>(kA),
+                                             keys_out, static_cast<uint32_t*>(oA), order_out,
+                                             (int)n, 0, 32, s));
+  void* tmp;
+  O2D_TRY(slot(d, kSortTemp, tbytes, s, &tmp));
+  O2D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, static_cast<uint32_t*>(kA), keys_out,
+                                             static_cast<uint32_t*>(oA), order_out, (int)n, 0, 32,
+                                             s));
+  return 0;
+}
+
+int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns, const double* sx,
+              const double* sy, const double* ss, const double* sr, int64_t nt, const double* tx,
+              const double* ty, const double* tr, double* tu, double* tv, double* tw) {
+  if (nt == 0 || ns == 0) return 0;
+  const bool trp = tr != nullptr;
+  if (prec == O2D_PLAIN_FP64) {
+    const int64_t twgs = (nt + kBlock * kTpl64 - 1) / (kBlock * kTpl64);
+    const int64_t ntiles = (ns + kTile - 1) / kTile;
+    ChunkPlan cp = plan_chunks(d, twgs, ntiles);
+    const int64_t chunk = cp.per_chunk * kTile;
+    const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
+    double *ou = tu, *ov = tv, *ow = tw;
+    const int direct = cp.nchunk == 1;
+    if (!direct) {
+      void* p;
+      const int nf = res == O2D_VELONLY ? 2 : 3;
+      O2D_TRY(slot(d, kPartials, sizeof(double) * nf * cp.nchunk * nt, s, &p));
+      ou = static_cast<double*>(p);
+      ov = ou + cp.nchunk * nt;
+      ow = res == O2D_VELONLY ? nullptr : ov + cp.nchunk * nt;
+    }
+    KernelTimer kt(O2D_KT_NBODY_PLAIN, s);
+    switch (core) {
+      case O2D_CORE_RM: dispatch64<O2D_CORE_RM>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_GAUSSIAN: dispatch64<O2D_CORE_GAUSSIAN>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_WL: dispatch64<O2D_CORE_WL>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_V2: dispatch64<O2D_CORE_V2>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
+      default: dispatch64<O2D_CORE_V3>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
+    }
+    kt.stop();
+    O2D_LAUNCH_CHECK();
+    if (!direct) {
+      reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, ou, ov, ow, tu, tv, tw);
+      O2D_LAUNCH_CHECK();
+    }
+    return 0;
+  }
+
+  // split form: velocity only
+  if (res != O2D_VELONLY)
+    return fail(O2D_ERR_RESULTS, "the split fp32 form computes velocity only (results=%d)", res);
+  if (ns > 0x7fffffffLL) return fail(O2D_ERR_SIZE, "too many sources for the split form");
+  const int64_t ntiles = (ns + kTile - 1) / kTile;
+  const int64_t npad = ntiles * kTile;
+  const int64_t ngroups = npad / kGroup;
+  void *kB, *oB, *tsrc, *tcent;
+  O2D_TRY(slot(d, kKeysB, ns * sizeof(uint32_t), s, &kB));
+  O2D_TRY(slot(d, kOrderB, ns * sizeof(uint32_t), s, &oB));
+  O2D_TRY(slot(d, kTileSrc, npad * sizeof(float4), s, &tsrc));
+  O2D_TRY(slot(d, kTileCent, ngroups * sizeof(double2), s, &tcent));
+  KernelTimer kprep(O2D_KT_SPLIT_PREP, s);
+  O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
+                       static_cast<uint32_t*>(oB)));
+  const uint32_t* order = static_cast<const uint32_t*>(oB);
+  double2* cent = static_cast<double2*>(tcent);
+  float4* src = static_cast<float4*>(tsrc);
+  centroid_kernel<<<grid1(ngroups), kBlock, 0, s>>>(ns, ngroups, order, sx, sy, cent);
+  switch (core) {
+    case O2D_CORE_RM: launch_rebase<O2D_CORE_RM>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_GAUSSIAN: launch_rebase<O2D_CORE_GAUSSIAN>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_WL: launch_rebase<O2D_CORE_WL>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_V2: launch_rebase<O2D_CORE_V2>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
+    default: launch_rebase<O2D_CORE_V3>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
+  }
+  kprep.stop();
+  O2D_LAUNCH_CHECK();
+
+  const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);
+  ChunkPlan cp = plan_chunks(d, twgs, ntiles);
+  const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
+  double *ou = tu, *ov = tv;
+  const int direct = cp.nchunk == 1;
+  if (!direct) {
+    void* p;
+    O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * cp.nchunk * nt, s, &p));
+    ou = static_cast<double*>(p);
+    ov = ou + cp.nchunk * nt;
+  }
+  KernelTimer kt(O2D_KT_NBODY_SPLIT, s);
+  switch (core) {
+    case O2D_CORE_RM: dispatch32<O2D_CORE_RM>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
+    case O2D_CORE_GAUSSIAN: dispatch32<O2D_CORE_GAUSSIAN>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
+    case O2D_CORE_WL: dispatch32<O2D_CORE_WL>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
+    case O2D_CORE_V2: dispatch32<O2D_CORE_V2>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
+    default: dispatch32<O2D_CORE_V3>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
+  }
+  kt.stop();
+  O2D_LAUNCH_CHECK();
+  if (!direct) {
+    reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, ou, ov, nullptr, tu, tv, nullptr);
+    O2D_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace o2d

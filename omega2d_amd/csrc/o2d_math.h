@@ -1,0 +1,214 @@
+// o2d_math.h - device-side regularised core functions (src/CoreFunc.h) in fp64 and fp32.
+//
+// fp64 forms follow the reference's arithmetic closely enough for 1e-12 relative agreement:
+// reciprocals and reciprocal square roots are the hardware seeds refined by two Newton steps
+// (about 1 ulp), exp/cbrt come from the device math library. fp32 forms are the split-precision
+// pair math: one hardware reciprocal (or rsq) per pair, the Gaussian's exp only inside its
+// core (d^2 <= 16 sigma^2), evaluated as an expm1-style series where 1 - e^-x would cancel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "omega2d_hip.h"
+
+namespace o2d {
+
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double t = __builtin_fma(-h * y, y, 0.5);
+  y = __builtin_fma(y, t, y);
+  t = __builtin_fma(-h * y, y, 0.5);
+  return __builtin_fma(y, t, y);
+}
+
+// ---- radius term ---------------------------------------------------------------------------
+// Every core depends on the two core radii only through w = f(sr) + f(tr): f(r) = r^2 for RM,
+// Gaussian and WL, r^4 for V2, r^6 for V3 (src/CoreFunc.h:48,56,136-147,190-200,239-249,283-294).
+// Sources carry f(sr), targets f(tr) (0 for point targets).
+template <int CORE, class T>
+__host__ __device__ __forceinline__ T rterm(T r) {
+  const T r2 = r * r;
+  if constexpr (CORE == O2D_CORE_V2) return r2 * r2;
+  else if constexpr (CORE == O2D_CORE_V3) return r2 * r2 * r2;
+  else return r2;
+}
+
+// ---- fp64 velocity factor K(d2; w) ---------------------------------------------------------
+
+// Gaussian inside its compact cutoff (src/CoreFunc.h:117-131), out of line like gauss_core32
+__device__ __noinline__ double gauss_core64(double d2, double w) {
+  const double cf = rcp64(w);
+  const double x = d2 * cf;
+  if (x < 0.001) return cf;
+  return (1.0 - exp(-x)) * rcp64(d2);
+}
+
+template <int CORE>
+__device__ __forceinline__ double core_k64(double d2, double w) {
+  if constexpr (CORE == O2D_CORE_RM) {
+    return rcp64(d2 + w);
+  } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+    if (d2 > 16.0 * w) return rcp64(d2);
+    return gauss_core64(d2, w);
+  } else if constexpr (CORE == O2D_CORE_WL) {
+    const double dd = d2 + w;
+    const double r = rcp64(dd);
+    return (dd + w) * r * r;
+  } else if constexpr (CORE == O2D_CORE_V2) {
+    return rsq64(__builtin_fma(d2, d2, w));
+  } else {
+    return rcbrt(__builtin_fma(d2 * d2, d2, w));
+  }
+}
+
+// K and bbb = 2 dK/d(d^2) in the reference's gradient forms (src/CoreFunc.h:64-81, 156-180,
+// 208-229, 256-274, 301-321). Gaussian at d2 == 0: bbb = 0 (the reference yields NaN there).
+template <int CORE>
+__device__ __forceinline__ void core_kg64(double d2, double w, double& k, double& bbb) {
+  if constexpr (CORE == O2D_CORE_RM) {
+    const double r = rcp64(d2 + w);
+    k = r;
+    bbb = -2.0 * r * r;
+  } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+    const double cf = rcp64(w);
+    const double x = d2 * cf;
+    const double ood2 = rcp64(d2);
+    const double e = exp(-x);
+    double r;
+    if (x > 16.0) r = ood2;
+    else if (x < 0.001) r = cf;
+    else r = ood2 * (1.0 - e);
+    k = r;
+    bbb = (d2 == 0.0) ? 0.0 : (-2.0 * r * ood2 + 2.0 * cf * ood2 * e);
+  } else if constexpr (CORE == O2D_CORE_WL) {
+    const double dd = d2 + w;
+    const double ood2s = rcp64(dd * dd);
+    const double r = (dd + w) * ood2s;
+    k = r;
+    bbb = (2.0 - 4.0 * r * dd) * ood2s;
+  } else if constexpr (CORE == O2D_CORE_V2) {
+    const double r = rsq64(__builtin_fma(d2, d2, w));
+    k = r;
+    bbb = -2.0 * r * r * r * d2;
+  } else {
+    const double ds6 = __builtin_fma(d2 * d2, d2, w);
+    const double r = rcbrt(ds6);
+    k = r;
+    bbb = -2.0 * d2 * d2 * r * rcp64(ds6);
+  }
+}
+
+// K, bbb consistent with the branch of K actually taken, and dK/dsigma_source, for the adjoint
+// of point-target sums (t2 = 0).
+template <int CORE>
+__device__ __forceinline__ void core_adj64(double d2, double sr, double& k, double& bbb,
+                                           double& dks) {
+  const double s2 = sr * sr;
+  if constexpr (CORE == O2D_CORE_RM) {
+    const double r = rcp64(d2 + s2);
+    k = r;
+    bbb = -2.0 * r * r;
+    dks = -2.0 * sr * r * r;
+  } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+    if (d2 > 16.0 * s2) {
+      const double ood2 = rcp64(d2);
+      k = ood2;
+      bbb = -2.0 * ood2 * ood2;
+      dks = 0.0;
+    } else {
+      const double cf = rcp64(s2);
+      const double x = d2 * cf;
+      if (x < 0.001) {
+        k = cf;
+        bbb = 0.0;
+        dks = -2.0 * cf * rcp64(sr);
+      } else {
+        const double ood2 = rcp64(d2);
+        const double e = exp(-x);
+        const double r = ood2 * (1.0 - e);
+        k = r;
+        bbb = -2.0 * r * ood2 + 2.0 * cf * ood2 * e;
+        dks = -2.0 * e * cf * rcp64(sr);
+      }
+    }
+  } else if constexpr (CORE == O2D_CORE_WL) {
+    const double dd = d2 + s2;
+    const double r1 = rcp64(dd);
+    const double r3 = r1 * r1 * r1;
+    k = (dd + s2) * r1 * r1;
+    bbb = -2.0 * (dd + 2.0 * s2) * r3;
+    dks = -4.0 * s2 * sr * r3;
+  } else if constexpr (CORE == O2D_CORE_V2) {
+    const double r = rsq64(__builtin_fma(d2, d2, s2 * s2));
+    const double r3 = r * r * r;
+    k = r;
+    bbb = -2.0 * r3 * d2;
+    dks = -2.0 * s2 * sr * r3;
+  } else {
+    const double ds6 = d2 * d2 * d2 + s2 * s2 * s2;
+    const double r = rcbrt(ds6);
+    const double q = r * rcp64(ds6);
+    k = r;
+    bbb = -2.0 * d2 * d2 * q;
+    dks = -2.0 * s2 * s2 * sr * q;
+  }
+}
+
+// ---- fp32 split-precision velocity factor -------------------------------------------------
+
+// 1 - e^-x for 0.001 <= x <= 16 without cancellation: Taylor series below 0.25 (truncation
+// < 2e-9 relative there), otherwise 1 - exp2(-x log2 e).
+__device__ __forceinline__ float one_minus_expm(float x) {
+  if (x < 0.25f) {
+    float p = __builtin_fmaf(x, -1.0f / 720.0f, 1.0f / 120.0f);
+    p = __builtin_fmaf(x, p, -1.0f / 24.0f);
+    p = __builtin_fmaf(x, p, 1.0f / 6.0f);
+    p = __builtin_fmaf(x, p, -0.5f);
+    p = __builtin_fmaf(x, p, 1.0f);
+    return x * p;
+  }
+  return 1.0f - __builtin_amdgcn_exp2f(-x * 1.4426950408889634f);
+}
+
+// Inside the Gaussian core (d2 <= 16 w, rare for all but the nearest pairs); kept out of line
+// so the unrolled far-field loop does not carry its registers.
+__device__ __forceinline__ float gauss_core32(float d2, float w, float rd2) {
+  const float cf = __builtin_amdgcn_rcpf(w);
+  const float x = d2 * cf;
+  return (x < 0.001f) ? cf : one_minus_expm(x) * rd2;
+}
+
+// w = radius term of the source plus that of the target (see rterm)
+template <int CORE>
+__device__ __forceinline__ float core_k32(float dx, float dy, float w) {
+  if constexpr (CORE == O2D_CORE_RM) {
+    return __builtin_amdgcn_rcpf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, w)));
+  } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+    const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+    float k = __builtin_amdgcn_rcpf(d2);
+    if (d2 <= 16.0f * w) k = gauss_core32(d2, w, k);
+    return k;
+  } else if constexpr (CORE == O2D_CORE_WL) {
+    const float dd = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, w));
+    const float r = __builtin_amdgcn_rcpf(dd);
+    return (dd + w) * r * r;
+  } else if constexpr (CORE == O2D_CORE_V2) {
+    const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+    return __builtin_amdgcn_rsqf(__builtin_fmaf(d2, d2, w));
+  } else {
+    const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+    const float v = __builtin_fmaf(d2 * d2, d2, w);
+    return __builtin_amdgcn_exp2f(-0.33333333333f * __builtin_amdgcn_logf(v));
+  }
+}
+
+}  // namespace o2d

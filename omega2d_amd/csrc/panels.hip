@@ -1,0 +1,391 @@
+// panels.hip - constant-strength straight panels acting on points, and point vortices acting on
+// panels (the BEM right-hand side). fp64 throughout.
+//
+// panels_on_points (src/Influence.h:422-614, kernels src/Kernels.h:314-405): one lane per
+// target point, 256-panel tiles staged in LDS, fp64 sums.
+//
+// points_on_panels (src/Influence.h:633-771): targets are the few panels, sources the many
+// particles. A workgroup owns kPanTile panels (held in scalar registers) and a chunk of
+// particles; each lane walks its share of particles and accumulates every panel's integral,
+// then the 256 lanes are reduced with wavefront shuffles and one LDS step. Chunk partials are
+// summed in a fixed order by a second kernel, which applies the -1/L factor.
+//
+// The subtended angle u* = atan2(dx1,dy1) - atan2(dx0,dy0) wrapped to [-pi, pi]
+// (src/Kernels.h:262-268) is evaluated as the single atan2 of the cross and dot products of the
+// two endpoint vectors, which is the same angle already wrapped (one atan2 instead of two).
+#include <algorithm>
+
+#include "nbody.h"
+#include "o2d_math.h"
+#include "o2d_runtime.h"
+
+namespace o2d {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kPTile = 256;  // panels per LDS stage (panels on points)
+constexpr int kPanTile = 8;  // panels per workgroup (points on panels)
+
+struct Panel {
+  double x0, y0, x1, y1, vs, ss;
+};
+
+// velocity at (tx, ty) of panel (x0,y0)-(x1,y1) with vortex strength vs and source strength ss
+// per unit length (src/Kernels.h:364-405); vortex-only panels pass ss = 0
+__device__ __forceinline__ void panel_vel(double x0, double y0, double x1, double y1, double vs,
+                                          double ss, double tx, double ty, double& u, double& v) {
+  const double dx0 = tx - x0, dy0 = ty - y0;
+  const double dx1 = tx - x1, dy1 = ty - y1;
+  const double px = x1 - x0, py = y1 - y0;
+  const double rij2 = __builtin_fma(dx0, dx0, dy0 * dy0);
+  const double rij12 = __builtin_fma(dx1, dx1, dy1 * dy1);
+  const double mult = rsq64(__builtin_fma(px, px, py * py));
+  const double us = atan2(__builtin_fma(py, dx0, -px * dy0), __builtin_fma(dy1, dy0, dx1 * dx0));
+  const double vstar = 0.5 * log(rij2 * rcp64(rij12));
+  const double tpx = px * mult, tpy = py * mult;
+  const double a = us * tpx - vstar * tpy;  // along-panel rotation of (u*, v*)
+  const double b = us * tpy + vstar * tpx;
+  u = vs * a + ss * b;
+  v = vs * b - ss * a;
+}
+
+template <bool SRC>
+__global__ __launch_bounds__(kBlock) void panels_on_points_kernel(
+    int64_t np, const double* __restrict__ nx, const double* __restrict__ ny,
+    const uint32_t* __restrict__ idx, const double* __restrict__ vs, const double* __restrict__ ss,
+    int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty, int64_t chunk,
+    double* __restrict__ ou, double* __restrict__ ov, int direct) {
+  __shared__ Panel tile[kPTile];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
+  const int64_t ii = i < nt ? i : nt - 1;
+  const double px = tx[ii], py = ty[ii];
+  double au = 0.0, av = 0.0;
+  const int64_t jb = (int64_t)blockIdx.y * chunk;
+  const int64_t je = jb + chunk < np ? jb + chunk : np;
+  for (int64_t j0 = jb; j0 < je; j0 += kPTile) {
+    const int64_t j = j0 + tid;
+    Panel p;
+    if (j < je) {
+      const uint32_t a = idx[2 * j], b = idx[2 * j + 1];
+      p.x0 = nx[a];
+      p.y0 = ny[a];
+      p.x1 = nx[b];
+      p.y1 = ny[b];
+      p.vs = vs[j];
+      p.ss = SRC ? ss[j] : 0.0;
+    }
+    __syncthreads();
+    tile[tid] = p;
+    __syncthreads();
+    const int nk = (int)(je - j0 < kPTile ? je - j0 : kPTile);
+    for (int k = 0; k < nk; ++k) {
+      const Panel q = tile[k];
+      double u, v;
+      panel_vel(q.x0, q.y0, q.x1, q.y1, q.vs, q.ss, px, py, u, v);
+      au += u;
+      av += v;
+    }
+  }
+  if (i >= nt) return;
+  if (direct) {
+    ou[i] += au;
+    ov[i] += av;
+  } else {
+    ou[(int64_t)blockIdx.y * nt + i] = au;
+    ov[(int64_t)blockIdx.y * nt + i] = av;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_chunks2_kernel(int64_t nchunk, int64_t n,
+                                                                const double* __restrict__ pu,
+                                                                const double* __restrict__ pv,
+                                                                double* __restrict__ ou,
+                                                                double* __restrict__ ov) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double su = 0.0, sv = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    su += pu[c * n + i];
+    sv += pv[c * n + i];
+  }
+  ou[i] += su;
+  ov[i] += sv;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// part[(c*2 + {0,1}) * np + p] = chunk c's (u, v) integral sum for panel p
+__global__ __launch_bounds__(kBlock) void points_on_panels_kernel(
+    int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ ss, int64_t np, const double* __restrict__ nx,
+    const double* __restrict__ ny, const uint32_t* __restrict__ idx, int64_t chunk,
+    double* __restrict__ part) {
+  __shared__ double red[kBlock / 64][2 * kPanTile];
+  const int tid = threadIdx.x;
+  const int64_t p0 = (int64_t)blockIdx.x * kPanTile;
+  double x0[kPanTile], y0[kPanTile], x1[kPanTile], y1[kPanTile];
+#pragma unroll
+  for (int k = 0; k < kPanTile; ++k) {
+    const int64_t p = p0 + k < np ? p0 + k : np - 1;
+    const uint32_t a = idx[2 * p], b = idx[2 * p + 1];
+    x0[k] = nx[a];
+    y0[k] = ny[a];
+    x1[k] = nx[b];
+    y1[k] = ny[b];
+  }
+  double au[kPanTile], av[kPanTile];
+#pragma unroll
+  for (int k = 0; k < kPanTile; ++k) au[k] = av[k] = 0.0;
+  const int64_t jb = (int64_t)blockIdx.y * chunk;
+  const int64_t je = jb + chunk < ns ? jb + chunk : ns;
+  for (int64_t j = jb + tid; j < je; j += kBlock) {
+    const double qx = sx[j], qy = sy[j], qs = ss[j];
+#pragma unroll
+    for (int k = 0; k < kPanTile; ++k) {
+      double u, v;
+      panel_vel(x0[k], y0[k], x1[k], y1[k], qs, 0.0, qx, qy, u, v);
+      au[k] += u;
+      av[k] += v;
+    }
+  }
+  const int wv = tid / 64, ln = tid % 64;
+#pragma unroll
+  for (int k = 0; k < kPanTile; ++k) {
+    const double su = wave_sum(au[k]);
+    const double sv = wave_sum(av[k]);
+    if (ln == 0) {
+      red[wv][2 * k] = su;
+      red[wv][2 * k + 1] = sv;
+    }
+  }
+  __syncthreads();
+  if (tid < 2 * kPanTile) {
+    const int k = tid / 2, c = tid % 2;
+    if (p0 + k < np) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += red[w][tid];
+      part[((int64_t)blockIdx.y * 2 + c) * np + p0 + k] = s;
+    }
+  }
+}
+
+// pu[p] -= (1/L_p) * sum over chunks (src/Influence.h:726-759)
+__global__ __launch_bounds__(kBlock) void points_on_panels_finish_kernel(
+    int64_t nchunk, int64_t np, const double* __restrict__ nx, const double* __restrict__ ny,
+    const uint32_t* __restrict__ idx, const double* __restrict__ part, double* __restrict__ pu,
+    double* __restrict__ pv) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= np) return;
+  double su = 0.0, sv = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    su += part[(c * 2) * np + p];
+    sv += part[(c * 2 + 1) * np + p];
+  }
+  const uint32_t a = idx[2 * p], b = idx[2 * p + 1];
+  const double ex = nx[b] - nx[a], ey = ny[b] - ny[a];
+  const double plen = 1.0 / sqrt(ex * ex + ey * ey);
+  pu[p] -= plen * su;
+  pv[p] -= plen * sv;
+}
+
+inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+}  // namespace
+
+int panels_on_points_dev(Device* d, hipStream_t s, int64_t nnodes, const double* nx,
+                         const double* ny, int64_t np, const uint32_t* idx, const double* vs,
+                         const double* ss, int64_t nt, const double* tx, const double* ty,
+                         double* tu, double* tv) {
+  (void)nnodes;
+  if (np == 0 || nt == 0) return 0;
+  const int64_t twgs = (nt + kBlock - 1) / kBlock;
+  const int64_t ntiles = (np + kPTile - 1) / kPTile;
+  const int64_t want = 8 * (int64_t)d->num_cu;
+  int64_t nchunk = twgs >= want ? 1 : std::min<int64_t>((want + twgs - 1) / twgs, np);
+  const int64_t chunk = ((np + nchunk - 1) / nchunk + 0);
+  nchunk = (np + chunk - 1) / chunk;
+  (void)ntiles;
+  double *ou = tu, *ov = tv;
+  const int direct = nchunk == 1;
+  if (!direct) {
+    void* p;
+    O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * nchunk * nt, s, &p));
+    ou = static_cast<double*>(p);
+    ov = ou + nchunk * nt;
+  }
+  const dim3 grid((unsigned)twgs, (unsigned)nchunk);
+  KernelTimer kt(O2D_KT_PANELS, s);
+  if (ss)
+    panels_on_points_kernel<true><<<grid, kBlock, 0, s>>>(np, nx, ny, idx, vs, ss, nt, tx, ty, chunk,
+                                                          ou, ov, direct);
+  else
+    panels_on_points_kernel<false><<<grid, kBlock, 0, s>>>(np, nx, ny, idx, vs, ss, nt, tx, ty,
+                                                           chunk, ou, ov, direct);
+  kt.stop();
+  O2D_LAUNCH_CHECK();
+  if (!direct) {
+    reduce_chunks2_kernel<<<grid1(nt), kBlock, 0, s>>>(nchunk, nt, ou, ov, tu, tv);
+    O2D_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+int points_on_panels_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const double* sy,
+                         const double* ss, int64_t nnodes, const double* nx, const double* ny,
+                         int64_t np, const uint32_t* idx, double* pu, double* pv) {
+  (void)nnodes;
+  if (np == 0 || ns == 0) return 0;
+  const int64_t pwgs = (np + kPanTile - 1) / kPanTile;
+  const int64_t want = 8 * (int64_t)d->num_cu;
+  int64_t nchunk = pwgs >= want ? 1 : (want + pwgs - 1) / pwgs;
+  const int64_t maxchunk = (ns + kBlock - 1) / kBlock;  // at least one particle per lane
+  if (nchunk > maxchunk) nchunk = maxchunk;
+  const int64_t chunk = (ns + nchunk - 1) / nchunk;
+  nchunk = (ns + chunk - 1) / chunk;
+  void* p;
+  O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * nchunk * np, s, &p));
+  double* part = static_cast<double*>(p);
+  KernelTimer kt(O2D_KT_PANEL_RHS, s);
+  points_on_panels_kernel<<<dim3((unsigned)pwgs, (unsigned)nchunk), kBlock, 0, s>>>(
+      ns, sx, sy, ss, np, nx, ny, idx, chunk, part);
+  kt.stop();
+  O2D_LAUNCH_CHECK();
+  points_on_panels_finish_kernel<<<grid1(np), kBlock, 0, s>>>(nchunk, np, nx, ny, idx, part, pu, pv);
+  O2D_LAUNCH_CHECK();
+  return 0;
+}
+
+// shared validation of panel geometry arguments
+static int check_panels(int64_t nnodes, const void* nx, const void* ny, int64_t np,
+                        const uint32_t* idx_host) {
+  if (nnodes < 0 || np < 0) return fail(O2D_ERR_SIZE, "negative count");
+  if (np > 0x7fffffffLL || nnodes > 0xffffffffLL) return fail(O2D_ERR_SIZE, "too many panels");
+  if (np > 0 && (!nx || !ny || !idx_host)) return fail(O2D_ERR_ARG, "NULL panel array");
+  return 0;
+}
+
+static int check_idx_host(int64_t nnodes, int64_t np, const uint32_t* idx) {
+  for (int64_t j = 0; j < 2 * np; ++j)
+    if ((int64_t)idx[j] >= nnodes)
+      return fail(O2D_ERR_SIZE, "panel node index %u at %lld >= nnodes %lld", idx[j],
+                  (long long)j, (long long)nnodes);
+  return 0;
+}
+
+}  // namespace o2d
+
+using namespace o2d;
+
+extern "C" {
+
+int o2d_hip_panels_on_points_dev_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                                   const double* ny, int64_t np, const uint32_t* idx,
+                                   const double* vstr, const double* sstr, int64_t nt,
+                                   const double* tx, const double* ty, double* tu, double* tv) {
+  clear_error();
+  O2D_TRY(check_env(env));
+  if (env->results != O2D_VELONLY)
+    return fail(O2D_ERR_RESULTS, "panels compute velocity only (results=%d)", env->results);
+  O2D_TRY(check_panels(nnodes, nx, ny, np, idx));
+  if (np > 0 && !vstr) return fail(O2D_ERR_ARG, "NULL vstr");
+  if (nt < 0 || nt > 0x7fffffffLL) return fail(O2D_ERR_SIZE, "bad target count");
+  if (nt > 0 && (!tx || !ty || !tu || !tv)) return fail(O2D_ERR_ARG, "NULL target array");
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  return panels_on_points_dev(d, dev_stream(env), nnodes, nx, ny, np, idx, vstr, sstr, nt, tx,
+                              ty, tu, tv);
+}
+
+int o2d_hip_panels_on_points_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                               const double* ny, int64_t np, const uint32_t* idx,
+                               const double* vstr, const double* sstr, int64_t nt,
+                               const double* tx, const double* ty, double* tu, double* tv) {
+  clear_error();
+  O2D_TRY(check_env(env));
+  if (env->results != O2D_VELONLY)
+    return fail(O2D_ERR_RESULTS, "panels compute velocity only (results=%d)", env->results);
+  O2D_TRY(check_panels(nnodes, nx, ny, np, idx));
+  if (np > 0 && !vstr) return fail(O2D_ERR_ARG, "NULL vstr");
+  if (nt < 0 || nt > 0x7fffffffLL) return fail(O2D_ERR_SIZE, "bad target count");
+  if (nt > 0 && (!tx || !ty || !tu || !tv)) return fail(O2D_ERR_ARG, "NULL target array");
+  O2D_TRY(check_idx_host(nnodes, np, idx));
+  if (np == 0 || nt == 0) return 0;
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  hipStream_t s = stream_of(env, d);
+  double *dnx, *dny, *dvs, *dss = nullptr, *dtx, *dty, *du, *dv;
+  uint32_t* didx;
+  O2D_TRY(upload<double>(d, kPanX, nx, nnodes, s, &dnx));
+  O2D_TRY(upload<double>(d, kPanY, ny, nnodes, s, &dny));
+  O2D_TRY(upload<uint32_t>(d, kPanIdx, idx, 2 * np, s, &didx));
+  O2D_TRY(upload<double>(d, kPanVs, vstr, np, s, &dvs));
+  if (sstr) O2D_TRY(upload<double>(d, kPanSs, sstr, np, s, &dss));
+  O2D_TRY(upload<double>(d, kTrgX, tx, nt, s, &dtx));
+  O2D_TRY(upload<double>(d, kTrgY, ty, nt, s, &dty));
+  O2D_TRY(upload<double>(d, kOutU, tu, nt, s, &du));
+  O2D_TRY(upload<double>(d, kOutV, tv, nt, s, &dv));
+  O2D_TRY(panels_on_points_dev(d, s, nnodes, dnx, dny, np, didx, dvs, dss, nt, dtx, dty, du, dv));
+  O2D_TRY(download<double>(tu, du, nt, s));
+  O2D_TRY(download<double>(tv, dv, nt, s));
+  O2D_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int o2d_hip_points_on_panels_dev_d(const o2d_env* env, int64_t ns, const double* sx,
+                                   const double* sy, const double* sstr, int64_t nnodes,
+                                   const double* nx, const double* ny, int64_t np,
+                                   const uint32_t* idx, double* pu, double* pv) {
+  clear_error();
+  O2D_TRY(check_env(env));
+  if (env->results != O2D_VELONLY)
+    return fail(O2D_ERR_RESULTS, "panels compute velocity only (results=%d)", env->results);
+  O2D_TRY(check_panels(nnodes, nx, ny, np, idx));
+  if (ns < 0) return fail(O2D_ERR_SIZE, "negative source count");
+  if (ns > 0 && (!sx || !sy || !sstr)) return fail(O2D_ERR_ARG, "NULL source array");
+  if (np > 0 && (!pu || !pv)) return fail(O2D_ERR_ARG, "NULL output");
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  return points_on_panels_dev(d, dev_stream(env), ns, sx, sy, sstr, nnodes, nx, ny, np, idx, pu,
+                              pv);
+}
+
+int o2d_hip_points_on_panels_d(const o2d_env* env, int64_t ns, const double* sx,
+                               const double* sy, const double* sstr, int64_t nnodes,
+                               const double* nx, const double* ny, int64_t np,
+                               const uint32_t* idx, double* pu, double* pv) {
+  clear_error();
+  O2D_TRY(check_env(env));
+  if (env->results != O2D_VELONLY)
+    return fail(O2D_ERR_RESULTS, "panels compute velocity only (results=%d)", env->results);
+  O2D_TRY(check_panels(nnodes, nx, ny, np, idx));
+  if (ns < 0) return fail(O2D_ERR_SIZE, "negative source count");
+  if (ns > 0 && (!sx || !sy || !sstr)) return fail(O2D_ERR_ARG, "NULL source array");
+  if (np > 0 && (!pu || !pv)) return fail(O2D_ERR_ARG, "NULL output");
+  O2D_TRY(check_idx_host(nnodes, np, idx));
+  if (np == 0 || ns == 0) return 0;
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  hipStream_t s = stream_of(env, d);
+  double *dsx, *dsy, *dss, *dnx, *dny, *du, *dv;
+  uint32_t* didx;
+  O2D_TRY(upload<double>(d, kSrcX, sx, ns, s, &dsx));
+  O2D_TRY(upload<double>(d, kSrcY, sy, ns, s, &dsy));
+  O2D_TRY(upload<double>(d, kSrcS, sstr, ns, s, &dss));
+  O2D_TRY(upload<double>(d, kPanX, nx, nnodes, s, &dnx));
+  O2D_TRY(upload<double>(d, kPanY, ny, nnodes, s, &dny));
+  O2D_TRY(upload<uint32_t>(d, kPanIdx, idx, 2 * np, s, &didx));
+  O2D_TRY(upload<double>(d, kOutU, pu, np, s, &du));
+  O2D_TRY(upload<double>(d, kOutV, pv, np, s, &dv));
+  O2D_TRY(points_on_panels_dev(d, s, ns, dsx, dsy, dss, nnodes, dnx, dny, np, didx, du, dv));
+  O2D_TRY(download<double>(pu, du, np, s));
+  O2D_TRY(download<double>(pv, dv, np, s));
+  O2D_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+// Convection.h - velocity evaluation and Runge-Kutta advection of the gpu_hip environment: host
+// mirror of src/Convection.h:172-598 (find_vels, find_derivs, advect with Euler, Ralston RK2
+// and Ralston RK3). Positions advance in fp64 on the device (o2d_hip_rk_move_d). The
+// near-wall clean-up the reference runs after each move (clear_inner_layer, src/Reflect.h) is
+// not part of this path.
+#pragma once
+
+#include <array>
+#include <stdexcept>
+#include <variant>
+#include <vector>
+
+#include "BEM.h"
+
+namespace omega2d {
+
+// x += dt * sum_k w[k] * u_k (src/ElementBase.h:276-359), stage velocities taken from `stages`
+template <class S>
+void move_points(Points<S>& p, double dt, const std::vector<double>& w,
+                 const std::vector<const Points<S>*>& stages, int32_t device) {
+  if (p.get_movet() != lagrangian) return;
+  const size_t n = p.get_n();
+  auto x = as_double(p.get_pos()[0]), y = as_double(p.get_pos()[1]);
+  std::vector<std::vector<double>> us, vs;
+  std::vector<const double*> up, vp;
+  for (const Points<S>* q : stages) {
+    us.push_back(as_double(q->get_vel()[0]));
+    vs.push_back(as_double(q->get_vel()[1]));
+  }
+  for (size_t k = 0; k < stages.size(); ++k) {
+    up.push_back(us[k].data());
+    vp.push_back(vs[k].data());
+  }
+  o2d_env e = {O2D_CORE_V2, O2D_PLAIN_FP64, O2D_VELONLY, device, nullptr};
+  o2d_check(o2d_hip_rk_move_d(&e, (int64_t)n, dt, (int32_t)stages.size(), w.data(), up.data(),
+                              vp.data(), x.data(), y.data()),
+            "o2d_hip_rk_move");
+  for (size_t i = 0; i < n; ++i) {
+    p.get_pos()[0][i] = S(x[i]);
+    p.get_pos()[1][i] = S(y[i]);
+  }
+}
+
+template <class S, class A, class I>
+class Convection {
+ public:
+  explicit Convection(const ExecEnv& env, int order = 2) : env(env), order(order) {}
+
+  // src/Convection.h:172-223
+  void find_vels(const std::array<double, Dimensions>& fs, std::vector<Collection<S>>& vort,
+                 std::vector<Collection<S>>& bdry, std::vector<Collection<S>>& targets,
+                 results_t results = velonly, bool force = false) {
+    InfluenceVisitor<S, A> visitor = {ResultsType(results), env};
+    for (auto& b : bdry)
+      if (auto* s = std::get_if<Surfaces<S>>(&b)) s->add_solved_rot_strengths(S(1.0));
+    for (auto& t : targets) {
+      const move_t mt = std::visit([](auto& el) { return el.get_movet(); }, t);
+      if (!(force || mt == lagrangian)) continue;
+      std::visit([](auto& el) { el.zero_vels(); }, t);
+      for (auto& s : vort) std::visit(visitor, std::as_const(s), t);
+      for (auto& s : bdry) std::visit(visitor, std::as_const(s), t);
+      std::visit([&](auto& el) { el.finalize_vels(fs); }, t);
+    }
+    for (auto& b : bdry)
+      if (auto* s = std::get_if<Surfaces<S>>(&b)) s->add_solved_rot_strengths(S(-1.0));
+  }
+
+  // src/Convection.h:229-243
+  void find_derivs(double time, const std::array<double, Dimensions>& fs, BEM<S, I>& bem,
+                   std::vector<Collection<S>>& bdry, std::vector<Collection<S>>& vort,
+                   std::vector<Collection<S>>& fldpt) {
+    solve_bem<S, A, I>(time, fs, vort, bdry, bem, env);
+    find_vels(fs, vort, bdry, vort);
+    find_vels(fs, vort, bdry, fldpt);
+  }
+
+  void advect(double time, double dt, const std::array<double, Dimensions>& fs,
+              std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
+              std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
+    if (order == 1) advect_1st(time, dt, fs, vort, bdry, fldpt, bem);
+    else if (order == 2) advect_2nd_ralston(time, dt, fs, vort, bdry, fldpt, bem);
+    else if (order == 3) advect_3rd(time, dt, fs, vort, bdry, fldpt, bem);
+    else throw std::runtime_error("convection order must be 1, 2 or 3");
+  }
+
+  // src/Convection.h:274-303
+  void advect_1st(double time, double dt, const std::array<double, Dimensions>& fs,
+                  std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
+                  std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
+    find_derivs(time, fs, bem, bdry, vort, fldpt);
+    for (auto* list : {&vort, &fldpt})
+      for (auto& c : *list)
+        if (auto* p = std::get_if<Points<S>>(&c)) move_points<S>(*p, dt, {1.0}, {p}, dev());
+  }
+
+  // src/Convection.h:390-466: stage at t + 2dt/3, u = u1/4 + 3 u2/4
+  void advect_2nd_ralston(double time, double dt, const std::array<double, Dimensions>& fs,
+                          std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
+                          std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
+    const double twothirds = 2.0 / 3.0;
+    find_derivs(time, fs, bem, bdry, vort, fldpt);
+    std::vector<Collection<S>> iv = vort, ifl = fldpt;
+    for (auto* list : {&iv, &ifl})
+      for (auto& c : *list)
+        if (auto* p = std::get_if<Points<S>>(&c)) move_points<S>(*p, twothirds * dt, {1.0}, {p}, dev());
+    find_derivs(time + twothirds * dt, fs, bem, bdry, iv, ifl);
+    combine(vort, {&iv}, dt, {0.25, 0.75});
+    combine(fldpt, {&ifl}, dt, {0.25, 0.75});
+  }
+
+  // src/Convection.h:472-598: stages at t, t + dt/2, t + 3dt/4 (the third from x0 with u1),
+  // weights 2/9, 3/9, 4/9
+  void advect_3rd(double time, double dt, const std::array<double, Dimensions>& fs,
+                  std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
+                  std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
+    find_derivs(time, fs, bem, bdry, vort, fldpt);
+    std::vector<Collection<S>> v1 = vort, f1 = fldpt;
+    for (auto* list : {&v1, &f1})
+      for (auto& c : *list)
+        if (auto* p = std::get_if<Points<S>>(&c)) move_points<S>(*p, 0.5 * dt, {1.0}, {p}, dev());
+    find_derivs(time + 0.5 * dt, fs, bem, bdry, v1, f1);
+    std::vector<Collection<S>> v2 = vort, f2 = fldpt;
+    for (auto pr : {std::make_pair(&v2, &v1), std::make_pair(&f2, &f1)})
+      for (size_t i = 0; i < pr.first->size(); ++i) {
+        auto* p2 = std::get_if<Points<S>>(&(*pr.first)[i]);
+        auto* p1 = std::get_if<Points<S>>(&(*pr.second)[i]);
+        if (p2 && p1) move_points<S>(*p2, 0.75 * dt, {1.0}, {p1}, dev());
+      }
+    find_derivs(time + 0.75 * dt, fs, bem, bdry, v2, f2);
+    combine(vort, {&v1, &v2}, dt, {2.0 / 9.0, 3.0 / 9.0, 4.0 / 9.0});
+    combine(fldpt, {&f1, &f2}, dt, {2.0 / 9.0, 3.0 / 9.0, 4.0 / 9.0});
+  }
+
+ private:
+  int32_t dev() const { return env.get_device(); }
+
+  void combine(std::vector<Collection<S>>& base, std::vector<std::vector<Collection<S>>*> more,
+               double dt, const std::vector<double>& w) {
+    for (size_t i = 0; i < base.size(); ++i) {
+      auto* p0 = std::get_if<Points<S>>(&base[i]);
+      if (!p0) continue;
+      std::vector<const Points<S>*> st = {p0};
+      for (auto* m : more) st.push_back(std::get_if<Points<S>>(&(*m)[i]));
+      move_points<S>(*p0, dt, w, st, dev());
+    }
+  }
+
+  ExecEnv env;
+  int order;
+};
+
+}  // namespace omega2d

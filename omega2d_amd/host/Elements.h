@@ -1,0 +1,338 @@
+// Elements.h - SoA element collections for the HIP path: particles/tracers (Points) and
+// straight two-node panels (Surfaces). Host mirror of the members of src/Points.h,
+// src/Surfaces.h and src/ElementBase.h that the direct Biot-Savart path reads and writes;
+// drawing, VTK output, diffusion and feature bookkeeping are not part of it.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cassert>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+namespace omega2d {
+
+constexpr size_t Dimensions = 2;
+using Int = uint32_t;
+
+enum elem_t { active = 1, reactive = 2, inert = 3, hybrid = 4 };
+enum move_t { lagrangian = 1, bodybound = 2, fixed = 3 };
+
+template <class S>
+using Vector = std::vector<S>;
+
+// A rigid body reduced to what the BEM path asks of it: its name (the "ground" body is never
+// augmented when it bounds internal flow) and its rotation rate.
+struct Body {
+  std::string name = "ground";
+  double rotvel = 0.0;
+  const std::string& get_name() const { return name; }
+  double get_rotvel() const { return rotvel; }
+};
+
+// ---------------------------------------------------------------------------------- Points
+
+template <class S>
+class Points {
+ public:
+  // particles (strengths and radii) or, with str == nullptr, inert tracers
+  Points(size_t n, const S* x, const S* y, const S* str, const S* rad, elem_t e = active,
+         move_t m = lagrangian)
+      : E(str ? e : inert), M(m), n(n) {
+    for (size_t d = 0; d < Dimensions; ++d) {
+      xs[d].assign(n, S(0));
+      u[d].assign(n, S(0));
+    }
+    for (size_t i = 0; i < n; ++i) {
+      xs[0][i] = x[i];
+      xs[1][i] = y[i];
+    }
+    if (str) {
+      s.assign(str, str + n);
+      r.assign(rad, rad + n);
+    }
+  }
+
+  size_t get_n() const { return n; }
+  bool is_inert() const { return E == inert; }
+  move_t get_movet() const { return M; }
+  const std::array<Vector<S>, Dimensions>& get_pos() const { return xs; }
+  std::array<Vector<S>, Dimensions>& get_pos() { return xs; }
+  const Vector<S>& get_str() const { return s; }
+  Vector<S>& get_str() { return s; }
+  const Vector<S>& get_rad() const { return r; }
+  const std::array<Vector<S>, Dimensions>& get_vel() const { return u; }
+  std::array<Vector<S>, Dimensions>& get_vel() { return u; }
+  Vector<S>& get_vort() {
+    if (!w) w = Vector<S>(n, S(0));
+    return *w;
+  }
+  Vector<S>& get_shear() {
+    if (!e) e = Vector<S>(n, S(0));
+    return *e;
+  }
+  bool has_vort() const { return (bool)w; }
+  bool has_shear() const { return (bool)e; }
+
+  // src/ElementBase.h:200-212
+  void zero_vels() {
+    for (size_t d = 0; d < Dimensions; ++d) std::fill(u[d].begin(), u[d].end(), S(0));
+    if (w) std::fill(w->begin(), w->end(), S(0));
+    if (e) std::fill(e->begin(), e->end(), S(0));
+  }
+
+  // src/ElementBase.h:215-234: u = fs + u/(2 pi) in double, stored to S
+  void finalize_vels(const std::array<double, Dimensions>& fs) {
+    const double factor = 0.5 / M_PI;
+    for (size_t d = 0; d < Dimensions; ++d)
+      for (size_t i = 0; i < n; ++i) u[d][i] = S(fs[d] + double(u[d][i]) * factor);
+    if (w)
+      for (auto& v : *w) v = S(double(v) * factor);
+    if (e)
+      for (auto& v : *e) v = S(double(v) * factor);
+  }
+
+  std::string to_string() const {
+    std::string m = E == active ? " Active" : (E == reactive ? " Reactive" : " Inert");
+    m += M == lagrangian ? " Lagrangian" : (M == bodybound ? " Body-fixed" : " Fixed");
+    return m + " Points";
+  }
+
+ private:
+  elem_t E;
+  move_t M;
+  size_t n;
+  std::array<Vector<S>, Dimensions> xs;
+  std::array<Vector<S>, Dimensions> u;
+  Vector<S> s, r;
+  std::optional<Vector<S>> w, e;
+};
+
+// -------------------------------------------------------------------------------- Surfaces
+
+template <class S>
+class Surfaces {
+ public:
+  // nodes (nx, ny) are taken as float like the reference's ElementPacket<float>; idx holds two
+  // node indices per panel; normal_bc is the per-panel normal-velocity boundary condition of a
+  // reactive surface (nullptr: zero). The reference always solves for vortex and source
+  // strengths (src/Surfaces.h:69); set_vortex_only() selects one unknown per panel.
+  Surfaces(size_t nnodes, const float* nx, const float* ny, size_t npanels, const Int* idx_in,
+           elem_t e, move_t m, std::shared_ptr<Body> body, const float* normal_bc = nullptr)
+      : E(e), M(m), B(std::move(body)), n(nnodes), np(npanels) {
+    for (size_t d = 0; d < Dimensions; ++d) x[d].assign(nnodes, S(0));
+    for (size_t i = 0; i < nnodes; ++i) {
+      x[0][i] = S(nx[i]);
+      x[1][i] = S(ny[i]);
+    }
+    ux = x;
+    idx.assign(idx_in, idx_in + 2 * np);
+    for (Int k : idx) assert(k < nnodes && "panel node index out of range");
+    compute_bases();
+    ps[0] = Vector<S>(np, S(0));
+    ps[1] = Vector<S>(np, S(0));
+    if (E == reactive) {
+      bc[0] = Vector<S>(np, S(0));
+      bc[1] = Vector<S>(np, S(0));
+      if (normal_bc)
+        for (size_t i = 0; i < np; ++i) (*bc[1])[i] = S(normal_bc[i]);
+    }
+    for (size_t d = 0; d < Dimensions; ++d) pu[d].assign(np, S(0));
+    if (B && M == bodybound) set_geom_center();
+  }
+
+  size_t get_n() const { return n; }
+  size_t get_npanels() const { return np; }
+  bool is_inert() const { return E == inert; }
+  move_t get_movet() const { return M; }
+  S get_vol() const { return vol; }
+  const std::array<Vector<S>, Dimensions>& get_pos() const { return x; }
+  const std::vector<Int>& get_idx() const { return idx; }
+  const std::array<Vector<S>, Dimensions>& get_tang() const { return b[0]; }
+  const std::array<Vector<S>, Dimensions>& get_norm() const { return b[1]; }
+  const Vector<S>& get_area() const { return area; }
+  const std::array<Vector<S>, Dimensions>& get_vel() const { return pu; }
+  std::array<Vector<S>, Dimensions>& get_vel() { return pu; }
+  const Vector<S>& get_str() const { return *ps[0]; }
+  Vector<S>& get_str() { return *ps[0]; }
+  bool have_src_str() const { return (bool)ps[1]; }
+  const Vector<S>& get_src_str() const { return *ps[1]; }
+  Vector<S>& get_src_str() { return *ps[1]; }
+  const Vector<S>& get_norm_bcs() const { return *bc[1]; }
+  const std::shared_ptr<Body>& get_body_ptr() const { return B; }
+
+  void set_vortex_only() { source_str_is_unknown = false; }
+  Int num_unknowns_per_panel() const { return source_str_is_unknown ? 2 : 1; }
+  void set_first_row(Int i) { istart = i; }
+  Int get_first_row() const { return istart; }
+  Int get_num_rows() const { return Int(np * num_unknowns_per_panel() + (is_augmented() ? 1 : 0)); }
+  Int get_next_row() const { return istart + get_num_rows(); }
+
+  // src/Surfaces.h:304-325
+  bool is_augmented() const {
+    bool aug = true;
+    if (B) {
+      if (B->get_name() == "ground" && vol < S(0)) aug = false;
+    } else {
+      aug = false;
+    }
+    if (E != reactive) aug = false;
+    return aug;
+  }
+
+  // src/Surfaces.h:266-301: pop the solved rotation rate, de-interleave [vortex, source]
+  void set_str(Vector<S> in) {
+    if (is_augmented()) {
+      solved_omega = in.back();
+      omega_error = solved_omega - S(B->get_rotvel());
+      in.pop_back();
+    }
+    assert(in.size() == np * num_unknowns_per_panel());
+    if (source_str_is_unknown) {
+      for (size_t i = 0; i < np; ++i) {
+        (*ps[0])[i] = in[2 * i];
+        (*ps[1])[i] = in[2 * i + 1];
+      }
+    } else {
+      *ps[0] = in;
+    }
+  }
+
+  void zero_strengths() {
+    for (auto& p : ps)
+      if (p) std::fill(p->begin(), p->end(), S(0));
+  }
+
+  // src/Surfaces.h:570-620, on the untransformed (here: unmoved) coordinates
+  void add_rot_strengths_base(S rotvel) {
+    for (size_t i = 0; i < np; ++i) {
+      const size_t j = idx[2 * i], jp1 = idx[2 * i + 1];
+      const S dx = S(0.5) * (ux[0][j] + ux[0][jp1]) - utc[0];
+      const S dy = S(0.5) * (ux[1][j] + ux[1][jp1]) - utc[1];
+      const S ui = -rotvel * dy;
+      const S vi = rotvel * dx;
+      S px = ux[0][jp1] - ux[0][j];
+      S py = ux[1][jp1] - ux[1][j];
+      const S oo = S(1.0) / area[i];
+      px *= oo;
+      py *= oo;
+      (*ps[0])[i] += S(-1.0) * (ui * px + vi * py);
+      (*ps[1])[i] += S(-1.0) * (ui * py - vi * px);
+    }
+  }
+  void add_unit_rot_strengths() { add_rot_strengths_base(S(1)); }
+  void add_solved_rot_strengths(S factor) {
+    if (is_augmented()) add_rot_strengths_base(factor * solved_omega);
+    else if (B) add_rot_strengths_base(factor * S(B->get_rotvel()));
+  }
+
+  void zero_vels() {
+    for (size_t d = 0; d < Dimensions; ++d) std::fill(pu[d].begin(), pu[d].end(), S(0));
+  }
+  // src/Surfaces.h:764-774
+  void finalize_vels(const std::array<double, Dimensions>& fs) {
+    const double factor = 0.5 / M_PI;
+    for (size_t d = 0; d < Dimensions; ++d)
+      for (size_t i = 0; i < np; ++i) pu[d][i] = S(fs[d] + double(pu[d][i]) * factor);
+  }
+
+  // src/Surfaces.h:830-878: points just off the panels, rounded to float as the reference
+  // stores them; strengths (sheet strength + vortex BC) times panel length
+  void represent_as_particles(S offset, S ips, std::vector<S>& px, std::vector<S>& py,
+                              std::vector<S>& pstr) const {
+    px.clear();
+    py.clear();
+    pstr.clear();
+    const S thisips = ips < S(0) ? area[0] : ips;
+    for (size_t i = 0; i < np; ++i) {
+      const Int id0 = idx[2 * i], id1 = idx[2 * i + 1];
+      float this_str = float((*ps[0])[i]);
+      if (E == reactive) this_str = float(S(this_str) + (*bc[0])[i]);
+      this_str = float(S(this_str) * area[i]);
+      const size_t nshed = 1 + size_t(std::max(0, int(area[i] / (S(1.2) * thisips))));
+      for (size_t j = 0; j < nshed; ++j) {
+        const S wgt = S(1 + 2 * j) / S(2 * nshed);
+        px.push_back(S(float(wgt * x[0][id1] + (S(1.0) - wgt) * x[0][id0] + offset * b[1][0][i])));
+        py.push_back(S(float(wgt * x[1][id1] + (S(1.0) - wgt) * x[1][id0] + offset * b[1][1][i])));
+        pstr.push_back(S(this_str / float(nshed)));
+      }
+    }
+  }
+
+  S get_last_body_circ() const { return S(2.0) * vol * this_omega; }
+  S get_last_body_circ_error() const { return S(2.0 * double(vol) * double(omega_error)); }
+  S get_reabsorbed() const { return reabsorbed_gamma; }
+  S get_solved_omega() const { return solved_omega; }
+
+  std::string to_string() const {
+    std::string m = E == active ? " Active" : (E == reactive ? " Reactive" : " Inert");
+    m += M == lagrangian ? " Lagrangian" : (M == bodybound ? " Body-fixed" : " Fixed");
+    return m + " Surfaces";
+  }
+
+ private:
+  // src/Surfaces.h:682-725
+  void compute_bases() {
+    for (size_t i = 0; i < Dimensions; ++i)
+      for (size_t j = 0; j < Dimensions; ++j) b[i][j].assign(np, S(0));
+    area.assign(np, S(0));
+    for (size_t i = 0; i < np; ++i) {
+      const size_t a = idx[2 * i], c = idx[2 * i + 1];
+      S t0 = x[0][c] - x[0][a], t1 = x[1][c] - x[1][a];
+      const S base = std::sqrt(t0 * t0 + t1 * t1);
+      t0 *= (S(1.0) / base);
+      t1 *= (S(1.0) / base);
+      area[i] = base;
+      b[0][0][i] = t0;
+      b[0][1][i] = t1;
+      b[1][0][i] = -t1;
+      b[1][1][i] = t0;
+    }
+  }
+
+  // src/Surfaces.h:623-677: signed area and centroid from the fan of triangles at the origin
+  void set_geom_center() {
+    double asum = 0.0, xsum = 0.0, ysum = 0.0;
+    for (size_t i = 0; i < np; ++i) {
+      const size_t j = idx[2 * i], jp1 = idx[2 * i + 1];
+      const double x0 = ux[0][j], y0 = ux[1][j], x1 = ux[0][jp1], y1 = ux[1][jp1];
+      const double xc = (0.0 + x0 + x1) / 3.0, yc = (0.0 + y0 + y1) / 3.0;
+      const double px = x1 - x0, py = y1 - y0;
+      const double a = std::sqrt(std::pow(x0, 2) + std::pow(y0, 2));
+      const double bb = std::sqrt(std::pow(px, 2) + std::pow(py, 2));
+      const double c = std::sqrt(std::pow(x1, 2) + std::pow(y1, 2));
+      const double hs = 0.5 * (a + bb + c);
+      const double asq = hs * (hs - a) * (hs - bb) * (hs - c);
+      double ar = asq > 0.0 ? std::sqrt(asq) : 0.0;
+      if (y0 * px - x0 * py < 0.0) ar = -ar;
+      asum += ar;
+      xsum += xc * ar;
+      ysum += yc * ar;
+    }
+    vol = S(asum);
+    utc[0] = S(xsum / asum);
+    utc[1] = S(ysum / asum);
+  }
+
+  elem_t E;
+  move_t M;
+  std::shared_ptr<Body> B;
+  size_t n, np;
+  std::array<Vector<S>, Dimensions> x, ux;
+  std::vector<Int> idx;
+  std::array<std::array<Vector<S>, Dimensions>, Dimensions> b;
+  Vector<S> area;
+  std::array<std::optional<Vector<S>>, 2> ps, bc;
+  std::array<Vector<S>, Dimensions> pu;
+  bool source_str_is_unknown = true;
+  Int istart = 0;
+  S vol = S(-1.0);
+  std::array<S, Dimensions> utc = {S(0), S(0)};
+  S solved_omega = S(0), omega_error = S(0), this_omega = S(0), reabsorbed_gamma = S(0);
+};
+
+}  // namespace omega2d

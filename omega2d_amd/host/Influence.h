@@ -1,0 +1,173 @@
+// Influence.h - the *_affect_* operators of the gpu_hip execution environment: host mirror of
+// src/Influence.h:56-901 (same signatures, accumulate semantics and per-call log line), each
+// forwarding to the C ABI in include/omega2d_hip.h. Only the gpu_hip accelerator is
+// implemented here; the CPU environments stay in the reference.
+#pragma once
+
+#include <chrono>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <variant>
+#include <vector>
+
+#include "Elements.h"
+#include "ExecEnv.h"
+#include "omega2d_hip.h"
+
+namespace omega2d {
+
+using Clock = std::chrono::steady_clock;
+
+inline void o2d_check(int rc, const char* what) {
+  if (rc != O2D_OK)
+    throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) +
+                             "): " + o2d_hip_last_error_string());
+}
+
+inline void require_gpu_hip(const ExecEnv& env) {
+  if (env.get_instrs() != gpu_hip || !env.is_internal())
+    throw std::runtime_error("this build implements the gpu_hip environment only (got" +
+                             env.to_string() + ")");
+}
+
+inline void log_rate(const char* name, Clock::time_point start, double flops) {
+  const double s = std::chrono::duration<double>(Clock::now() - start).count();
+  std::printf("    %s: [%.4f] seconds at %.3f GFlop/s\n", name, (float)s,
+              (float)(1.e-9 * flops / s));
+}
+
+template <class S>
+std::vector<double> as_double(const std::vector<S>& v) {
+  return std::vector<double>(v.begin(), v.end());
+}
+
+// ---------------------------------------------------------------------- points on points
+
+template <class S, class A>
+void points_affect_points(const Points<S>& src, Points<S>& targ, const ResultsType& restype,
+                          const ExecEnv& env) {
+  require_gpu_hip(env);
+  if (restype.compute_psi() || restype.compute_grad())
+    throw std::runtime_error("points cannot compute streamfunction or velocity gradients");
+  const auto start = Clock::now();
+  const o2d_env e = env.c_env(restype.get_type());
+  const auto& sx = src.get_pos();
+  const auto& tx = targ.get_pos();
+  auto& tu = targ.get_vel();
+  S* tw = restype.compute_vort() ? targ.get_vort().data() : nullptr;
+  S* te = restype.compute_shear() ? targ.get_shear().data() : nullptr;
+  const S* tr = targ.is_inert() ? nullptr : targ.get_rad().data();
+  int rc;
+  if constexpr (std::is_same_v<S, float>) {
+    rc = o2d_hip_points_on_points_f(&e, (int64_t)src.get_n(), sx[0].data(), sx[1].data(),
+                                    src.get_str().data(), src.get_rad().data(),
+                                    (int64_t)targ.get_n(), tx[0].data(), tx[1].data(), tr,
+                                    tu[0].data(), tu[1].data(), tw, te);
+  } else {
+    static_assert(std::is_same_v<S, double>, "storage type must be float or double");
+    rc = o2d_hip_points_on_points_d(&e, (int64_t)src.get_n(), sx[0].data(), sx[1].data(),
+                                    src.get_str().data(), src.get_rad().data(),
+                                    (int64_t)targ.get_n(), tx[0].data(), tx[1].data(), tr,
+                                    tu[0].data(), tu[1].data(), tw, te);
+  }
+  o2d_check(rc, "o2d_hip_points_on_points");
+  const int per = o2d_hip_flops_per_pair(env.get_core(), targ.is_inert() ? 1 : 0, 0);
+  log_rate("points_affect_points", start,
+           (double)targ.get_n() * (2.0 + per * (double)src.get_n()));
+}
+
+// ---------------------------------------------------------------------- panels on points
+
+template <class S, class A>
+void panels_affect_points(const Surfaces<S>& src, Points<S>& targ, const ResultsType& restype,
+                          const ExecEnv& env) {
+  require_gpu_hip(env);
+  if (!restype.compute_vel() || restype.get_type() != velonly)
+    throw std::runtime_error("surfaces compute velocity only");
+  const auto start = Clock::now();
+  const o2d_env e = env.c_env(velonly);
+  const auto nx = as_double(src.get_pos()[0]), ny = as_double(src.get_pos()[1]);
+  const auto vs = as_double(src.get_str());
+  const auto ss = as_double(src.get_src_str());
+  const auto tx = as_double(targ.get_pos()[0]), ty = as_double(targ.get_pos()[1]);
+  std::vector<double> tu(targ.get_n(), 0.0), tv(targ.get_n(), 0.0);
+  o2d_check(o2d_hip_panels_on_points_d(&e, (int64_t)src.get_n(), nx.data(), ny.data(),
+                                       (int64_t)src.get_npanels(), src.get_idx().data(),
+                                       vs.data(), src.have_src_str() ? ss.data() : nullptr,
+                                       (int64_t)targ.get_n(), tx.data(), ty.data(), tu.data(),
+                                       tv.data()),
+            "o2d_hip_panels_on_points");
+  auto& u = targ.get_vel();
+  for (size_t i = 0; i < targ.get_n(); ++i) {
+    u[0][i] = S(double(u[0][i]) + tu[i]);
+    u[1][i] = S(double(u[1][i]) + tv[i]);
+  }
+  log_rate("panels_affect_points", start,
+           (double)targ.get_n() *
+               (2.0 + o2d_hip_flops_per_pair(env.get_core(), 1, src.have_src_str() ? 2 : 1) *
+                          (double)src.get_npanels()));
+}
+
+// ---------------------------------------------------------------------- points on panels
+
+template <class S, class A>
+void points_affect_panels(const Points<S>& src, Surfaces<S>& targ, const ResultsType& restype,
+                          const ExecEnv& env) {
+  require_gpu_hip(env);
+  if (restype.get_type() != velonly) throw std::runtime_error("panels receive velocity only");
+  const auto start = Clock::now();
+  const o2d_env e = env.c_env(velonly);
+  const auto sx = as_double(src.get_pos()[0]), sy = as_double(src.get_pos()[1]);
+  const auto ss = as_double(src.get_str());
+  const auto nx = as_double(targ.get_pos()[0]), ny = as_double(targ.get_pos()[1]);
+  std::vector<double> pu(targ.get_npanels(), 0.0), pv(targ.get_npanels(), 0.0);
+  o2d_check(o2d_hip_points_on_panels_d(&e, (int64_t)src.get_n(), sx.data(), sy.data(), ss.data(),
+                                       (int64_t)targ.get_n(), nx.data(), ny.data(),
+                                       (int64_t)targ.get_npanels(), targ.get_idx().data(),
+                                       pu.data(), pv.data()),
+            "o2d_hip_points_on_panels");
+  auto& u = targ.get_vel();
+  for (size_t i = 0; i < targ.get_npanels(); ++i) {
+    u[0][i] = S(double(u[0][i]) + pu[i]);
+    u[1][i] = S(double(u[1][i]) + pv[i]);
+  }
+  log_rate("points_affect_panels", start,
+           (double)targ.get_npanels() * (11.0 + 39.0 * (double)src.get_n()));
+}
+
+// ---------------------------------------------------------------------- panels on panels
+
+// src/Influence.h:774-795: panels_affect_points at points just off the target panels, added
+// (first npanels values) into the target panel velocities
+template <class S, class A>
+void panels_affect_panels(const Surfaces<S>& src, Surfaces<S>& targ, const ResultsType& restype,
+                          const ExecEnv& env) {
+  std::vector<S> px, py, pstr;
+  targ.represent_as_particles(S(0.0001), S(-1.0), px, py, pstr);
+  std::vector<S> rad(px.size(), S(0.0001));
+  Points<S> pts(px.size(), px.data(), py.data(), pstr.data(), rad.data(), active, lagrangian);
+  panels_affect_points<S, A>(src, pts, restype, env);
+  auto& tov = targ.get_vel();
+  const auto& fromv = pts.get_vel();
+  for (size_t d = 0; d < Dimensions; ++d)
+    for (size_t i = 0; i < targ.get_npanels() && i < pts.get_n(); ++i) tov[d][i] += fromv[d][i];
+}
+
+// ---------------------------------------------------------------------- dispatch
+
+template <class S>
+using Collection = std::variant<Points<S>, Surfaces<S>>;
+
+template <class S, class A>
+struct InfluenceVisitor {
+  void operator()(const Points<S>& s, Points<S>& t) { points_affect_points<S, A>(s, t, restype, env); }
+  void operator()(const Surfaces<S>& s, Points<S>& t) { panels_affect_points<S, A>(s, t, restype, env); }
+  void operator()(const Points<S>& s, Surfaces<S>& t) { points_affect_panels<S, A>(s, t, restype, env); }
+  void operator()(const Surfaces<S>& s, Surfaces<S>& t) { panels_affect_panels<S, A>(s, t, restype, env); }
+  ResultsType restype;
+  ExecEnv env;
+};
+
+}  // namespace omega2d

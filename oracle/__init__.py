@@ -114,6 +114,7 @@ class _Quiet:
         return self
 
     def __exit__(self, *exc):
+        ctypes.CDLL(None).fflush(None)  # C stdio buffers still hold the reference's log lines
         os.dup2(self._saved, 1)
         os.close(self._saved)
         os.close(self._null)

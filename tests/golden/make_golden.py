@@ -1,0 +1,106 @@
+"""Write tests/golden/reference_vectors.npz from the reference's own templates.
+
+Runs in a container that has the reference tree: it calls oracle/_ref/libo2d_ref.so (built by
+``make -C oracle ref`` from src/Influence.h, src/Coefficients.h, src/RHS.h, src/Points.h,
+src/Surfaces.h, src/ElementBase.h where they lie) on small seeded inputs and stores inputs and
+outputs. The fixture is data only: inputs and the reference's outputs.
+
+    python tests/golden/make_golden.py
+
+Core function: the reference compiles in Vatistas n=2 (src/CoreFunc.h:38), so every
+particle-on-point vector uses that core.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import oracle as O  # noqa: E402
+from helpers import circle_panels  # noqa: E402
+
+OUT = os.path.join(HERE, "reference_vectors.npz")
+
+
+def main():
+    if not O.have_ref():
+        raise SystemExit("oracle/_ref/libo2d_ref.so missing: run make -C oracle ref")
+    rng = np.random.default_rng(20240601)
+    d = {}
+
+    # particles on points, S=double A=double and S=float A=double
+    ns, nt = 512, 300
+    sx, sy = rng.uniform(-1, 1, ns), rng.uniform(-1, 1, ns)
+    ss, sr = rng.normal(0, 1, ns), rng.uniform(0.02, 0.1, ns)
+    tx, ty = rng.uniform(-1.2, 1.2, nt), rng.uniform(-1.2, 1.2, nt)
+    tr = rng.uniform(0.02, 0.1, nt)
+    d.update(pp_sx=sx, pp_sy=sy, pp_ss=ss, pp_sr=sr, pp_tx=tx, pp_ty=ty, pp_tr=tr)
+    for res in ("vel", "vort", "shear"):
+        for tag, trr in (("0p", None), ("0b", tr)):
+            out = O.ref_points_on_points(sx, sy, ss, sr, tx, ty, trr, res)
+            for k, a in zip("uvw", out):
+                d[f"pp_{tag}_{res}_{k}"] = a
+    # self interaction (targets are the sources), vorticity included
+    out = O.ref_points_on_points(sx, sy, ss, sr, sx, sy, sr, "vort")
+    d["pp_self_u"], d["pp_self_v"], d["pp_self_w"] = out
+    # float storage (the application's STORE=float)
+    f = [a.astype(np.float32) for a in (sx, sy, ss, sr, tx, ty, tr)]
+    fu, fv = O.ref_points_on_points(*f, "vel", dtype=np.float32)
+    d["ppf_0b_u"], d["ppf_0b_v"] = fu, fv
+
+    # panels on points (vortex + source panels), points on panels
+    npan = 64
+    nx, ny, idx = circle_panels(npan)
+    d.update(pan_nx=nx, pan_ny=ny, pan_idx=idx)
+    vs, sst = rng.normal(0, 1, npan), rng.normal(0, 1, npan)
+    px, py = rng.uniform(-1.5, 1.5, 400), rng.uniform(-1.5, 1.5, 400)
+    d.update(pan_vs=vs, pan_ss=sst, pan_tx=px, pan_ty=py)
+    d["pan_u"], d["pan_v"] = O.ref_panels_on_points(nx, ny, idx, vs, sst, px, py)
+    qx, qy = rng.uniform(-1.5, 1.5, 500), rng.uniform(-1.5, 1.5, 500)
+    qs, qr = rng.normal(0, 1, 500), np.full(500, 0.01)
+    d.update(ptp_sx=qx, ptp_sy=qy, ptp_ss=qs, ptp_sr=qr)
+    d["ptp_u"], d["ptp_v"] = O.ref_points_on_panels(qx, qy, qs, qr, nx, ny, idx)
+
+    # BEM self blocks: vortex-only and vortex+source, no body / ground body / moving body
+    bnp = 32
+    bx, by, bidx = circle_panels(bnp)
+    d.update(bem_nx=bx, bem_ny=by, bem_idx=bidx)
+    for nunk in (1, 2):
+        for kind in (0, 1, 2):
+            R = O.ref_bem_coeffs(bx, by, bidx, nunk, kind)
+            for k in ("A", "tang_x", "tang_y", "area", "urot_u", "urot_v"):
+                d[f"bem_{nunk}_{kind}_{k}"] = R[k]
+            d[f"bem_{nunk}_{kind}_vol"] = np.array([R["vol"]])
+    # internal flow (reversed panel order): negative volume, the ground body is not augmented
+    ix, iy, iidx = circle_panels(bnp, external=False)
+    R = O.ref_bem_coeffs(ix, iy, iidx, 2, 1)
+    d["bem_int_idx"], d["bem_int_A"], d["bem_int_vol"] = iidx, R["A"], np.array([R["vol"]])
+
+    # RHS from panel velocities
+    pu, pv = rng.normal(0, 1, bnp), rng.normal(0, 1, bnp)
+    nbc = rng.normal(0, 0.1, bnp).astype(np.float32)
+    d.update(rhs_pu=pu, rhs_pv=pv, rhs_nbc=nbc)
+    for nunk in (1, 2):
+        d[f"rhs_{nunk}"] = O.ref_rhs_panels(bx, by, bidx, nunk, pu, pv, nbc)
+
+    # finalize and RK moves
+    n = 257
+    x, y = rng.normal(0, 1, n), rng.normal(0, 1, n)
+    us = [rng.normal(0, 1, n) for _ in range(3)]
+    vv = [rng.normal(0, 1, n) for _ in range(3)]
+    d.update(mv_x=x, mv_y=y, mv_u1=us[0], mv_u2=us[1], mv_u3=us[2], mv_v1=vv[0], mv_v2=vv[1],
+             mv_v3=vv[2])
+    d["mv2_x"], d["mv2_y"] = O.ref_move(x, y, us[:2], vv[:2], [0.25, 0.75], 0.01)
+    d["mv3_x"], d["mv3_y"] = O.ref_move(x, y, us, vv, [2 / 9, 3 / 9, 4 / 9], 0.01)
+    d["fin_u"], d["fin_v"] = O.ref_finalize(us[0], vv[0], (1.0, -0.5))
+
+    np.savez_compressed(OUT, **d)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(d)} arrays)")
+
+
+if __name__ == "__main__":
+    main()

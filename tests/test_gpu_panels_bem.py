@@ -1,0 +1,106 @@
+"""GPU parity of the panel kernels, BEM assembly/solve, RK update and adjoint vs the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import omega2d_amd as o2d
+from helpers import circle_panels, particles, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("with_source", [False, True])
+@pytest.mark.parametrize("npan", [5, 64, 300])
+def test_panels_on_points(with_source, npan):
+    rng = np.random.default_rng(npan)
+    nx, ny, idx = circle_panels(npan)
+    nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+    vs = rng.normal(size=npan)
+    ss = rng.normal(size=npan) if with_source else None
+    tx, ty, _, _ = particles(rng, 3001, lo=-2.0, hi=2.0)
+    got = o2d.panels_on_points(nx, ny, idx, vs, ss, tx, ty)
+    ref = O.panels_on_points(nx, ny, idx, vs, ss, tx, ty)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= 1e-12
+
+
+@pytest.mark.parametrize("npan", [7, 128, 1000])
+@pytest.mark.parametrize("ns", [1, 300, 20000])
+def test_points_on_panels(npan, ns):
+    rng = np.random.default_rng(npan + ns)
+    nx, ny, idx = circle_panels(npan)
+    nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+    r = rng.uniform(0.55, 2.5, ns)
+    th = rng.uniform(0, 2 * np.pi, ns)
+    sx, sy, ss = r * np.cos(th), r * np.sin(th), rng.normal(size=ns)
+    got = o2d.points_on_panels(sx, sy, ss, nx, ny, idx)
+    ref = O.points_on_panels(sx, sy, ss, nx, ny, idx)
+    for g, rr in zip(got, ref):
+        assert rel_l2(g, rr) <= 1e-12
+
+
+@pytest.mark.parametrize("nunk", [1, 2])
+@pytest.mark.parametrize("augmented", [False, True])
+@pytest.mark.parametrize("npan", [16, 257])
+def test_bem_assemble(nunk, augmented, npan):
+    nx, ny, idx = circle_panels(npan)
+    nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+    rng = np.random.default_rng(npan)
+    uu, uv = rng.normal(size=npan), rng.normal(size=npan)
+    vol, _, _ = O.body_volume(nx, ny, idx)
+    got = o2d.bem_assemble(nx, ny, idx, nunk, augmented, uu, uv, vol)
+    ref = O.bem_assemble(nx, ny, idx, nunk, augmented, uu, uv, vol)
+    assert got.shape == ref.shape
+    assert rel_l2(got, ref) <= 1e-12
+    assert np.max(np.abs(got - ref)) <= 1e-13
+
+
+@pytest.mark.parametrize("nunk", [1, 2])
+def test_bem_factor_solve_matches_lapack(nunk):
+    npan = 200
+    nx, ny, idx = circle_panels(npan)
+    nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+    tx, ty, tnx, tny, area = O.panel_geometry(nx, ny, idx)
+    vol, cx, cy = O.body_volume(nx, ny, idx)
+    vs, ss = O.unit_rot_strengths(nx, ny, idx, cx, cy)
+    A = O.bem_assemble(nx, ny, idx, nunk, True, np.zeros(npan), np.zeros(npan), vol)
+    rng = np.random.default_rng(1)
+    b = rng.normal(size=A.shape[0])
+    o2d.bem_factor(A)
+    x = o2d.bem_solve(b)
+    xr = np.linalg.solve(A, b)
+    assert rel_l2(x, xr) <= 1e-12
+    # solve again with the same factors (the reference re-uses its solver until A changes)
+    x2 = o2d.bem_solve(2.0 * b)
+    assert rel_l2(x2, 2.0 * xr) <= 1e-12
+
+
+def test_bem_solve_without_factor_of_that_size_fails():
+    o2d.bem_factor(np.eye(3) * 2.0)
+    with pytest.raises(o2d.O2DError):
+        o2d.bem_solve(np.ones(5))
+
+
+@pytest.mark.parametrize("nstage,w", [(1, [1.0]), (2, [0.25, 0.75]), (3, [2 / 9, 3 / 9, 4 / 9])])
+def test_rk_move_bit_exact(nstage, w):
+    rng = np.random.default_rng(nstage)
+    n = 10007
+    x, y = rng.normal(size=n), rng.normal(size=n)
+    us = [rng.normal(size=n) for _ in range(nstage)]
+    vs = [rng.normal(size=n) for _ in range(nstage)]
+    gx, gy = o2d.rk_move(x.copy(), y.copy(), us, vs, w, 0.01)
+    rx, ry = O.rk_combine(x, y, us, vs, w, 0.01)
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+
+
+@pytest.mark.parametrize("core", ["rm", "gauss", "wl", "v2", "v3"])
+def test_adjoint_matches_oracle(core):
+    rng = np.random.default_rng(4)
+    sx, sy, ss, sr = particles(rng, 3000, rad=(0.05, 0.3))
+    th = 2 * np.pi * np.arange(500) / 500
+    tx, ty = 0.8 * np.cos(th), 0.8 * np.sin(th)
+    gu, gv = rng.normal(size=500), rng.normal(size=500)
+    got = o2d.points_on_points_adjoint(sx, sy, ss, sr, tx, ty, gu, gv, core=core)
+    ref = O.points_on_points_adjoint(core, sx, sy, ss, sr, tx, ty, gu, gv)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= 1e-12
