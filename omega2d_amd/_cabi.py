@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libomega2d_hip.so")
+# O2D_HIP_LIB selects another build of the same library (used for launch-shape tuning)
+LIB_PATH = os.environ.get("O2D_HIP_LIB") or os.path.join(_HERE, "lib", "libomega2d_hip.so")
 
 # values from include/omega2d_hip.h
 O2D_OK = 0

@@ -29,8 +29,18 @@ namespace {
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kTile = 256;   // sources per LDS stage
 constexpr int kGroup = 64;   // sources sharing one centroid in the split form
-constexpr int kTpl64 = 2;    // targets per lane, plain form
-constexpr int kTpl32 = 2;    // targets per lane, split form
+// launch shape, tuned on MI355X (overridable at build time for tuning sweeps)
+#ifndef O2D_TPL64
+#define O2D_TPL64 2
+#endif
+#ifndef O2D_TPL32
+#define O2D_TPL32 2
+#endif
+#ifndef O2D_WAVES32
+#define O2D_WAVES32 8
+#endif
+constexpr int kTpl64 = O2D_TPL64;  // targets per lane, plain form
+constexpr int kTpl32 = O2D_TPL32;  // targets per lane, split form
 
 struct alignas(16) Src64 {
   double x, y, s, w;
@@ -132,17 +142,52 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
 
 // ---------------------------------------------------------------------------- split fp32
 
-// sources prepared by split_prep: fp32 (x - c, y - c, strength, radius term) in Morton order,
-// padded to whole tiles; cent[g] is the fp64 centroid of group g
+// One source group: fp64 centroid, the largest fp32 offset length of its sources from it, and
+// their largest radius term (rterm units of the core).
+struct alignas(16) GroupInfo {
+  double cx, cy;
+  float rad, wmax, pad0, pad1;
+};
+
+// pair loop over one group; CHECKED = false drops the Gaussian's in-core test (the caller has
+// shown that every pair of this group is beyond the compact cutoff, where K = 1/d^2)
+template <int CORE, bool TR, int TPL, int UNROLL, bool CHECKED>
+__device__ __forceinline__ void group_pairs(const float4* __restrict__ q0, const float* ex,
+                                            const float* ey, const float* pw, float* fu,
+                                            float* fv) {
+#pragma unroll UNROLL
+  for (int k = 0; k < kGroup; ++k) {
+    const float4 q = q0[k];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      const float dx = ex[t] - q.x;
+      const float dy = ey[t] - q.y;
+      float kk;
+      if constexpr (CORE == O2D_CORE_GAUSSIAN && !CHECKED) {
+        kk = __builtin_amdgcn_rcpf(__builtin_fmaf(dx, dx, dy * dy));
+      } else {
+        const float w = TR ? q.w + pw[t] : q.w;
+        kk = core_k32<CORE>(dx, dy, w);
+      }
+      const float gk = q.z * kk;
+      fu[t] = __builtin_fmaf(-gk, dy, fu[t]);
+      fv[t] = __builtin_fmaf(gk, dx, fv[t]);
+    }
+  }
+}
+
+// sources prepared by the split prep: fp32 (x - c, y - c, strength, radius term) in Morton
+// order, padded to whole tiles; grp[g] describes group g
 template <int CORE, bool TR, int TPL, int UNROLL>
-__global__ __launch_bounds__(kBlock) void nbody32_kernel(
-    const float4* __restrict__ src, const double2* __restrict__ cent, int64_t ntiles,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(O2D_WAVES32, 8)))
+void nbody32_kernel(
+    const float4* __restrict__ src, const GroupInfo* __restrict__ grp, int64_t ntiles,
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty,
     const double* __restrict__ tr, int64_t tiles_per_chunk, double* __restrict__ ou,
     double* __restrict__ ov, int direct) {
   constexpr int kGroups = kTile / kGroup;
   __shared__ float4 tile[kTile];
-  __shared__ double2 tcent[kGroups];
+  __shared__ GroupInfo tgrp[kGroups];
   const int tid = threadIdx.x;
   const int64_t tbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
 
@@ -162,35 +207,38 @@ __global__ __launch_bounds__(kBlock) void nbody32_kernel(
   const int64_t te = tb + tiles_per_chunk < ntiles ? tb + tiles_per_chunk : ntiles;
   for (int64_t it = tb; it < te; ++it) {
     const float4 s = src[it * kTile + tid];
-    double2 c;
-    if (tid < kGroups) c = cent[it * kGroups + tid];
     __syncthreads();
     tile[tid] = s;
-    if (tid < kGroups) tcent[tid] = c;
+    if (tid < kGroups) tgrp[tid] = grp[it * kGroups + tid];
     __syncthreads();
 #pragma unroll 1
     for (int g = 0; g < kGroups; ++g) {
-      const double2 cg = tcent[g];
+      const GroupInfo cg = tgrp[g];
       float ex[TPL], ey[TPL], fu[TPL], fv[TPL];
 #pragma unroll
       for (int t = 0; t < TPL; ++t) {
-        ex[t] = (float)(px[t] - cg.x);
-        ey[t] = (float)(py[t] - cg.y);
+        ex[t] = (float)(px[t] - cg.cx);
+        ey[t] = (float)(py[t] - cg.cy);
         fu[t] = 0.0f;
         fv[t] = 0.0f;
       }
-#pragma unroll UNROLL
-      for (int k = 0; k < kGroup; ++k) {
-        const float4 q = tile[g * kGroup + k];
+      if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+        // every pair of the group is outside the compact core (d^2 > 16 w) when the target's
+        // distance to the centroid exceeds the group radius plus 4 sqrt(wmax + w_target);
+        // margins of 1e-5 absorb the fp32 rounding of the offsets and of this test
+        bool far = true;
 #pragma unroll
         for (int t = 0; t < TPL; ++t) {
-          const float dx = ex[t] - q.x;
-          const float dy = ey[t] - q.y;
-          const float w = TR ? q.w + pw[t] : q.w;
-          const float gk = q.z * core_k32<CORE>(dx, dy, w);
-          fu[t] = __builtin_fmaf(-gk, dy, fu[t]);
-          fv[t] = __builtin_fmaf(gk, dx, fv[t]);
+          const float lim = cg.rad + 4.0f * __builtin_sqrtf(cg.wmax + pw[t]);
+          const float e2 = __builtin_fmaf(ex[t], ex[t], ey[t] * ey[t]);
+          far = far && (e2 * (1.0f - 1e-5f) > lim * lim * (1.0f + 1e-5f));
         }
+        if (__all(far))
+          group_pairs<CORE, TR, TPL, 16, false>(tile + g * kGroup, ex, ey, pw, fu, fv);
+        else
+          group_pairs<CORE, TR, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv);
+      } else {
+        group_pairs<CORE, TR, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv);
       }
 #pragma unroll
       for (int t = 0; t < TPL; ++t) {
@@ -364,17 +412,22 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(int64_t n, const double*
   order[i] = (uint32_t)i;
 }
 
-// one thread per group: fp64 centroid of the group's real sources in sorted order
-__global__ __launch_bounds__(kBlock) void centroid_kernel(int64_t ns, int64_t ngroups,
-                                                          const uint32_t* __restrict__ order,
-                                                          const double* __restrict__ sx,
-                                                          const double* __restrict__ sy,
-                                                          double2* __restrict__ cent) {
+// one thread per group of the sorted sources: fp64 centroid of the group's real sources, then
+// the largest length of their fp32 offsets (as the rebase kernel rounds them) and the largest
+// radius term; padding sources (zero strength, offset 0) need no entry
+template <int CORE>
+__global__ __launch_bounds__(kBlock) void group_kernel(int64_t ns, int64_t ngroups,
+                                                       const uint32_t* __restrict__ order,
+                                                       const double* __restrict__ sx,
+                                                       const double* __restrict__ sy,
+                                                       const double* __restrict__ sr,
+                                                       GroupInfo* __restrict__ grp) {
   const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (g >= ngroups) return;
   const int64_t b = g * kGroup;
   const int64_t e = b + kGroup < ns ? b + kGroup : ns;
   double cx = 0.0, cy = 0.0;
+  float rad = 0.0f, wmax = 0.0f;
   if (e > b) {
     for (int64_t j = b; j < e; ++j) {
       const uint32_t o = order[j];
@@ -384,8 +437,20 @@ __global__ __launch_bounds__(kBlock) void centroid_kernel(int64_t ns, int64_t ng
     const double inv = 1.0 / (double)(e - b);
     cx *= inv;
     cy *= inv;
+    for (int64_t j = b; j < e; ++j) {
+      const uint32_t o = order[j];
+      const float ox = (float)(sx[o] - cx), oy = (float)(sy[o] - cy);
+      rad = fmaxf(rad, __builtin_sqrtf(__builtin_fmaf(ox, ox, oy * oy)));
+      wmax = fmaxf(wmax, (float)rterm<CORE>(sr[o]));
+    }
   }
-  cent[g] = make_double2(cx, cy);
+  GroupInfo gi;
+  gi.cx = cx;
+  gi.cy = cy;
+  gi.rad = rad;
+  gi.wmax = wmax;
+  gi.pad0 = gi.pad1 = 0.0f;
+  grp[g] = gi;
 }
 
 template <int CORE>
@@ -395,14 +460,14 @@ __global__ __launch_bounds__(kBlock) void rebase_kernel(int64_t ns, int64_t npad
                                                         const double* __restrict__ sy,
                                                         const double* __restrict__ ss,
                                                         const double* __restrict__ sr,
-                                                        const double2* __restrict__ cent,
+                                                        const GroupInfo* __restrict__ grp,
                                                         float4* __restrict__ out) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= npad) return;
   if (j < ns) {
     const uint32_t o = order[j];
-    const double2 c = cent[j / kGroup];
-    out[j] = make_float4((float)(sx[o] - c.x), (float)(sy[o] - c.y), (float)ss[o],
+    const GroupInfo& c = grp[j / kGroup];
+    out[j] = make_float4((float)(sx[o] - c.cx), (float)(sy[o] - c.cy), (float)ss[o],
                          (float)rterm<CORE>(sr[o]));
   } else {
     out[j] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
@@ -456,24 +521,26 @@ void dispatch64(int res, bool tr_present, dim3 grid, hipStream_t s, int64_t ns, 
 }
 
 template <int CORE>
-void dispatch32(bool tr_present, dim3 grid, hipStream_t s, const float4* src, const double2* cent,
+void dispatch32(bool tr_present, dim3 grid, hipStream_t s, const float4* src, const GroupInfo* grp,
                 int64_t ntiles, int64_t nt, const double* tx, const double* ty, const double* tr,
                 int64_t per, double* ou, double* ov, int direct) {
-  // the Gaussian's in-core branch makes a deep unroll expensive in registers
-  constexpr int kUnroll = CORE == O2D_CORE_GAUSSIAN ? 4 : 16;
+  constexpr int kUnroll = 16;
   if (tr_present)
     nbody32_kernel<CORE, true, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
-        src, cent, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
+        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
   else
     nbody32_kernel<CORE, false, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
-        src, cent, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
+        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
 }
 
 template <int CORE>
-void launch_rebase(dim3 grid, hipStream_t s, int64_t ns, int64_t npad, const uint32_t* order,
-                   const double* sx, const double* sy, const double* ss, const double* sr,
-                   const double2* cent, float4* out) {
-  rebase_kernel<CORE><<<grid, kBlock, 0, s>>>(ns, npad, order, sx, sy, ss, sr, cent, out);
+void launch_prep(int64_t ngroups, int64_t npad, hipStream_t s, int64_t ns, const uint32_t* order,
+                 const double* sx, const double* sy, const double* ss, const double* sr,
+                 GroupInfo* grp, float4* out) {
+  group_kernel<CORE><<<(unsigned)((ngroups + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      ns, ngroups, order, sx, sy, sr, grp);
+  rebase_kernel<CORE><<<(unsigned)((npad + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      ns, npad, order, sx, sy, ss, sr, grp, out);
 }
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
@@ -562,20 +629,19 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   O2D_TRY(slot(d, kKeysB, ns * sizeof(uint32_t), s, &kB));
   O2D_TRY(slot(d, kOrderB, ns * sizeof(uint32_t), s, &oB));
   O2D_TRY(slot(d, kTileSrc, npad * sizeof(float4), s, &tsrc));
-  O2D_TRY(slot(d, kTileCent, ngroups * sizeof(double2), s, &tcent));
+  O2D_TRY(slot(d, kTileCent, ngroups * sizeof(GroupInfo), s, &tcent));
   KernelTimer kprep(O2D_KT_SPLIT_PREP, s);
   O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
                        static_cast<uint32_t*>(oB)));
   const uint32_t* order = static_cast<const uint32_t*>(oB);
-  double2* cent = static_cast<double2*>(tcent);
+  GroupInfo* cent = static_cast<GroupInfo*>(tcent);
   float4* src = static_cast<float4*>(tsrc);
-  centroid_kernel<<<grid1(ngroups), kBlock, 0, s>>>(ns, ngroups, order, sx, sy, cent);
   switch (core) {
-    case O2D_CORE_RM: launch_rebase<O2D_CORE_RM>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
-    case O2D_CORE_GAUSSIAN: launch_rebase<O2D_CORE_GAUSSIAN>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
-    case O2D_CORE_WL: launch_rebase<O2D_CORE_WL>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
-    case O2D_CORE_V2: launch_rebase<O2D_CORE_V2>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
-    default: launch_rebase<O2D_CORE_V3>(grid1(npad), s, ns, npad, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_RM: launch_prep<O2D_CORE_RM>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_GAUSSIAN: launch_prep<O2D_CORE_GAUSSIAN>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_WL: launch_prep<O2D_CORE_WL>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
+    case O2D_CORE_V2: launch_prep<O2D_CORE_V2>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
+    default: launch_prep<O2D_CORE_V3>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
   }
   kprep.stop();
   O2D_LAUNCH_CHECK();

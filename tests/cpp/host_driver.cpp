@@ -1,0 +1,153 @@
+// host_driver.cpp - drives the C++ host mirror (omega2d_amd/host) the way the reference's
+// Simulation drives Convection: reads a binary case written by tests/test_gpu_host_driver.py,
+// runs it through the gpu_hip environment, and writes the results back for the test to check
+// against the fp64 oracle.
+//
+//   o2d_host_driver <mode> <in.bin> <out.bin>
+//   mode: rk       one advection step of free particles (order, core, storage from the input)
+//         bem      BEM solve on a reactive circle + velocities of the wake particles
+//         bemstep  one advection step of the wake with the BEM solved at every stage
+//
+// Input: int64 header [mode-specific counts], then float64 arrays (node coordinates as float32).
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "Convection.h"
+
+using namespace omega2d;
+
+namespace {
+
+struct Reader {
+  std::ifstream f;
+  explicit Reader(const char* p) : f(p, std::ios::binary) {
+    if (!f) throw std::runtime_error(std::string("cannot open ") + p);
+  }
+  int64_t i64() {
+    int64_t v;
+    f.read(reinterpret_cast<char*>(&v), 8);
+    return v;
+  }
+  double f64() {
+    double v;
+    f.read(reinterpret_cast<char*>(&v), 8);
+    return v;
+  }
+  template <class T>
+  std::vector<T> vec(size_t n) {
+    std::vector<T> v(n);
+    f.read(reinterpret_cast<char*>(v.data()), sizeof(T) * n);
+    if (!f) throw std::runtime_error("short input file");
+    return v;
+  }
+};
+
+struct Writer {
+  std::ofstream f;
+  explicit Writer(const char* p) : f(p, std::ios::binary) {}
+  template <class T>
+  void vec(const std::vector<T>& v) {
+    std::vector<double> d(v.begin(), v.end());
+    f.write(reinterpret_cast<const char*>(d.data()), 8 * d.size());
+  }
+  void f64(double v) { f.write(reinterpret_cast<const char*>(&v), 8); }
+};
+
+// header: n, order, core, storage(0 double / 1 float), precision; then dt, x, y, s, r
+template <class S>
+void run_rk(Reader& in, Writer& out, int64_t n, int order, int core, int prec) {
+  const double dt = in.f64();
+  const auto x = in.vec<double>(n), y = in.vec<double>(n), s = in.vec<double>(n),
+             r = in.vec<double>(n);
+  std::vector<S> xs(x.begin(), x.end()), ys(y.begin(), y.end()), ss(s.begin(), s.end()),
+      rs(r.begin(), r.end());
+  std::vector<Collection<S>> vort, bdry, fldpt;
+  vort.emplace_back(Points<S>(n, xs.data(), ys.data(), ss.data(), rs.data()));
+  ExecEnv env(true, direct, gpu_hip);
+  env.set_core(core);
+  env.set_precision(prec);
+  Convection<S, double, Int> conv(env, order);
+  BEM<S, Int> bem;
+  conv.advect(0.0, dt, {0.0, 0.0}, vort, bdry, fldpt, bem);
+  const auto& p = std::get<Points<S>>(vort[0]);
+  out.vec(p.get_pos()[0]);
+  out.vec(p.get_pos()[1]);
+}
+
+// header: npan, nnodes, nwake, nunk, core, order; then fsx, fsy, dt, nodes x/y (float32),
+// idx (uint32 pairs), wake x, y, s, r
+void run_bem(Reader& in, Writer& out, const std::string& mode) {
+  const int64_t npan = in.i64(), nnodes = in.i64(), nw = in.i64(), nunk = in.i64(),
+                core = in.i64(), order = in.i64();
+  const double fsx = in.f64(), fsy = in.f64(), dt = in.f64();
+  const auto nx = in.vec<float>(nnodes), ny = in.vec<float>(nnodes);
+  const auto idx = in.vec<uint32_t>(2 * npan);
+  auto wx = in.vec<double>(nw), wy = in.vec<double>(nw), ws = in.vec<double>(nw),
+       wr = in.vec<double>(nw);
+  auto ground = std::make_shared<Body>();
+  Surfaces<double> circle(nnodes, nx.data(), ny.data(), npan, idx.data(), reactive, bodybound,
+                          ground);
+  if (nunk == 1) circle.set_vortex_only();
+  std::vector<Collection<double>> vort, bdry, fldpt;
+  vort.emplace_back(Points<double>(nw, wx.data(), wy.data(), ws.data(), wr.data()));
+  bdry.emplace_back(circle);
+  ExecEnv env(true, direct, gpu_hip);
+  env.set_core((int32_t)core);
+  Convection<double, double, Int> conv(env, (int)order);
+  BEM<double, Int> bem;
+  const std::array<double, Dimensions> fs = {fsx, fsy};
+  if (mode == "bem") {
+    conv.find_derivs(0.0, fs, bem, bdry, vort, fldpt);
+    const auto& c = std::get<Surfaces<double>>(bdry[0]);
+    out.vec(c.get_str());
+    out.vec(c.get_src_str());
+    out.f64(c.get_solved_omega());
+    out.vec(bem.rhs());
+    const auto& p = std::get<Points<double>>(vort[0]);
+    out.vec(p.get_vel()[0]);
+    out.vec(p.get_vel()[1]);
+  } else {
+    conv.advect(0.0, dt, fs, vort, bdry, fldpt, bem);
+    const auto& p = std::get<Points<double>>(vort[0]);
+    out.vec(p.get_pos()[0]);
+    out.vec(p.get_pos()[1]);
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 4) {
+    std::cerr << "usage: o2d_host_driver <rk|bem|bemstep> <in.bin> <out.bin>\n";
+    return 2;
+  }
+  try {
+    const std::string mode = argv[1];
+    Reader in(argv[2]);
+    Writer out(argv[3]);
+    if (mode == "rk") {
+      const int64_t n = in.i64(), order = in.i64(), core = in.i64(), storage = in.i64(),
+                    prec = in.i64();
+      if (storage == 1) run_rk<float>(in, out, n, (int)order, (int)core, (int)prec);
+      else run_rk<double>(in, out, n, (int)order, (int)core, (int)prec);
+    } else if (mode == "bem" || mode == "bemstep") {
+      run_bem(in, out, mode);
+    } else {
+      std::cerr << "unknown mode " << mode << "\n";
+      return 2;
+    }
+    if (!out.f) throw std::runtime_error("write failed");
+  } catch (const std::exception& e) {
+    std::cerr << "host_driver: " << e.what() << "\n";
+    return 1;
+  }
+  o2d_hip_release();
+  return 0;
+}
