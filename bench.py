@@ -48,7 +48,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the bounded reference-baseline sample")
     ap.add_argument("--n", type=int, default=1 << 20, help="particles (default 2^20)")
+    ap.add_argument("--only", choices=["split"], default=None,
+                    help="run only the split-form steps (short runs under a profiler)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run one device-to-device copy of BYTES (HBM counter calibration) "
+                         "and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -60,7 +67,8 @@ def main():
     import omega2d_amd as o2d
     from omega2d_amd import device as dv
     from omega2d_amd import workloads as wl
-    from omega2d_amd._cabi import (KT_ADJOINT, KT_NBODY_PLAIN, KT_NBODY_SPLIT, KT_SPLIT_PREP,
+    from omega2d_amd._cabi import (KT_ADJOINT, KT_NBODY_PLAIN, KT_NBODY_PLAIN_PT,
+                                   KT_NBODY_SPLIT, KT_SPLIT_PREP,
                                    timing_enable, timing_read)
 
     dev = torch.device("cuda", 0)
@@ -109,9 +117,14 @@ def main():
     prep_ms, prep_n = timing_read(KT_SPLIT_PREP)
     ms_split = (t1 - t0) * 1e3 / args.steps
     value = pairs / (ms_split * 1e-3) / 1e9
-    log(f"split: {ms_split:.2f} ms/step, {value:.1f} G-int/s; main kernel {k_ms / max(k_n, 1):.3f} ms"
-        f" x{k_n}, prep {prep_ms / max(prep_n, 1):.3f} ms x{prep_n}")
+    log(f"split: {ms_split:.2f} ms/step, {value:.1f} G-int/s; particle-target kernel "
+        f"{k_ms / max(k_n, 1):.3f} ms x{k_n}, prep {prep_ms / max(prep_n, 1):.3f} ms x{prep_n}")
     su, sv_, squ, sqv = (a.cpu().numpy() for a in (pu, pv, qu, qv))
+    if args.only == "split":
+        print(json.dumps({"only": "split", "ms_per_step": round(ms_split, 3),
+                          "value": round(value, 2), "kernel_ms": round(k_ms / max(k_n, 1), 4)}),
+              flush=True)
+        return 0
 
     # ---- plain fp64 form
     step("fp64")
@@ -173,7 +186,7 @@ def main():
     fwd()
     adj()
     torch.cuda.synchronize()
-    timing_read(KT_NBODY_PLAIN)
+    timing_read(KT_NBODY_PLAIN_PT)
     timing_read(KT_ADJOINT)
     timing_enable(True)
     reps = 5
@@ -188,23 +201,22 @@ def main():
     t2 = time.perf_counter()
     timing_enable(False)
     ms_fwd, ms_adj = (t1 - t0) * 1e3 / reps, (t2 - t1) * 1e3 / reps
-    kf_ms, kf_n = timing_read(KT_NBODY_PLAIN)
+    kf_ms, kf_n = timing_read(KT_NBODY_PLAIN_PT)
     ka_ms, ka_n = timing_read(KT_ADJOINT)
     log(f"adjoint workload: fwd {ms_fwd:.3f} ms, adjoint {ms_adj:.3f} ms, ratio {ms_adj / ms_fwd:.2f}")
 
-    # ---- roofline of the dominant kernel (split N-body, particle targets), from HIP events
-    kernel_ms = k_ms / max(k_n, 1)
-    flops_per_launch = flops / 2.0   # two launches per step: particles (dominant) and tracers
+    # ---- roofline of the dominant kernel: the split launch with particle targets (N x N pairs,
+    # 19 flops each in the reference's convention plus 2 per target), timed by the library's
+    # HIP events around each such launch on its stream
+    main_ms = k_ms / max(k_n, 1)
     flops_main = float(n) * n * flops_b + 2.0 * n
-    # the two launches have different sizes; apportion the measured per-step kernel time by pairs
-    share = (float(n) * n) / pairs
-    main_ms = kernel_ms * 2.0 * share
     achieved = flops_main / (main_ms * 1e-3) / 1e12
-    roofline = {"bound": "valu-fp32", "achieved": round(achieved, 3), "peak": PEAK_FP32 / 1e12,
-                "unit": "TFLOP/s", "frac": round(achieved * 1e12 / PEAK_FP32, 4), "traffic": None,
+    roofline = {"bound": "valu", "achieved": round(achieved, 3),
+                "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                "frac": round(achieved * 1e12 / PEAK_FP32, 4),
+                "traffic": measured_traffic(),
                 "kernel": "nbody32_kernel<Gaussian, particle targets>",
                 "flops_per_pair": flops_b, "kernel_ms": round(main_ms, 4)}
-    del flops_per_launch
 
     # ---- CPU baseline: the reference's own OpenMP path (S=float, A=double), bounded sample
     cpu = None
@@ -230,9 +242,9 @@ def main():
                    "particles": n, "tracers": nt, "pairs_per_step": pairs, "core": "gaussian",
                    "parallelism": "single GPU"},
         "fp64_plain": {"value": round(value_64, 2), "ms_per_step": round(ms_64, 3),
-                       "kernel_ms": round(k64_ms / max(k64_n, 1) * 2 * share, 4),
-                       "frac_fp64_peak": round(flops_main / (k64_ms / max(k64_n, 1) * 2 * share
-                                                             * 1e-3) / PEAK_FP64, 4)},
+                       "kernel_ms": round(k64_ms / max(k64_n, 1), 4),
+                       "frac_fp64_peak": round(flops_main / (k64_ms / max(k64_n, 1) * 1e-3)
+                                               / PEAK_FP64, 4)},
         "split_over_fp64": round(ms_64 / ms_split, 3),
         "err_vs_fp64": err_fp64,
         "err_vs_oracle_subsample": err_oracle,
@@ -247,6 +259,31 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel, from the FETCH_SIZE / WRITE_SIZE runs
+    recorded in profiles/ (each counter scaled by what it reads for a copy of known size)."""
+    p = os.path.join(ROOT, "profiles", "nbody32_c2_split.traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes):
+    """One device-to-device copy of a known size: FETCH_SIZE/WRITE_SIZE of this dispatch over
+    nbytes gives the factor that turns those counters into HBM bytes."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 8}), flush=True)
     return 0
 
 

@@ -84,14 +84,16 @@ int32_t o2d_hip_flops_per_pair(int32_t core, int32_t point_targets, int32_t pane
 /* ---- kernel timing ------------------------------------------------------------------------- */
 
 /* Kernel classes timed by o2d_hip_timing_*: the main kernel of each entry point. */
-#define O2D_KT_NBODY_SPLIT 1   /* split fp32 particles-on-points kernel */
-#define O2D_KT_NBODY_PLAIN 2   /* plain fp64 particles-on-points kernel */
+#define O2D_KT_NBODY_SPLIT 1   /* split fp32 particles-on-points kernel, particle targets */
+#define O2D_KT_NBODY_PLAIN 2   /* plain fp64 particles-on-points kernel, particle targets */
 #define O2D_KT_PANELS 3        /* panels on points */
 #define O2D_KT_PANEL_RHS 4     /* points on panels */
 #define O2D_KT_BEM_ASSEMBLE 5  /* influence-matrix assembly */
 #define O2D_KT_ADJOINT 6       /* adjoint kernel */
 #define O2D_KT_SPLIT_PREP 7    /* Morton order + rebasing of the split form */
-#define O2D_KT_COUNT 8
+#define O2D_KT_NBODY_SPLIT_PT 8 /* split fp32 kernel, point targets */
+#define O2D_KT_NBODY_PLAIN_PT 9 /* plain fp64 kernel, point targets */
+#define O2D_KT_COUNT 10
 
 /* enable = 1: from now on record a HIP event pair on the launch stream around every timed
  * kernel launch (results are unaffected); enable = 0: stop recording. */

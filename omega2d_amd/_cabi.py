@@ -18,7 +18,7 @@ CORE_RM, CORE_GAUSSIAN, CORE_WL, CORE_V2, CORE_V3 = 1, 2, 3, 4, 5
 VELONLY, VELANDVORT, VELANDSHEAR = 1, 4, 5
 PLAIN_FP64, SPLIT_FP32 = 1, 2
 KT_NBODY_SPLIT, KT_NBODY_PLAIN, KT_PANELS, KT_PANEL_RHS, KT_BEM_ASSEMBLE, KT_ADJOINT, \
-    KT_SPLIT_PREP = 1, 2, 3, 4, 5, 6, 7
+    KT_SPLIT_PREP, KT_NBODY_SPLIT_PT, KT_NBODY_PLAIN_PT = 1, 2, 3, 4, 5, 6, 7, 8, 9
 
 CORES = {"rm": CORE_RM, "gauss": CORE_GAUSSIAN, "gaussian": CORE_GAUSSIAN, "wl": CORE_WL,
          "v2": CORE_V2, "v3": CORE_V3}

@@ -1,0 +1,241 @@
+"""Per-configuration measurements beside the headline bench line (one JSON object per config).
+
+    python -m omega2d_amd.bench_configs [--configs c1,c3,c4,c5] [--out FILE]
+
+C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update
+C3  circle of 2048 panels + 200k wake, V2 core: BEM assembly (vortex-only + augmented 2049^2,
+    and vortex+source + augmented 4097^2) with HBM GB/s, LU factor + solve, RHS (points on
+    panels), panels on particles, particles on particles (split and plain)
+C4  250k particles -> 4096 probes, Gaussian core: forward and adjoint (fp64), ratio
+C5  4M particles in a shear layer, RM core, one Ralston RK3 step in the split form with the
+    fp64 position update; error of the split velocity against the fp64 form on 16,384 targets
+
+Timing: torch.cuda.synchronize() around each measured piece; kernel times from the library's
+HIP-event timers where a single kernel dominates. Inputs are resident in HBM.
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import device as dv
+from . import flops_per_pair, workloads as wl
+from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_SPLIT, KT_PANEL_RHS,
+                    KT_PANELS, timing_enable, timing_read)
+
+PEAK_FP32, PEAK_FP64, PEAK_HBM = 157.3e12, 78.6e12, 8.0e12
+
+
+def T(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype)
+
+
+def timed(fn, reps=1):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def kernel_ms(cls, fn, reps=1):
+    fn()
+    torch.cuda.synchronize()
+    timing_read(cls)
+    timing_enable(True)
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    timing_enable(False)
+    ms, n = timing_read(cls)
+    return ms / max(n, 1), n / reps
+
+
+def rel(a, b):
+    a, b = (x.detach().cpu().numpy() if torch.is_tensor(x) else x for x in (a, b))
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def vel(x, y, s, r, tx, ty, tr, core, prec, out=None):
+    u, v = dv.points_on_points(x, y, s, r, tx, ty, tr, core=core, precision=prec)
+    c = 0.5 / math.pi
+    return u * c, v * c
+
+
+def c1():
+    w = wl.c1_disc()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    n = x.numel()
+    res = {"config": "C1", "name": w["name"], "particles": n, "core": "rm"}
+    for prec in ("split", "fp64"):
+        xs, ys = x.clone(), y.clone()
+
+        def step():
+            xs.copy_(x)
+            ys.copy_(y)
+            u1, v1 = vel(xs, ys, s, r, xs, ys, r, "rm", prec)
+            x1, y1 = xs.clone(), ys.clone()
+            dv.rk_move(x1, y1, [u1], [v1], [1.0], 2.0 / 3.0 * w["dt"])
+            u2, v2 = vel(x1, y1, s, r, x1, y1, r, "rm", prec)
+            dv.rk_move(xs, ys, [u1, u2], [v1, v2], [0.25, 0.75], w["dt"])
+        t = timed(step, reps=5)
+        res[prec] = {"ms_per_step": round(t * 1e3, 4),
+                     "Gint_s": round(2.0 * n * n / t / 1e9, 2)}
+        res[prec + "_x"] = xs
+    res["split_pos_err_vs_fp64"] = rel(res.pop("split_x"), res["fp64_x"])
+    res.pop("fp64_x")
+    return res
+
+
+def c3():
+    w = wl.c3_circle_wake()
+    npan = len(w["idx"]) // 2
+    nx, ny = T(w["nx"]), T(w["ny"])
+    idx = torch.from_numpy(w["idx"].astype(np.int32)).cuda()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    nw = x.numel()
+    res = {"config": "C3", "name": w["name"], "panels": npan, "wake": nw, "core": "v2"}
+    rng = np.random.default_rng(0)
+    uu, uv = T(rng.normal(size=npan)), T(rng.normal(size=npan))
+    for nunk in (1, 2):
+        n = npan * nunk + 1
+        A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        ms, _ = kernel_ms(KT_BEM_ASSEMBLE, lambda: dv.bem_assemble(nx, ny, idx, nunk, True, uu, uv,
+                                                                  0.78, out=A), reps=3)
+        entries = float(npan) * npan * nunk * nunk
+        flops = float(npan) * npan * (92.0 if nunk == 1 else 122.0)
+        t_fac = timed(lambda: dv.bem_factor(A))
+        b = T(rng.normal(size=n))
+        t_sol = timed(lambda: dv.bem_solve(b), reps=5)
+        res[f"assemble_{n}"] = {"kernel_ms": round(ms, 4),
+                                "GB_s": round(8.0 * entries / (ms * 1e-3) / 1e9, 1),
+                                "frac_hbm": round(8.0 * entries / (ms * 1e-3) / PEAK_HBM, 4),
+                                "TFLOP_s": round(flops / (ms * 1e-3) / 1e12, 2),
+                                "frac_fp64": round(flops / (ms * 1e-3) / PEAK_FP64, 4),
+                                "lu_factor_ms": round(t_fac * 1e3, 3),
+                                "lu_solve_ms": round(t_sol * 1e3, 4)}
+    vs, ss = T(rng.normal(size=npan)), T(rng.normal(size=npan))
+    ms, _ = kernel_ms(KT_PANEL_RHS, lambda: dv.points_on_panels(x, y, s, nx, ny, idx), reps=3)
+    res["points_on_panels"] = {"kernel_ms": round(ms, 4),
+                               "Gint_s": round(float(nw) * npan / (ms * 1e-3) / 1e9, 2),
+                               "frac_fp64": round(39.0 * nw * npan / (ms * 1e-3) / PEAK_FP64, 4)}
+    ms, _ = kernel_ms(KT_PANELS, lambda: dv.panels_on_points(nx, ny, idx, vs, ss, x, y), reps=3)
+    res["panels_on_points"] = {"kernel_ms": round(ms, 4),
+                               "Gint_s": round(float(nw) * npan / (ms * 1e-3) / 1e9, 2),
+                               "frac_fp64": round(47.0 * nw * npan / (ms * 1e-3) / PEAK_FP64, 4)}
+    for prec, cls in (("split", KT_NBODY_SPLIT), ("fp64", KT_NBODY_PLAIN)):
+        ms, _ = kernel_ms(cls, lambda: dv.points_on_points(x, y, s, r, x, y, r, core="v2",
+                                                           precision=prec), reps=2)
+        fp = flops_per_pair("v2", False)
+        res[f"particles_{prec}"] = {"kernel_ms": round(ms, 3),
+                                    "Gint_s": round(float(nw) * nw / (ms * 1e-3) / 1e9, 1),
+                                    "frac_peak": round(fp * float(nw) * nw / (ms * 1e-3) /
+                                                       (PEAK_FP32 if prec == "split" else PEAK_FP64), 4)}
+    us, vs_ = vel(x, y, s, r, x, y, r, "v2", "split")
+    u6, v6 = vel(x, y, s, r, x, y, r, "v2", "fp64")
+    res["split_err_vs_fp64"] = math.sqrt((rel(us, u6) ** 2 * float((u6 ** 2).sum())
+                                          + rel(vs_, v6) ** 2 * float((v6 ** 2).sum()))
+                                         / float((u6 ** 2).sum() + (v6 ** 2).sum()))
+    return res
+
+
+def c4():
+    w = wl.c4_adjoint()
+    x, y, s, r, tx, ty = (T(w[k]) for k in ("x", "y", "s", "r", "tx", "ty"))
+    n, m = x.numel(), tx.numel()
+    u = torch.zeros(m, dtype=torch.float64, device="cuda")
+    v = torch.zeros_like(u)
+    g = tuple(torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(4))
+
+    def fwd():
+        u.zero_()
+        v.zero_()
+        dv.points_on_points(x, y, s, r, tx, ty, None, core="gauss", tu=u, tv=v)
+
+    def adj():
+        for a in g:
+            a.zero_()
+        dv.points_on_points_adjoint(x, y, s, r, tx, ty, 2 * u, 2 * v, core="gauss", out=g)
+    tf = timed(fwd, reps=10)
+    ta = timed(adj, reps=10)
+    kf, _ = kernel_ms(KT_NBODY_PLAIN, fwd, reps=5)
+    ka, _ = kernel_ms(KT_ADJOINT, adj, reps=5)
+    pairs = float(n) * m
+    return {"config": "C4", "name": w["name"], "particles": n, "probes": m,
+            "fwd_ms": round(tf * 1e3, 4), "adjoint_ms": round(ta * 1e3, 4),
+            "adjoint_over_fwd": round(ta / tf, 3), "fwd_kernel_ms": round(kf, 4),
+            "adjoint_kernel_ms": round(ka, 4), "fwd_Gint_s": round(pairs / tf / 1e9, 1),
+            "adjoint_Gint_s": round(pairs / ta / 1e9, 1)}
+
+
+def c5():
+    w = wl.c5_shear_layer()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    n = x.numel()
+    dt = w["dt"]
+    xs, ys = x.clone(), y.clone()
+    stage = {}
+
+    def step():
+        # Ralston RK3 (src/Convection.h:472-598): stages at t, t+dt/2, t+3dt/4 (from x0 with u1)
+        xs.copy_(x)
+        ys.copy_(y)
+        u0, v0 = vel(xs, ys, s, r, xs, ys, r, "rm", "split")
+        x1, y1 = xs.clone(), ys.clone()
+        dv.rk_move(x1, y1, [u0], [v0], [1.0], 0.5 * dt)
+        u1, v1 = vel(x1, y1, s, r, x1, y1, r, "rm", "split")
+        x2, y2 = xs.clone(), ys.clone()
+        dv.rk_move(x2, y2, [u1], [v1], [1.0], 0.75 * dt)
+        u2, v2 = vel(x2, y2, s, r, x2, y2, r, "rm", "split")
+        dv.rk_move(xs, ys, [u0, u1, u2], [v0, v1, v2], [2 / 9, 3 / 9, 4 / 9], dt)
+        stage["u0"], stage["v0"] = u0, v0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ms, launches = kernel_ms(KT_NBODY_SPLIT, lambda: vel(x, y, s, r, x, y, r, "rm", "split"))
+    # error of the first-stage split velocity against the fp64 form on a target subsample
+    rng = np.random.default_rng(4)
+    sub = torch.from_numpy(np.sort(rng.choice(n, 16384, replace=False))).cuda()
+    u6, v6 = vel(x, y, s, r, x[sub].contiguous(), y[sub].contiguous(), r[sub].contiguous(),
+                 "rm", "fp64")
+    us, vs_ = stage["u0"][sub], stage["v0"][sub]
+    err = math.sqrt(float(((us - u6) ** 2).sum() + ((vs_ - v6) ** 2).sum())
+                    / float((u6 ** 2).sum() + (v6 ** 2).sum()))
+    fp = flops_per_pair("rm", False)
+    pairs = float(n) * n
+    return {"config": "C5", "name": w["name"], "particles": n, "core": "rm",
+            "ms_per_step": round(t * 1e3, 1), "Gint_s_step": round(3 * pairs / t / 1e9, 1),
+            "eval_kernel_ms": round(ms, 2), "Gint_s_kernel": round(pairs / (ms * 1e-3) / 1e9, 1),
+            "frac_fp32_peak": round(fp * pairs / (ms * 1e-3) / PEAK_FP32, 4),
+            "split_err_vs_fp64_16k": err}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--configs", default="c1,c3,c4,c5")
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    fns = {"c1": c1, "c3": c3, "c4": c4, "c5": c5}
+    out = []
+    for name in args.configs.split(","):
+        t0 = time.perf_counter()
+        res = fns[name]()
+        res["wall_s"] = round(time.perf_counter() - t0, 2)
+        print(json.dumps(res), flush=True)
+        out.append(res)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,7 +34,7 @@ constexpr int kGroup = 64;   // sources sharing one centroid in the split form
 #define O2D_TPL64 2
 #endif
 #ifndef O2D_TPL32
-#define O2D_TPL32 2
+#define O2D_TPL32 4
 #endif
 #ifndef O2D_WAVES32
 #define O2D_WAVES32 8
@@ -601,7 +601,7 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
       ov = ou + cp.nchunk * nt;
       ow = res == O2D_VELONLY ? nullptr : ov + cp.nchunk * nt;
     }
-    KernelTimer kt(O2D_KT_NBODY_PLAIN, s);
+    KernelTimer kt(trp ? O2D_KT_NBODY_PLAIN : O2D_KT_NBODY_PLAIN_PT, s);
     switch (core) {
       case O2D_CORE_RM: dispatch64<O2D_CORE_RM>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
       case O2D_CORE_GAUSSIAN: dispatch64<O2D_CORE_GAUSSIAN>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
@@ -657,7 +657,7 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
     ou = static_cast<double*>(p);
     ov = ou + cp.nchunk * nt;
   }
-  KernelTimer kt(O2D_KT_NBODY_SPLIT, s);
+  KernelTimer kt(trp ? O2D_KT_NBODY_SPLIT : O2D_KT_NBODY_SPLIT_PT, s);
   switch (core) {
     case O2D_CORE_RM: dispatch32<O2D_CORE_RM>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
     case O2D_CORE_GAUSSIAN: dispatch32<O2D_CORE_GAUSSIAN>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
