@@ -93,7 +93,8 @@ int32_t o2d_hip_flops_per_pair(int32_t core, int32_t point_targets, int32_t pane
 #define O2D_KT_SPLIT_PREP 7    /* Morton order + rebasing of the split form */
 #define O2D_KT_NBODY_SPLIT_PT 8 /* split fp32 kernel, point targets */
 #define O2D_KT_NBODY_PLAIN_PT 9 /* plain fp64 kernel, point targets */
-#define O2D_KT_COUNT 10
+#define O2D_KT_NEAR 10          /* particle/panel proximity scan (distances, reflect, clear) */
+#define O2D_KT_COUNT 11
 
 /* enable = 1: from now on record a HIP event pair on the launch stream around every timed
  * kernel launch (results are unaffected); enable = 0: stop recording. */
@@ -204,6 +205,67 @@ int o2d_hip_points_on_points_adjoint_dev_d(const o2d_env* env, int64_t ns, const
                                            const double* srad, int64_t nt, const double* tx,
                                            const double* ty, const double* gu, const double* gv,
                                            double* gx, double* gy, double* gs, double* gr);
+
+/* ---- particle/panel proximity: src/Reflect.h ---------------------------------------------- */
+/*
+ * Panels are straight segments (node arrays nx, ny; idx = 2 node indices per panel, as in
+ * the velocity entry points); pnx, pny are the panels' unit normals pointing into the fluid
+ * as Surfaces::compute_bases stores them (src/Surfaces.h:682-725). Each target's nearest
+ * element (a panel interior or a node) is found by a scan over panels in index order with the
+ * reference's tie rule (a distance below nextafter(min, 0) replaces the set of nearest
+ * elements, one below nextafter(min, +max) joins it), in IEEE double without fused
+ * multiply-adds. A node's normal is the sum of the normals of the panels that share it.
+ *
+ * The host entry points check node indices and zero-length panels before launching (the
+ * reference asserts on a zero-length panel, src/Reflect.h:131). The device entry points
+ * count such panels into counts[2] instead (an out-of-range index is clamped, never read).
+ */
+
+/* Distance from each target to its nearest panel or node (get_nearest_distances,
+ * src/Reflect.h:792-826); with no panels every distance is sqrt(DBL_MAX). */
+int o2d_hip_nearest_distances_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                                const double* ny, int64_t np, const uint32_t* idx, int64_t nt,
+                                const double* tx, const double* ty, double* dist);
+int o2d_hip_nearest_distances_dev_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                                    const double* ny, int64_t np, const uint32_t* idx,
+                                    int64_t nt, const double* tx, const double* ty,
+                                    double* dist);
+
+/* Move each target that lies behind its nearest element to its mirror image in front of it
+ * (reflect_panp2, src/Reflect.h:167-311); tx, ty are updated in place. *num_reflected (host
+ * entry, may be NULL) receives the count; the device entry writes counts[0] = reflected,
+ * counts[1] = 0, counts[2] = bad panels (counts may be NULL). */
+int o2d_hip_reflect_d(const o2d_env* env, int64_t nnodes, const double* nx, const double* ny,
+                      int64_t np, const uint32_t* idx, const double* pnx, const double* pny,
+                      int64_t nt, double* tx, double* ty, int64_t* num_reflected);
+int o2d_hip_reflect_dev_d(const o2d_env* env, int64_t nnodes, const double* nx,
+                          const double* ny, int64_t np, const uint32_t* idx, const double* pnx,
+                          const double* pny, int64_t nt, double* tx, double* ty,
+                          int64_t* counts);
+
+/* Near-wall clearing pass (clear_inner_panp2, src/Reflect.h:487-727), called after every
+ * position update with method 1 and cutoff_mult = 0.5/sqrt(2 pi) (src/Convection.h:301).
+ * With dotp the height of a target above the layer cutoff_mult * ips over its nearest element:
+ *   method 1: a target with dotp < -0.001 ips moves out along the mean normal by -dotp;
+ *   method 0: a target with dotp < its radius (ips for field points, trad == NULL) loses
+ *             strength and moves out by the cut-table entry at dotp/radius (particles only).
+ * A target that moved is tested again, as the reference's outer loop does, up to 64 passes.
+ * tx, ty (and tstr for method 0) are updated in place. Host entry: *circ_removed and
+ * *num_cropped (each may be NULL) receive the circulation removed and the number of moves;
+ * returns O2D_ERR_STATE, after writing every result, if a target is still moving after 64
+ * passes (the reference would loop for ever). Device entry: *circ_removed (device, may be
+ * NULL); counts (device int64[3], may be NULL) = moves, unsettled targets, bad panels. */
+int o2d_hip_clear_inner_d(const o2d_env* env, int32_t method, double cutoff_mult, double ips,
+                          int64_t nnodes, const double* nx, const double* ny, int64_t np,
+                          const uint32_t* idx, const double* pnx, const double* pny, int64_t nt,
+                          double* tx, double* ty, double* tstr, const double* trad,
+                          double* circ_removed, int64_t* num_cropped);
+int o2d_hip_clear_inner_dev_d(const o2d_env* env, int32_t method, double cutoff_mult,
+                              double ips, int64_t nnodes, const double* nx, const double* ny,
+                              int64_t np, const uint32_t* idx, const double* pnx,
+                              const double* pny, int64_t nt, double* tx, double* ty,
+                              double* tstr, const double* trad, double* circ_removed,
+                              int64_t* counts);
 
 /* ---- index work of the split-precision form (exposed for testing) ------------------------ */
 

@@ -18,7 +18,8 @@ from ._cabi import (CORES, PRECISIONS, RESULTS, Env, O2DError, call, lib, make_e
 
 __all__ = ["Env", "O2DError", "make_env", "points_on_points", "panels_on_points",
            "points_on_panels", "bem_assemble", "bem_factor", "bem_solve", "rk_move",
-           "points_on_points_adjoint", "flops_per_pair", "release"]
+           "points_on_points_adjoint", "nearest_distances", "reflect", "clear_inner",
+           "flops_per_pair", "release"]
 
 
 def _p(a):
@@ -157,3 +158,45 @@ def points_on_points_adjoint(sx, sy, ss, sr, tx, ty, gu, gv, *, core="gauss", de
     call("o2d_hip_points_on_points_adjoint_d", ctypes.byref(env), ns, _p(sx), _p(sy), _p(ss),
          _p(sr), len(tx), _p(tx), _p(ty), _p(gu), _p(gv), *[_p(o) for o in out])
     return tuple(out)
+
+
+def nearest_distances(nx, ny, idx, tx, ty, *, device=0):
+    """get_nearest_distances (src/Reflect.h:792-826): distance from each target to its nearest
+    panel or node."""
+    nx, ny, tx, ty = (_arr(a, np.float64) for a in (nx, ny, tx, ty))
+    idx = _arr(idx, np.uint32)
+    dist = np.zeros(len(tx))
+    env = make_env("v2", "fp64", "vel", device)
+    call("o2d_hip_nearest_distances_d", ctypes.byref(env), len(nx), _p(nx), _p(ny),
+         len(idx) // 2, _p(idx), len(tx), _p(tx), _p(ty), _p(dist))
+    return dist
+
+
+def reflect(nx, ny, idx, pnx, pny, tx, ty, *, device=0):
+    """reflect_panp2 (src/Reflect.h:167-311) on copies of tx, ty: returns (x, y, reflected)."""
+    nx, ny, pnx, pny = (_arr(a, np.float64) for a in (nx, ny, pnx, pny))
+    tx, ty = _arr(tx, np.float64).copy(), _arr(ty, np.float64).copy()
+    idx = _arr(idx, np.uint32)
+    n = ctypes.c_int64()
+    env = make_env("v2", "fp64", "vel", device)
+    call("o2d_hip_reflect_d", ctypes.byref(env), len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx),
+         _p(pnx), _p(pny), len(tx), _p(tx), _p(ty), ctypes.byref(n))
+    return tx, ty, n.value
+
+
+def clear_inner(method, nx, ny, idx, pnx, pny, cutoff_mult, ips, tx, ty, ts=None, tr=None, *,
+                device=0):
+    """clear_inner_panp2 (src/Reflect.h:487-727) on copies of the targets: returns dict(x, y,
+    str, circ_removed, num_cropped). tr=None makes the targets field points."""
+    nx, ny, pnx, pny = (_arr(a, np.float64) for a in (nx, ny, pnx, pny))
+    tx, ty = _arr(tx, np.float64).copy(), _arr(ty, np.float64).copy()
+    ts = None if ts is None else _arr(ts, np.float64).copy()
+    tr = None if tr is None else _arr(tr, np.float64)
+    idx = _arr(idx, np.uint32)
+    circ, n = ctypes.c_double(), ctypes.c_int64()
+    env = make_env("v2", "fp64", "vel", device)
+    call("o2d_hip_clear_inner_d", ctypes.byref(env), int(method), float(cutoff_mult), float(ips),
+         len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), _p(pnx), _p(pny), len(tx), _p(tx),
+         _p(ty), _p(ts), _p(tr), ctypes.byref(circ), ctypes.byref(n))
+    return dict(x=tx, y=ty, str=ts, circ_removed=circ.value, num_cropped=n.value)
+

@@ -24,8 +24,8 @@ import torch
 
 from . import device as dv
 from . import flops_per_pair, workloads as wl
-from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_SPLIT, KT_PANEL_RHS,
-                    KT_PANELS, timing_enable, timing_read)
+from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_PLAIN_PT,
+                    KT_NBODY_SPLIT, KT_PANEL_RHS, KT_PANELS, timing_enable, timing_read)
 
 PEAK_FP32, PEAK_FP64, PEAK_HBM = 157.3e12, 78.6e12, 8.0e12
 
@@ -164,7 +164,7 @@ def c4():
         dv.points_on_points_adjoint(x, y, s, r, tx, ty, 2 * u, 2 * v, core="gauss", out=g)
     tf = timed(fwd, reps=10)
     ta = timed(adj, reps=10)
-    kf, _ = kernel_ms(KT_NBODY_PLAIN, fwd, reps=5)
+    kf, _ = kernel_ms(KT_NBODY_PLAIN_PT, fwd, reps=5)  # probes are point targets
     ka, _ = kernel_ms(KT_ADJOINT, adj, reps=5)
     pairs = float(n) * m
     return {"config": "C4", "name": w["name"], "particles": n, "probes": m,

@@ -44,6 +44,13 @@ int adjoint_dev(Device* d, hipStream_t s, int core, int64_t ns, const double* sx
                 const double* tx, const double* ty, const double* gu, const double* gv,
                 double* gx, double* gy, double* gs, double* gr);
 
+// particle/panel proximity (mode 0 nearest distances, 1 reflect, 2 clear), device pointers
+int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult, double ips,
+             int64_t nnodes, const double* nx, const double* ny, int64_t np,
+             const uint32_t* idx, const double* pnx, const double* pny, int64_t nt, double* tx,
+             double* ty, double* ts, const double* tr, double* dist, double* value,
+             int64_t* counts);
+
 // host <-> device staging for the synchronous entry points
 template <class T>
 int upload(Device* d, Slot sl, const T* h, int64_t n, hipStream_t s, T** out) {

@@ -28,6 +28,7 @@ enum Slot : int {
   kLU, kPivots, kInfo,
   kRkPtrs, kRkStage,
   kGradX, kGradY, kGradS, kGradR,
+  kNearPart, kNodeNorm, kPanRec, kCutTable, kNearCounts,
   kNumSlots
 };
 

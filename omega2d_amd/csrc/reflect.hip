@@ -1,0 +1,581 @@
+// reflect.hip - particle/panel proximity on the device (src/Reflect.h): the distance from each
+// target to the nearest panel (get_nearest_distances, :792-826), reflection of targets that lie
+// behind their nearest panel (reflect_panp2, :167-311) and the near-wall clearing pass
+// (clear_inner_panp2, :487-727; method 1 pushes targets out to the cutoff layer, method 0 trims
+// strength with the cut tables of :333-432).
+//
+// Every target scans all panels in index order, as the reference does, because the set of
+// nearest elements depends on that order: a distance below nextafter(min, 0) replaces the set,
+// one below nextafter(min, +max) joins it. The scan is one thread per target over panels staged
+// through LDS. All arithmetic is IEEE double with no fused multiply-add, in the reference's
+// operation order, so results are bit-identical to a plain C loop. Two shortcuts keep the
+// common case free of divisions without changing any result:
+//   * t = dot/|b|^2 lies in (0,1) exactly when 0 < dot < |b|^2, unless the quotient could
+//     underflow (dot < |b|^2 * 2^-1000), where the division is done;
+//   * a panel-interior distance c^2/|b|^2 is divided exactly only when c^2 * (1/|b|^2) does not
+//     already exceed the current acceptance bound by a margin far above its rounding error.
+// Node normals are the sums of the normals of the panels that share the node, added in panel
+// order (end 0 before end 1), from a stable sort of the panel ends by node.
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "nbody.h"
+#include "o2d_runtime.h"
+
+#pragma clang fp contract(off)
+
+namespace o2d {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kPanTile = 128;     // panels per LDS tile
+constexpr int kMaxPasses = 64;    // clearing passes per target before it counts as unsettled
+constexpr int kCutCap = 64;       // rows of the cut table
+enum Mode { kNearest = 0, kReflect = 1, kClear = 2 };
+
+// one panel as the scan reads it
+struct PanelRec {
+  double x0, y0, x1, y1;        // end nodes
+  double bx, by, blensq, rbl;   // segment vector, |b|^2 and 1/|b|^2 (0 when not finite)
+  double pnx, pny;              // panel normal
+  double n0x, n0y, n1x, n1y;    // node normals at the two ends
+};
+constexpr int kRecDoubles = sizeof(PanelRec) / sizeof(double);
+static_assert(sizeof(PanelRec) == 14 * sizeof(double), "PanelRec layout");
+
+__device__ inline double next_down(double m) {  // nextafter(m, 0) for m >= 0
+  return m > 0.0 ? __longlong_as_double(__double_as_longlong(m) - 1) : m;
+}
+__device__ inline double next_up(double m) {    // nextafter(m, DBL_MAX) for 0 <= m <= DBL_MAX
+  return m < DBL_MAX ? __longlong_as_double(__double_as_longlong(m) + 1) : m;
+}
+
+// key/value pairs (node, panel end) for the node-normal sort; out-of-range nodes are clamped
+// here and counted by panel_rec_kernel
+__global__ __launch_bounds__(kBlock) void end_keys_kernel(int64_t nends, int64_t nnodes,
+                                                          const uint32_t* __restrict__ idx,
+                                                          uint32_t* __restrict__ key,
+                                                          uint32_t* __restrict__ val) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nends) return;
+  const uint32_t k = idx[e];
+  key[e] = (int64_t)k < nnodes ? k : (uint32_t)(nnodes - 1);
+  val[e] = (uint32_t)e;
+}
+
+// node normal = sum over the node's panel ends in end order; the first end of each run sums it
+__global__ __launch_bounds__(kBlock) void node_normal_kernel(int64_t nends,
+                                                             const uint32_t* __restrict__ key,
+                                                             const uint32_t* __restrict__ val,
+                                                             const double* __restrict__ pnx,
+                                                             const double* __restrict__ pny,
+                                                             double* __restrict__ nnx,
+                                                             double* __restrict__ nny) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nends) return;
+  const uint32_t k = key[p];
+  if (p > 0 && key[p - 1] == k) return;
+  double sx = 0.0, sy = 0.0;
+  for (int64_t q = p; q < nends && key[q] == k; ++q) {
+    const uint32_t j = val[q] >> 1;
+    sx = sx + pnx[j];
+    sy = sy + pny[j];
+  }
+  nnx[k] = sx;
+  nny[k] = sy;
+}
+
+__global__ void zero_counter_kernel(int* c) {
+  if (threadIdx.x == 0) c[0] = 0;
+}
+
+// panel records; counts panels with an out-of-range node or zero length into *bad
+__global__ __launch_bounds__(kBlock) void panel_rec_kernel(
+    int64_t np, int64_t nnodes, const double* __restrict__ nx, const double* __restrict__ ny,
+    const uint32_t* __restrict__ idx, const double* __restrict__ pnx,
+    const double* __restrict__ pny, const double* __restrict__ nnx,
+    const double* __restrict__ nny, PanelRec* __restrict__ rec, int* __restrict__ bad) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= np) return;
+  uint32_t a = idx[2 * j], b = idx[2 * j + 1];
+  bool isbad = false;
+  if ((int64_t)a >= nnodes) { a = (uint32_t)(nnodes - 1); isbad = true; }
+  if ((int64_t)b >= nnodes) { b = (uint32_t)(nnodes - 1); isbad = true; }
+  PanelRec r;
+  r.x0 = nx[a];
+  r.y0 = ny[a];
+  r.x1 = nx[b];
+  r.y1 = ny[b];
+  r.bx = r.x1 - r.x0;
+  r.by = r.y1 - r.y0;
+  r.blensq = r.bx * r.bx + r.by * r.by;
+  const double rb = 1.0 / r.blensq;
+  r.rbl = (rb < DBL_MAX) ? rb : 0.0;  // 0 disables the division shortcut
+  if (!(r.blensq != 0.0)) isbad = true;
+  r.pnx = pnx ? pnx[j] : 0.0;
+  r.pny = pny ? pny[j] : 0.0;
+  r.n0x = nnx ? nnx[a] : 0.0;
+  r.n0y = nny ? nny[a] : 0.0;
+  r.n1x = nnx ? nnx[b] : 0.0;
+  r.n1y = nny ? nny[b] : 0.0;
+  rec[j] = r;
+  if (isbad) atomicAdd(bad, 1);
+}
+
+struct CutTable {
+  int rows;
+  const double* pos;
+  const double* sm;
+  const double* ds;
+};
+
+struct ClearParams {
+  int method;
+  int max_passes;
+  double cutoff;   // cutoff_mult * ips
+  double ips;
+  double push_tol; // -0.001 * ips
+  CutTable ct;
+};
+
+// get_cut_entry (src/Reflect.h:400-432)
+__device__ void cut_entry(const CutTable& ct, double p, double* smult, double* dshift) {
+  *smult = 1.0;
+  *dshift = 0.0;
+  if (p < ct.pos[0]) {
+    *smult = 0.0;
+    *dshift = 1.0 - p;
+  } else if (p > ct.pos[ct.rows - 1]) {
+    *smult = 1.0;
+    *dshift = 0.0;
+  } else {
+    for (int i = 1; i < ct.rows; ++i) {
+      if (p < ct.pos[i]) {
+        const double frac = (p - ct.pos[i - 1]) / (ct.pos[i] - ct.pos[i - 1]);
+        *smult = frac * ct.sm[i] + (1.0 - frac) * ct.sm[i - 1];
+        *dshift = frac * ct.ds[i] + (1.0 - frac) * ct.ds[i - 1];
+        break;
+      }
+    }
+  }
+}
+
+// running state of one target's scan
+struct Scan {
+  double m, lo, hi, thr;   // current minimum, nextafter(m, 0), nextafter(m, max), filter bound
+  int64_t nh;              // hits in the current set
+  double snx, sny, scx, scy;
+  double h0d, h0nx, h0ny, h0cx, h0cy;  // first hit: distance, normal, contact point
+
+  __device__ void start(double m0) {
+    set_min(m0);
+    nh = 0;
+    snx = sny = scx = scy = 0.0;
+    h0d = h0nx = h0ny = h0cx = h0cy = 0.0;
+  }
+  __device__ void set_min(double v) {
+    m = v;
+    lo = next_down(v);
+    hi = next_up(v);
+    thr = hi * (1.0 + 0x1p-40) + 0x1p-1000;
+  }
+};
+
+template <int MODE>
+__device__ __forceinline__ void scan_panel(const PanelRec& p, double x, double y, Scan& st) {
+  const double ax = x - p.x0, ay = y - p.y0;
+  const double dot = ax * p.bx + ay * p.by;
+  bool in = (dot > 0.0) && (dot < p.blensq);
+  if (in && dot < p.blensq * 0x1p-1000) {
+    const double t = dot / p.blensq;
+    in = (t > 0.0) && (t < 1.0);
+  }
+  double d;
+  int kind;  // 0 panel, 1 node 0, 2 node 1
+  if (in) {
+    const double c = ay * p.bx - ax * p.by;
+    const double sq = c * c;
+    if (sq * p.rbl > st.thr) return;  // cannot reach the acceptance bound
+    d = sq / p.blensq;
+    kind = 0;
+  } else {
+    const double rij2 = ax * ax + ay * ay;
+    const double ex = x - p.x1, ey = y - p.y1;
+    const double rij12 = ex * ex + ey * ey;
+    if (rij2 < rij12) {
+      d = rij2;
+      kind = 1;
+    } else {
+      d = rij12;
+      kind = 2;
+    }
+  }
+  if (MODE == kNearest) {
+    if (d < st.lo) st.set_min(d);
+    return;
+  }
+  if (d < st.lo) {
+    st.set_min(d);
+    st.nh = 0;
+    st.snx = st.sny = st.scx = st.scy = 0.0;
+  } else if (!(d < st.hi)) {
+    return;
+  }
+  double hnx, hny, hcx, hcy;
+  if (kind == 0) {
+    const double t = dot / p.blensq;
+    hnx = p.pnx;
+    hny = p.pny;
+    hcx = (1.0 - t) * p.x0 + t * p.x1;
+    hcy = (1.0 - t) * p.y0 + t * p.y1;
+  } else if (kind == 1) {
+    hnx = p.n0x;
+    hny = p.n0y;
+    hcx = p.x0;
+    hcy = p.y0;
+  } else {
+    hnx = p.n1x;
+    hny = p.n1y;
+    hcx = p.x1;
+    hcy = p.y1;
+  }
+  if (st.nh == 0) {
+    st.h0d = d;
+    st.h0nx = hnx;
+    st.h0ny = hny;
+    st.h0cx = hcx;
+    st.h0cy = hcy;
+  }
+  st.snx = st.snx + hnx;
+  st.sny = st.sny + hny;
+  st.scx = st.scx + hcx;
+  st.scy = st.scy + hcy;
+  st.nh += 1;
+}
+
+// fixed-order block sums of (count0, count1, value) into partial[blockIdx.x]
+struct Partial {
+  long long c0, c1;
+  double v;
+  double pad;
+};
+
+__device__ void block_partial(long long c0, long long c1, double v, Partial* out) {
+  __shared__ long long s0[kBlock], s1[kBlock];
+  __shared__ double sv[kBlock];
+  const int t = threadIdx.x;
+  s0[t] = c0;
+  s1[t] = c1;
+  sv[t] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      s0[t] += s0[t + w];
+      s1[t] += s1[t + w];
+      sv[t] = sv[t] + sv[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = Partial{s0[0], s1[0], sv[0], 0.0};
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void near_kernel(int64_t np, const PanelRec* __restrict__ pan,
+                                                      int64_t nt, double* __restrict__ tx,
+                                                      double* __restrict__ ty,
+                                                      double* __restrict__ ts,
+                                                      const double* __restrict__ tr,
+                                                      double* __restrict__ dist, ClearParams cp,
+                                                      Partial* __restrict__ part) {
+  __shared__ double tile[kPanTile * kRecDoubles];
+  const PanelRec* lp = reinterpret_cast<const PanelRec*>(tile);
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < nt;
+  double x = valid ? tx[i] : 0.0, y = valid ? ty[i] : 0.0;
+  double s = (valid && ts && tr) ? ts[i] : 0.0;
+  const double rad = (valid && tr) ? tr[i] : cp.ips;
+  bool untested = valid;
+  bool moved = false;
+  int passes = 0;
+  long long c0 = 0;
+  double removed = 0.0;
+  const int maxp = (MODE == kClear) ? cp.max_passes : 1;
+  const double m0 = (MODE == kClear) ? 0.1 * DBL_MAX : DBL_MAX;
+  for (;;) {
+    const bool act = untested && passes < maxp;
+    if (!__syncthreads_or(act)) break;
+    Scan st;
+    st.start(m0);
+    for (int64_t base = 0; base < np; base += kPanTile) {
+      const int n = (int)((np - base) < kPanTile ? (np - base) : kPanTile);
+      const double* src = reinterpret_cast<const double*>(pan + base);
+      __syncthreads();
+      for (int e = threadIdx.x; e < n * kRecDoubles; e += kBlock) tile[e] = src[e];
+      __syncthreads();
+      if (act)
+        for (int k = 0; k < n; ++k) scan_panel<MODE>(lp[k], x, y, st);
+    }
+    if (!act) continue;
+    ++passes;
+    if (MODE == kNearest) {
+      dist[i] = sqrt(st.m);
+      untested = false;
+      continue;
+    }
+    if (st.nh == 0) {  // no nearest element (reference: assert / retry)
+      if (MODE == kReflect) untested = false;
+      continue;
+    }
+    double normx = st.snx, normy = st.sny, cpx = st.scx, cpy = st.scy;
+    if (MODE == kReflect && normx * normx + normy * normy == 0.0) {
+      normx = normx + st.h0nx;
+      normy = normy + st.h0ny;
+      cpx = st.h0cx;
+      cpy = st.h0cy;
+    }
+    const double normilen = 1.0 / sqrt(normx * normx + normy * normy);
+    normx = normx * normilen;
+    normy = normy * normilen;
+    cpx = cpx / (double)st.nh;
+    cpy = cpy / (double)st.nh;
+    if (MODE == kReflect) {
+      const double dotp = normx * (x - cpx) + normy * (y - cpy);
+      if (dotp < 0.0) {
+        const double dd = sqrt(st.h0d);
+        x = cpx + dd * normx;
+        y = cpy + dd * normy;
+        moved = true;
+        ++c0;
+      }
+      untested = false;
+      continue;
+    }
+    const double dotp = normx * (x - cpx) + normy * (y - cpy) - cp.cutoff;
+    if (cp.method == 0) {
+      if (dotp < rad) {
+        if (tr) {
+          double sm, dsh;
+          cut_entry(cp.ct, dotp / rad, &sm, &dsh);
+          removed = removed + s * (1.0 - sm);
+          s = s * sm;
+          x = x + dsh * rad * normx;
+          y = y + dsh * rad * normy;
+          moved = true;
+        }
+        ++c0;
+      } else {
+        untested = false;
+      }
+    } else {
+      if (dotp < cp.push_tol) {
+        x = x - dotp * normx;
+        y = y - dotp * normy;
+        moved = true;
+        ++c0;
+      } else {
+        untested = false;
+      }
+    }
+  }
+  if (MODE == kNearest) return;
+  if (valid && moved) {
+    tx[i] = x;
+    ty[i] = y;
+    if (MODE == kClear && ts && tr) ts[i] = s;
+  }
+  block_partial(c0, (valid && untested) ? 1 : 0, removed, part);
+}
+
+// fixed-order sum of the block partials: counts[0..1], *value; counts[2] = bad panels
+__global__ __launch_bounds__(kBlock) void finish_kernel(int nblk, const Partial* __restrict__ part,
+                                                        const int* __restrict__ bad,
+                                                        double* __restrict__ value,
+                                                        int64_t* __restrict__ counts) {
+  __shared__ long long s0[kBlock], s1[kBlock];
+  __shared__ double sv[kBlock];
+  const int t = threadIdx.x;
+  long long a = 0, b = 0;
+  double v = 0.0;
+  for (int k = t; k < nblk; k += kBlock) {
+    a += part[k].c0;
+    b += part[k].c1;
+    v = v + part[k].v;
+  }
+  s0[t] = a;
+  s1[t] = b;
+  sv[t] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      s0[t] += s0[t + w];
+      s1[t] += s1[t + w];
+      sv[t] = sv[t] + sv[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (counts) {
+      counts[0] = s0[0];
+      counts[1] = s1[0];
+      counts[2] = bad ? bad[0] : 0;
+    }
+    if (value) value[0] = sv[0];
+  }
+}
+
+inline unsigned blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// init_cut_tables<double>(0.1) (src/Reflect.h:333-394), computed once on the host in IEEE double
+// without fused operations, in the reference's order
+struct HostCutTable {
+  int rows = 0;
+  double v[3 * kCutCap];
+  HostCutTable() {
+    const double dxin = 0.1, max_rad = 3.0;
+    const int nx = (int)(0.5 + max_rad / dxin);
+    const double dx = max_rad / (double)(float)nx;
+    double* pos = v;
+    double* sm = v + kCutCap;
+    double* ds = v + 2 * kCutCap;
+    int r = 0;
+    pos[r] = (double)(-nx - 0.5) * dx;
+    sm[r] = 0.0;
+    ds[r] = 0.0;
+    ++r;
+    double twgt = 0.0, tmom = 0.0;
+    for (int i = -nx; i < nx + 1 && r < kCutCap - 1; ++i) {
+      const double xi = (double)i * dx;
+      const double distxs = xi * xi;
+      double rwgt = 0.0;
+      for (int j = -nx; j < nx + 1; ++j) {
+        const double yj = (double)j * dx;
+        const double distxys = distxs + yj * yj;
+        rwgt += std::exp(-distxys);
+      }
+      rwgt *= dx * dx;
+      rwgt *= 1.0 / M_PI;
+      twgt += rwgt;
+      tmom += ((double)i * dx) * rwgt;
+      pos[r] = (double)(i + 0.5) * dx;
+      sm[r] = twgt;
+      ds[r] = -tmom / twgt;
+      ++r;
+    }
+    pos[r] = (double)(nx + 1.5) * dx;
+    sm[r] = 1.0;
+    ds[r] = 0.0;
+    rows = r + 1;
+  }
+};
+
+const HostCutTable& host_cut_table() {
+  static const HostCutTable t;
+  return t;
+}
+
+}  // namespace
+
+// Shared driver of the three operations (device pointers, enqueued on s). dist: nearest
+// distances (mode kNearest); value: removed circulation (kClear); counts[3]: moved target
+// events, unsettled targets, bad panels.
+int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult, double ips,
+             int64_t nnodes, const double* nx, const double* ny, int64_t np,
+             const uint32_t* idx, const double* pnx, const double* pny, int64_t nt, double* tx,
+             double* ty, double* ts, const double* tr, double* dist, double* value,
+             int64_t* counts) {
+  const unsigned nblk = blocks(nt);
+  void* pb;
+  O2D_TRY(slot(d, kNearPart, sizeof(Partial) * (nblk > 0 ? nblk : 1) + 64, s, &pb));
+  Partial* part = static_cast<Partial*>(pb);
+  int* bad = reinterpret_cast<int*>(part + (nblk > 0 ? nblk : 1));
+  zero_counter_kernel<<<1, 64, 0, s>>>(bad);
+  O2D_LAUNCH_CHECK();
+  PanelRec* rec = nullptr;
+  if (np > 0) {
+    double *nnx = nullptr, *nny = nullptr;
+    if (mode != kNearest) {
+      const int64_t nends = 2 * np;
+      void *kA, *vA, *kB, *vB, *nn;
+      O2D_TRY(slot(d, kKeysA, nends * sizeof(uint32_t), s, &kA));
+      O2D_TRY(slot(d, kOrderA, nends * sizeof(uint32_t), s, &vA));
+      O2D_TRY(slot(d, kKeysB, nends * sizeof(uint32_t), s, &kB));
+      O2D_TRY(slot(d, kOrderB, nends * sizeof(uint32_t), s, &vB));
+      O2D_TRY(slot(d, kNodeNorm, 2 * sizeof(double) * (size_t)nnodes, s, &nn));
+      nnx = static_cast<double*>(nn);
+      nny = nnx + nnodes;
+      end_keys_kernel<<<blocks(nends), kBlock, 0, s>>>(nends, nnodes, idx,
+                                                       static_cast<uint32_t*>(kA),
+                                                       static_cast<uint32_t*>(vA));
+      O2D_LAUNCH_CHECK();
+      int bits = 1;
+      while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)nnodes) ++bits;
+      size_t tbytes = 0;
+      O2D_HIP(hipcub::DeviceRadixSort::SortPairs(
+          nullptr, tbytes, static_cast<uint32_t*>(kA), static_cast<uint32_t*>(kB),
+          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nends, 0, bits, s));
+      void* tmp;
+      O2D_TRY(slot(d, kSortTemp, tbytes, s, &tmp));
+      O2D_HIP(hipcub::DeviceRadixSort::SortPairs(
+          tmp, tbytes, static_cast<uint32_t*>(kA), static_cast<uint32_t*>(kB),
+          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nends, 0, bits, s));
+      node_normal_kernel<<<blocks(nends), kBlock, 0, s>>>(
+          nends, static_cast<uint32_t*>(kB), static_cast<uint32_t*>(vB), pnx, pny, nnx, nny);
+      O2D_LAUNCH_CHECK();
+    }
+    void* rp;
+    O2D_TRY(slot(d, kPanRec, sizeof(PanelRec) * (size_t)np, s, &rp));
+    rec = static_cast<PanelRec*>(rp);
+    panel_rec_kernel<<<blocks(np), kBlock, 0, s>>>(np, nnodes, nx, ny, idx,
+                                                   mode != kNearest ? pnx : nullptr,
+                                                   mode != kNearest ? pny : nullptr, nnx, nny,
+                                                   rec, bad);
+    O2D_LAUNCH_CHECK();
+  }
+  ClearParams cp{};
+  cp.method = method;
+  cp.max_passes = kMaxPasses;
+  cp.cutoff = cutoff_mult * ips;
+  cp.ips = ips;
+  cp.push_tol = -0.001 * ips;
+  if (mode == kClear && method == 0) {
+    const HostCutTable& h = host_cut_table();
+    void* ctp;
+    O2D_TRY(slot(d, kCutTable, sizeof(h.v), s, &ctp));
+    O2D_HIP(hipMemcpyAsync(ctp, h.v, sizeof(h.v), hipMemcpyHostToDevice, s));
+    const double* dv = static_cast<const double*>(ctp);
+    cp.ct = CutTable{h.rows, dv, dv + kCutCap, dv + 2 * kCutCap};
+  }
+  if (nt > 0) {
+    // clear and reflect with no panels leave every target where it is
+    const bool run = (mode == kNearest) || np > 0;
+    if (run) {
+      KernelTimer kt(O2D_KT_NEAR, s);
+      if (mode == kNearest)
+        near_kernel<kNearest><<<nblk, kBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+      else if (mode == kReflect)
+        near_kernel<kReflect><<<nblk, kBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+      else
+        near_kernel<kClear><<<nblk, kBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+      kt.stop();
+      O2D_LAUNCH_CHECK();
+    }
+    if (mode != kNearest && (value || counts)) {
+      if (run) {
+        finish_kernel<<<1, kBlock, 0, s>>>((int)nblk, part, bad, value, counts);
+      } else {
+        finish_kernel<<<1, kBlock, 0, s>>>(0, part, bad, value, counts);
+      }
+      O2D_LAUNCH_CHECK();
+    }
+  } else if (mode != kNearest && (value || counts)) {
+    finish_kernel<<<1, kBlock, 0, s>>>(0, part, bad, value, counts);
+    O2D_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace o2d

@@ -12,7 +12,7 @@ from ._cabi import VELANDSHEAR, VELANDVORT, VELONLY, call, make_env
 
 __all__ = ["points_on_points", "panels_on_points", "points_on_panels", "bem_assemble",
            "bem_factor", "bem_solve", "rk_move", "points_on_points_adjoint", "morton_order",
-           "env_for"]
+           "nearest_distances", "reflect", "clear_inner", "env_for"]
 
 
 def _ptr(t):
@@ -145,3 +145,42 @@ def morton_order(x, y, x0, y0, inv_extent):
     call("o2d_hip_morton_order_dev_d", ctypes.byref(env), n, _ptr(x), _ptr(y), float(x0),
          float(y0), float(inv_extent), _ptr(keys), _ptr(order))
     return keys, order
+
+
+def nearest_distances(nx, ny, idx, tx, ty, dist=None):
+    dev = _check(nx, ny, tx, ty)
+    _check(idx, dtype=torch.int32)
+    nt = tx.numel()
+    dist = torch.empty(nt, dtype=torch.float64, device=dev) if dist is None else dist
+    env = env_for(dev)
+    call("o2d_hip_nearest_distances_dev_d", ctypes.byref(env), nx.numel(), _ptr(nx), _ptr(ny),
+         idx.numel() // 2, _ptr(idx), nt, _ptr(tx), _ptr(ty), _ptr(dist))
+    return dist
+
+
+def reflect(nx, ny, idx, pnx, pny, tx, ty):
+    """In place on tx, ty; returns a device int64[3] (reflected, 0, bad panels)."""
+    dev = _check(nx, ny, pnx, pny, tx, ty)
+    _check(idx, dtype=torch.int32)
+    counts = torch.zeros(3, dtype=torch.int64, device=dev)
+    env = env_for(dev)
+    call("o2d_hip_reflect_dev_d", ctypes.byref(env), nx.numel(), _ptr(nx), _ptr(ny),
+         idx.numel() // 2, _ptr(idx), _ptr(pnx), _ptr(pny), tx.numel(), _ptr(tx), _ptr(ty),
+         _ptr(counts))
+    return counts
+
+
+def clear_inner(method, nx, ny, idx, pnx, pny, cutoff_mult, ips, tx, ty, ts=None, tr=None):
+    """In place on tx, ty (and ts for method 0); returns (circ_removed, counts) as device
+    tensors, counts = (moves, unsettled targets, bad panels)."""
+    dev = _check(nx, ny, pnx, pny, tx, ty, ts, tr)
+    _check(idx, dtype=torch.int32)
+    counts = torch.zeros(3, dtype=torch.int64, device=dev)
+    circ = torch.zeros(1, dtype=torch.float64, device=dev)
+    env = env_for(dev)
+    call("o2d_hip_clear_inner_dev_d", ctypes.byref(env), int(method), float(cutoff_mult),
+         float(ips), nx.numel(), _ptr(nx), _ptr(ny), idx.numel() // 2, _ptr(idx), _ptr(pnx),
+         _ptr(pny), tx.numel(), _ptr(tx), _ptr(ty), _ptr(ts), _ptr(tr), _ptr(circ),
+         _ptr(counts))
+    return circ, counts
+

@@ -67,6 +67,11 @@ def lib():
         L.oc_points_on_points_adjoint.argtypes = [_int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                                   _vp, _vp, _vp, _vp, _vp, _vp]
         L.oc_morton_keys.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _dbl, _vp]
+        L.oc_nearest_distances.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp]
+        L.oc_reflect.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp]
+        L.oc_cut_table.argtypes = [_dbl, _int, _vp, _vp, _vp]
+        L.oc_clear_inner.argtypes = [_int, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _dbl, _dbl, _i64,
+                                     _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -98,6 +103,12 @@ def ref():
         R.ref_points_move3_d.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
                                          _dbl, _dbl, _dbl]
         R.ref_finalize_d.argtypes = [_i64, _vp, _vp, _dbl, _dbl]
+        R.ref_nearest_distances_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp]
+        R.ref_panel_normals_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp]
+        R.ref_reflect_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp]
+        R.ref_clear_inner_d.argtypes = [_int, _i64, _vp, _vp, _i64, _vp, _dbl, _dbl, _i64, _vp,
+                                        _vp, _vp, _vp, _vp]
+        R.ref_cut_table_d.argtypes = [_int, _vp, _vp, _vp]
         _ref = R
     return _ref
 
@@ -256,6 +267,53 @@ def morton_keys(x, y, x0, y0, inv_extent):
     return keys
 
 
+def nearest_distances(nx, ny, idx, tx, ty):
+    nx, ny, tx, ty = map(_f64, (nx, ny, tx, ty))
+    idx = _u32(idx)
+    d = np.zeros(len(tx))
+    rc = lib().oc_nearest_distances(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), len(tx),
+                                    _p(tx), _p(ty), _p(d))
+    assert rc == 0, rc
+    return d
+
+
+def reflect(nx, ny, idx, pnx, pny, tx, ty):
+    """Returns (new x, new y, number reflected)."""
+    nx, ny, pnx, pny = map(_f64, (nx, ny, pnx, pny))
+    tx, ty = _f64(tx).copy(), _f64(ty).copy()
+    idx = _u32(idx)
+    n = _i64()
+    rc = lib().oc_reflect(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), _p(pnx), _p(pny),
+                          len(tx), _p(tx), _p(ty), ctypes.byref(n))
+    assert rc == 0, rc
+    return tx, ty, n.value
+
+
+def cut_table(dx=0.1):
+    pos, sm, ds = (np.zeros(256) for _ in range(3))
+    rows = lib().oc_cut_table(float(dx), 256, _p(pos), _p(sm), _p(ds))
+    assert rows > 0, rows
+    return pos[:rows], sm[:rows], ds[:rows]
+
+
+def clear_inner(method, nx, ny, idx, pnx, pny, cutoff_mult, ips, tx, ty, ts=None, tr=None,
+                max_passes=64):
+    """Returns dict(x, y, str, circ_removed, num_cropped, unsettled)."""
+    nx, ny, pnx, pny = map(_f64, (nx, ny, pnx, pny))
+    tx, ty = _f64(tx).copy(), _f64(ty).copy()
+    ts = None if ts is None else _f64(ts).copy()
+    tr = None if tr is None else _f64(tr)
+    idx = _u32(idx)
+    circ, ncrop, nuns = _dbl(), _i64(), _i64()
+    rc = lib().oc_clear_inner(int(method), len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx),
+                              _p(pnx), _p(pny), float(cutoff_mult), float(ips), len(tx), _p(tx),
+                              _p(ty), _p(ts), _p(tr), int(max_passes), ctypes.byref(circ),
+                              ctypes.byref(ncrop), ctypes.byref(nuns))
+    assert rc == 0, rc
+    return dict(x=tx, y=ty, str=ts, circ_removed=circ.value, num_cropped=ncrop.value,
+                unsettled=nuns.value)
+
+
 # ----------------------------------------------------------------------------- reference build
 
 def ref_points_on_points(sx, sy, ss, sr, tx, ty, tr=None, restype="vel", dtype=np.float64):
@@ -366,3 +424,65 @@ def ref_finalize(u, v, fs):
     with quiet():
         ref().ref_finalize_d(len(u), _p(u), _p(v), float(fs[0]), float(fs[1]))
     return u, v
+
+
+def _f32nodes(nx, ny):
+    return np.ascontiguousarray(nx, np.float32), np.ascontiguousarray(ny, np.float32)
+
+
+def ref_nearest_distances(nx, ny, idx, tx, ty):
+    nx, ny = _f32nodes(nx, ny)
+    idx = _u32(idx)
+    tx, ty = _f64(tx), _f64(ty)
+    d = np.zeros(len(tx))
+    with quiet():
+        rc = ref().ref_nearest_distances_d(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx),
+                                           len(tx), _p(tx), _p(ty), _p(d))
+    assert rc == 0, rc
+    return d
+
+
+def ref_panel_normals(nx, ny, idx):
+    nx, ny = _f32nodes(nx, ny)
+    idx = _u32(idx)
+    pnx, pny = np.zeros(len(idx) // 2), np.zeros(len(idx) // 2)
+    with quiet():
+        rc = ref().ref_panel_normals_d(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), _p(pnx),
+                                       _p(pny))
+    assert rc == 0, rc
+    return pnx, pny
+
+
+def ref_reflect(nx, ny, idx, tx, ty):
+    nx, ny = _f32nodes(nx, ny)
+    idx = _u32(idx)
+    tx, ty = _f64(tx).copy(), _f64(ty).copy()
+    with quiet():
+        rc = ref().ref_reflect_d(len(nx), _p(nx), _p(ny), len(idx) // 2, _p(idx), len(tx), _p(tx),
+                                 _p(ty))
+    assert rc == 0, rc
+    return tx, ty
+
+
+def ref_clear_inner(method, nx, ny, idx, cutoff_mult, ips, tx, ty, ts=None, tr=None):
+    """Reference clear_inner_panp2<double>. Method 0 needs particles (ts and tr)."""
+    assert method == 1 or (ts is not None and tr is not None)
+    nx, ny = _f32nodes(nx, ny)
+    idx = _u32(idx)
+    tx, ty = _f64(tx).copy(), _f64(ty).copy()
+    ts = None if ts is None else _f64(ts).copy()
+    tr = None if tr is None else _f64(tr)
+    circ = np.zeros(1)
+    with quiet():
+        rc = ref().ref_clear_inner_d(int(method), len(nx), _p(nx), _p(ny), len(idx) // 2,
+                                     _p(idx), float(cutoff_mult), float(ips), len(tx), _p(tx),
+                                     _p(ty), _p(ts), _p(tr), _p(circ))
+    assert rc == 0, rc
+    return dict(x=tx, y=ty, str=ts, circ_removed=float(circ[0]))
+
+
+def ref_cut_table():
+    pos, sm, ds = (np.zeros(256) for _ in range(3))
+    rows = ref().ref_cut_table_d(256, _p(pos), _p(sm), _p(ds))
+    assert rows > 0, rows
+    return pos[:rows], sm[:rows], ds[:rows]

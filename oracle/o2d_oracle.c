@@ -32,6 +32,15 @@
  *                          equals the transposed sum).
  *   oc_morton_keys         (no reference counterpart; index work for the split-precision
  *                          form)  pinned: brute-force bit interleave in Python.
+ *   oc_nearest_distances   src/Reflect.h:792-826       pinned: golden vectors; brute force
+ *                          over segments in Python; distance to a circle's polygon.
+ *   oc_reflect             src/Reflect.h:167-311       pinned: golden vectors; mirror image
+ *                          across a flat wall; reflected points lie outside a convex body.
+ *   oc_cut_table           src/Reflect.h:333-394       pinned: golden table; the strength
+ *                          fraction is the 2D Gaussian's cumulative weight (-> 1 at +3 radii).
+ *   oc_clear_inner         src/Reflect.h:487-727       pinned: golden vectors (method 1 push
+ *                          and method 0 trim); pushed points sit at the cutoff layer of a flat
+ *                          wall; the removed circulation equals the strength lost.
  *
  * Deliberate difference from the reference, kept identical in the HIP library: at a coincident
  * pair (d^2 == 0) the Gaussian core's gradient factor is taken as 0 (the reference evaluates
@@ -40,6 +49,7 @@
  */
 #include "o2d_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -603,5 +613,305 @@ int oc_morton_keys(int64_t n, const double* x, const double* y, double x0, doubl
     qy = qy < 0.0 ? 0.0 : (qy > 65535.0 ? 65535.0 : qy);
     keys[i] = spread16((uint32_t)qx) | (spread16((uint32_t)qy) << 1);
   }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Particle-panel proximity (src/Reflect.h). Panels are straight two-node segments; a target's
+ * nearest element is a panel interior or a node, found by a scan over panels in index order
+ * with the reference's one-ulp tie window.
+ * ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  int is_node;      /* 0: panel interior, 1: node */
+  int64_t jidx;     /* panel index, or node index for a node hit */
+  double distsq, cpx, cpy;
+} oc_hit;
+
+/* panel_point_distance (src/Reflect.h:110-165). The reference's std::pow(v, 2) is v*v. */
+static oc_hit panel_point(double sx0, double sy0, double sx1, double sy1, double tx, double ty) {
+  oc_hit h;
+  const double bx = sx1 - sx0, by = sy1 - sy0;
+  const double blensq = bx * bx + by * by;
+  const double ax = tx - sx0, ay = ty - sy0;
+  const double t = (ax * bx + ay * by) / blensq;
+  if (t > 0.0 && t < 1.0) {
+    const double c = ay * bx - ax * by;
+    h.is_node = 0;
+    h.jidx = -1;
+    h.distsq = (c * c) / blensq;
+    h.cpx = (1.0 - t) * sx0 + t * sx1;
+    h.cpy = (1.0 - t) * sy0 + t * sy1;
+    return h;
+  }
+  h.is_node = 1;
+  const double rij2 = ax * ax + ay * ay;
+  const double ex = tx - sx1, ey = ty - sy1;
+  const double rij12 = ex * ex + ey * ey;
+  if (rij2 < rij12) {
+    h.jidx = 0;
+    h.distsq = rij2;
+    h.cpx = sx0;
+    h.cpy = sy0;
+  } else {
+    h.jidx = 1;
+    h.distsq = rij12;
+    h.cpx = sx1;
+    h.cpy = sy1;
+  }
+  return h;
+}
+
+/* node normals: the sum of the normals of the panels that use the node, added in panel order,
+ * end 0 before end 1 (src/Reflect.h:186-197, 510-521) */
+static void node_normals(int64_t nnodes, int64_t np, const uint32_t* idx, const double* pnx,
+                         const double* pny, double* nnx, double* nny) {
+  for (int64_t k = 0; k < nnodes; ++k) nnx[k] = nny[k] = 0.0;
+  for (int64_t j = 0; j < np; ++j) {
+    const uint32_t a = idx[2 * j], b = idx[2 * j + 1];
+    nnx[a] += pnx[j];
+    nny[a] += pny[j];
+    nnx[b] += pnx[j];
+    nny[b] += pny[j];
+  }
+}
+
+/* The scan over panels shared by reflect_panp2 and clear_inner_panp2 (src/Reflect.h:207-240,
+ * 553-584): a result strictly below nextafter(min, 0) replaces the hit list; one below
+ * nextafter(min, +max) joins it. Returns the number of hits; the first hit goes to *first and
+ * the sums of the hits' normals and contact points (in hit order) to sums[0..3]. */
+static int64_t scan_hits(double mindist, int64_t np, const uint32_t* idx, const double* nx,
+                         const double* ny, const double* pnx, const double* pny,
+                         const double* nnx, const double* nny, double tx, double ty,
+                         oc_hit* first, double* sums) {
+  int64_t nh = 0;
+  for (int64_t j = 0; j < np; ++j) {
+    const uint32_t a = idx[2 * j], b = idx[2 * j + 1];
+    oc_hit h = panel_point(nx[a], ny[a], nx[b], ny[b], tx, ty);
+    int take = 0;
+    if (h.distsq < nextafter(mindist, 0.0)) {
+      mindist = h.distsq;
+      nh = 0;
+      sums[0] = sums[1] = sums[2] = sums[3] = 0.0;
+      take = 1;
+    } else if (h.distsq < nextafter(mindist, DBL_MAX)) {
+      take = 1;
+    }
+    if (!take) continue;
+    h.jidx = h.is_node ? (int64_t)idx[2 * j + h.jidx] : j;
+    if (nh == 0) *first = h;
+    sums[0] += h.is_node ? nnx[h.jidx] : pnx[h.jidx];
+    sums[1] += h.is_node ? nny[h.jidx] : pny[h.jidx];
+    sums[2] += h.cpx;
+    sums[3] += h.cpy;
+    ++nh;
+  }
+  return nh;
+}
+
+int oc_nearest_distances(int64_t nnodes, const double* nx, const double* ny, int64_t np,
+                         const uint32_t* idx, int64_t nt, const double* tx, const double* ty,
+                         double* dist) {
+  for (int64_t j = 0; j < np; ++j)
+    if ((int64_t)idx[2 * j] >= nnodes || (int64_t)idx[2 * j + 1] >= nnodes) return -2;
+  for (int64_t i = 0; i < nt; ++i) {
+    double mindist = DBL_MAX;
+    for (int64_t j = 0; j < np; ++j) {
+      const uint32_t a = idx[2 * j], b = idx[2 * j + 1];
+      const oc_hit h = panel_point(nx[a], ny[a], nx[b], ny[b], tx[i], ty[i]);
+      if (h.distsq < nextafter(mindist, 0.0)) mindist = h.distsq;
+    }
+    dist[i] = sqrt(mindist);
+  }
+  return 0;
+}
+
+int oc_reflect(int64_t nnodes, const double* nx, const double* ny, int64_t np,
+               const uint32_t* idx, const double* pnx, const double* pny, int64_t nt, double* tx,
+               double* ty, int64_t* num_reflected) {
+  for (int64_t j = 0; j < np; ++j)
+    if ((int64_t)idx[2 * j] >= nnodes || (int64_t)idx[2 * j + 1] >= nnodes) return -2;
+  double* nnx = (double*)calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(double));
+  double* nny = (double*)calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(double));
+  if (!nnx || !nny) {
+    free(nnx);
+    free(nny);
+    return -1;
+  }
+  node_normals(nnodes, np, idx, pnx, pny, nnx, nny);
+  int64_t nref = 0;
+  for (int64_t i = 0; i < nt; ++i) {
+    oc_hit h0;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t nh = scan_hits(DBL_MAX, np, idx, nx, ny, pnx, pny, nnx, nny, tx[i], ty[i],
+                                 &h0, s);
+    if (nh == 0) continue; /* the reference asserts here */
+    double normx = s[0], normy = s[1], cpx = s[2], cpy = s[3];
+    if (normx * normx + normy * normy == 0.0) {
+      normx += h0.is_node ? nnx[h0.jidx] : pnx[h0.jidx];
+      normy += h0.is_node ? nny[h0.jidx] : pny[h0.jidx];
+      cpx = h0.cpx;
+      cpy = h0.cpy;
+    }
+    const double normilen = 1.0 / sqrt(normx * normx + normy * normy);
+    normx *= normilen;
+    normy *= normilen;
+    cpx /= (double)nh;
+    cpy /= (double)nh;
+    const double dotp = normx * (tx[i] - cpx) + normy * (ty[i] - cpy);
+    if (dotp < 0.0) {
+      const double dist = sqrt(h0.distsq);
+      tx[i] = cpx + dist * normx;
+      ty[i] = cpy + dist * normy;
+      ++nref;
+    }
+  }
+  free(nnx);
+  free(nny);
+  if (num_reflected) *num_reflected = nref;
+  return 0;
+}
+
+/* init_cut_tables (src/Reflect.h:333-394): (position, kept strength fraction, shift) rows of the
+ * cumulative 2D Gaussian, positions in units of the particle radius. Returns the row count. */
+int oc_cut_table(double dxin, int cap, double* pos, double* smult, double* dshift) {
+  const double max_rad = 3.0;
+  const int nx = (int)(0.5 + max_rad / dxin);
+  const double dx = max_rad / (double)(float)nx;
+  const int rows = 2 * nx + 3;
+  if (rows > cap) return -rows;
+  int r = 0;
+  pos[r] = (double)(-nx - 0.5) * dx;
+  smult[r] = 0.0;
+  dshift[r] = 0.0;
+  ++r;
+  double twgt = 0.0, tmom = 0.0;
+  for (int i = -nx; i < nx + 1; ++i) {
+    const double xi = (double)i * dx;
+    const double distxs = xi * xi;
+    double rwgt = 0.0;
+    for (int j = -nx; j < nx + 1; ++j) {
+      const double yj = (double)j * dx;
+      const double distxys = distxs + yj * yj;
+      rwgt += exp(-distxys);
+    }
+    rwgt *= dx * dx;
+    rwgt *= 1.0 / O2D_PI;
+    twgt += rwgt;
+    tmom += ((double)i * dx) * rwgt;
+    pos[r] = (double)(i + 0.5) * dx;
+    smult[r] = twgt;
+    dshift[r] = -tmom / twgt;
+    ++r;
+  }
+  pos[r] = (double)(nx + 1.5) * dx;
+  smult[r] = 1.0;
+  dshift[r] = 0.0;
+  ++r;
+  return r;
+}
+
+/* get_cut_entry (src/Reflect.h:400-432) */
+static void cut_entry(int rows, const double* pos, const double* sm, const double* ds, double p,
+                      double* smult, double* dshift) {
+  *smult = 1.0;
+  *dshift = 0.0;
+  if (p < pos[0]) {
+    *smult = 0.0;
+    *dshift = 1.0 - p;
+  } else if (p > pos[rows - 1]) {
+    *smult = 1.0;
+    *dshift = 0.0;
+  } else {
+    for (int i = 1; i < rows; ++i) {
+      if (p < pos[i]) {
+        const double frac = (p - pos[i - 1]) / (pos[i] - pos[i - 1]);
+        *smult = frac * sm[i] + (1.0 - frac) * sm[i - 1];
+        *dshift = frac * ds[i] + (1.0 - frac) * ds[i - 1];
+        break;
+      }
+    }
+  }
+}
+
+/* clear_inner_panp2 (src/Reflect.h:487-727). Each target is re-tested until it passes, as the
+ * reference's outer while loop does; a target still moving after max_passes tests is counted
+ * in *unsettled and left where it is. The removed circulation is summed per target in pass
+ * order, then over targets in index order. */
+int oc_clear_inner(int method, int64_t nnodes, const double* nx, const double* ny, int64_t np,
+                   const uint32_t* idx, const double* pnx, const double* pny, double cutoff_mult,
+                   double ips, int64_t nt, double* tx, double* ty, double* ts, const double* tr,
+                   int max_passes, double* circ_removed, int64_t* num_cropped,
+                   int64_t* unsettled) {
+  if (method != 0 && method != 1) return -1;
+  if (method == 0 && tr && !ts) return -1;
+  for (int64_t j = 0; j < np; ++j)
+    if ((int64_t)idx[2 * j] >= nnodes || (int64_t)idx[2 * j + 1] >= nnodes) return -2;
+  double cpos[128], csm[128], cds[128];
+  const int rows = oc_cut_table(0.1, 128, cpos, csm, cds);
+  if (rows <= 0) return -1;
+  double* nnx = (double*)calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(double));
+  double* nny = (double*)calloc((size_t)(nnodes > 0 ? nnodes : 1), sizeof(double));
+  if (!nnx || !nny) {
+    free(nnx);
+    free(nny);
+    return -1;
+  }
+  node_normals(nnodes, np, idx, pnx, pny, nnx, nny);
+  const double start = 0.1 * DBL_MAX;
+  double circ = 0.0;
+  int64_t ncrop = 0, nuns = 0;
+  for (int64_t i = 0; i < nt; ++i) {
+    double mine = 0.0;
+    int settled = 0;
+    for (int pass = 0; pass < max_passes && !settled; ++pass) {
+      oc_hit h0;
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      const int64_t nh = scan_hits(start, np, idx, nx, ny, pnx, pny, nnx, nny, tx[i], ty[i],
+                                   &h0, s);
+      if (nh == 0) continue; /* the reference skips the target and tests it again */
+      double normx = s[0], normy = s[1], cpx = s[2], cpy = s[3];
+      const double normilen = 1.0 / sqrt(normx * normx + normy * normy);
+      normx *= normilen;
+      normy *= normilen;
+      cpx /= (double)nh;
+      cpy /= (double)nh;
+      const double dotp =
+          normx * (tx[i] - cpx) + normy * (ty[i] - cpy) - cutoff_mult * ips;
+      if (method == 0) {
+        /* field points (no radii) take the spacing as radius and are never modified, so
+         * one below the layer stays unsettled (the reference loops on it for ever) */
+        const double rad = tr ? tr[i] : ips;
+        if (dotp < rad) {
+          if (tr) {
+            double sm, dsh;
+            cut_entry(rows, cpos, csm, cds, dotp / rad, &sm, &dsh);
+            mine += ts[i] * (1.0 - sm);
+            ts[i] *= sm;
+            tx[i] += dsh * rad * normx;
+            ty[i] += dsh * rad * normy;
+          }
+          ++ncrop;
+        } else {
+          settled = 1;
+        }
+      } else {
+        if (dotp < -0.001 * ips) {
+          tx[i] -= dotp * normx;
+          ty[i] -= dotp * normy;
+          ++ncrop;
+        } else {
+          settled = 1;
+        }
+      }
+    }
+    if (!settled) ++nuns;
+    circ += mine;
+  }
+  free(nnx);
+  free(nny);
+  if (circ_removed) *circ_removed = circ;
+  if (num_cropped) *num_cropped = ncrop;
+  if (unsettled) *unsettled = nuns;
   return 0;
 }

@@ -61,6 +61,26 @@ int oc_points_on_points_adjoint(int core, int64_t ns, const double* sx, const do
 int oc_morton_keys(int64_t n, const double* x, const double* y, double x0, double y0,
                    double inv_extent, uint32_t* keys);
 
+/* src/Reflect.h: nearest distance from each target to a set of panels, reflection of targets
+ * that lie behind the nearest panel, and the near-wall clearing pass (method 1 pushes targets
+ * out to the cutoff layer, method 0 trims strength with the cut tables). pnx/pny are the
+ * panels' unit normals (Surfaces::get_norm). */
+int oc_nearest_distances(int64_t nnodes, const double* nx, const double* ny, int64_t np,
+                         const uint32_t* idx, int64_t nt, const double* tx, const double* ty,
+                         double* dist);
+
+int oc_reflect(int64_t nnodes, const double* nx, const double* ny, int64_t np,
+               const uint32_t* idx, const double* pnx, const double* pny, int64_t nt, double* tx,
+               double* ty, int64_t* num_reflected);
+
+int oc_cut_table(double dx, int cap, double* pos, double* smult, double* dshift);
+
+int oc_clear_inner(int method, int64_t nnodes, const double* nx, const double* ny, int64_t np,
+                   const uint32_t* idx, const double* pnx, const double* pny, double cutoff_mult,
+                   double ips, int64_t nt, double* tx, double* ty, double* ts, const double* tr,
+                   int max_passes, double* circ_removed, int64_t* num_cropped,
+                   int64_t* unsettled);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,7 +1,8 @@
 // ref_harness.cpp - a C-ABI shim over the Omega2D reference's own templates.
 //
 // TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile against the reference headers where
-// they lie (src/Influence.h, src/Coefficients.h, src/RHS.h, src/Points.h, src/Surfaces.h),
+// they lie (src/Influence.h, src/Coefficients.h, src/RHS.h, src/Reflect.h, src/Points.h,
+// src/Surfaces.h),
 // never copied. The shared library lands in oracle/_ref/ and is used only to write the golden
 // fixtures under tests/golden/, to validate the C restatement in oracle/o2d_oracle.c, and as
 // the "reference" CPU baseline timed beside the GPU numbers. Nothing in the product links it.
@@ -14,6 +15,7 @@
 #include "Influence.h"
 #include "Coefficients.h"
 #include "RHS.h"
+#include "Reflect.h"
 
 #include <cstdint>
 #include <cstring>
@@ -286,6 +288,71 @@ int ref_finalize_d(int64_t n, double* u, double* v, double fsx, double fsy) {
   copy_out<double>(p.get_vel()[0], u);
   copy_out<double>(p.get_vel()[1], v);
   return 0;
+}
+
+// get_nearest_distances (src/Reflect.h:792-826) from inert targets to panels
+int ref_nearest_distances_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                            const uint32_t* idx, int64_t nt, const double* tx, const double* ty,
+                            double* dist) {
+  HarnessSurfaces<double> surf =
+      make_surfaces<double>(nnodes, nx, ny, np, idx, reactive, fixed, nullptr);
+  Points<double> targ = make_points<double>(nt, tx, ty, nullptr, nullptr, true);
+  Vector<double> d = get_nearest_distances<double>(surf, targ);
+  copy_out<double>(d, dist);
+  return 0;
+}
+
+// the panels' unit normals as Surfaces::compute_bases stores them (src/Surfaces.h:682-725)
+int ref_panel_normals_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                        const uint32_t* idx, double* pnx, double* pny) {
+  HarnessSurfaces<double> surf =
+      make_surfaces<double>(nnodes, nx, ny, np, idx, reactive, fixed, nullptr);
+  copy_out<double>(surf.get_norm()[0], pnx);
+  copy_out<double>(surf.get_norm()[1], pny);
+  return 0;
+}
+
+// reflect_panp2 (src/Reflect.h:167-311) on particles (radius 0.01, strength 1)
+int ref_reflect_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                  const uint32_t* idx, int64_t nt, double* tx, double* ty) {
+  HarnessSurfaces<double> surf =
+      make_surfaces<double>(nnodes, nx, ny, np, idx, reactive, fixed, nullptr);
+  std::vector<double> s(nt, 1.0), r(nt, 0.01);
+  Points<double> targ = make_points<double>(nt, tx, ty, s.data(), r.data(), false);
+  reflect_panp2<double>(surf, targ);
+  copy_out<double>(targ.get_pos()[0], tx);
+  copy_out<double>(targ.get_pos()[1], ty);
+  return 0;
+}
+
+// clear_inner_panp2 (src/Reflect.h:487-727). tr == NULL makes the targets field points (no
+// radius); method 0 on field points below the layer never returns in the reference, so the
+// caller must not ask for it. Returns the removed circulation in *circ.
+int ref_clear_inner_d(int method, int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                      const uint32_t* idx, double cutoff_mult, double ips, int64_t nt,
+                      double* tx, double* ty, double* ts, const double* tr, double* circ) {
+  HarnessSurfaces<double> surf =
+      make_surfaces<double>(nnodes, nx, ny, np, idx, reactive, fixed, nullptr);
+  std::vector<double> s(nt, 0.0);
+  if (ts) s.assign(ts, ts + nt);
+  Points<double> targ = make_points<double>(nt, tx, ty, tr ? s.data() : nullptr, tr, tr == nullptr);
+  *circ = clear_inner_panp2<double>(method, surf, targ, cutoff_mult, ips);
+  copy_out<double>(targ.get_pos()[0], tx);
+  copy_out<double>(targ.get_pos()[1], ty);
+  if (tr && ts) copy_out<double>(targ.get_str(), ts);
+  return 0;
+}
+
+// init_cut_tables<double>(0.1) (src/Reflect.h:333-394); returns the row count
+int ref_cut_table_d(int cap, double* pos, double* smult, double* dshift) {
+  std::vector<std::tuple<double, double, double>> ct = init_cut_tables<double>(0.1);
+  if ((int)ct.size() > cap) return -(int)ct.size();
+  for (size_t i = 0; i < ct.size(); ++i) {
+    pos[i] = std::get<0>(ct[i]);
+    smult[i] = std::get<1>(ct[i]);
+    dshift[i] = std::get<2>(ct[i]);
+  }
+  return (int)ct.size();
 }
 
 }  // extern "C"

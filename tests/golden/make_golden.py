@@ -2,7 +2,7 @@
 
 Runs in a container that has the reference tree: it calls oracle/_ref/libo2d_ref.so (built by
 ``make -C oracle ref`` from src/Influence.h, src/Coefficients.h, src/RHS.h, src/Points.h,
-src/Surfaces.h, src/ElementBase.h where they lie) on small seeded inputs and stores inputs and
+src/Surfaces.h, src/ElementBase.h, src/Reflect.h where they lie) on small seeded inputs and stores inputs and
 outputs. The fixture is data only: inputs and the reference's outputs.
 
     python tests/golden/make_golden.py
@@ -97,6 +97,52 @@ def main():
     d["mv2_x"], d["mv2_y"] = O.ref_move(x, y, us[:2], vv[:2], [0.25, 0.75], 0.01)
     d["mv3_x"], d["mv3_y"] = O.ref_move(x, y, us, vv, [2 / 9, 3 / 9, 4 / 9], 0.01)
     d["fin_u"], d["fin_v"] = O.ref_finalize(us[0], vv[0], (1.0, -0.5))
+
+    # particle/panel proximity (src/Reflect.h), its own seed so the arrays above do not move
+    r2 = np.random.default_rng(20241202)
+    cnp = 48
+    cx_, cy_, cidx = circle_panels(cnp)
+    d.update(rf_nx=cx_, rf_ny=cy_, rf_idx=cidx)
+    d["rf_pnx"], d["rf_pny"] = O.ref_panel_normals(cx_, cy_, cidx)
+    nrf = 700
+    rho, ang = r2.uniform(0.0, 0.8, nrf), r2.uniform(0.0, 2 * np.pi, nrf)
+    fx, fy = rho * np.cos(ang), rho * np.sin(ang)
+    d.update(rf_tx=fx, rf_ty=fy)
+    d["rf_dist"] = O.ref_nearest_distances(cx_, cy_, cidx, fx, fy)
+    d["rf_refl_x"], d["rf_refl_y"] = O.ref_reflect(cx_, cy_, cidx, fx, fy)
+    cm, ips = 0.5 / np.sqrt(2.0 * np.pi), 0.05
+    d["rf_cut"] = np.array([cm, ips])
+    ps, pr = r2.normal(0, 1e-3, nrf), r2.uniform(0.015, 0.025, nrf)
+    d.update(rf_ts=ps, rf_tr=pr)
+    R = O.ref_clear_inner(1, cx_, cy_, cidx, cm, ips, fx, fy, ps, pr)
+    d["rf_c1_x"], d["rf_c1_y"] = R["x"], R["y"]
+    R = O.ref_clear_inner(1, cx_, cy_, cidx, cm, ips, fx, fy)
+    d["rf_c1f_x"], d["rf_c1f_y"] = R["x"], R["y"]
+    # method 0 on a shell that starts inside the cut table (a target more than 3.05 radii
+    # below the layer is moved to exactly 1 radius above it, where rounding decides whether
+    # it is trimmed once more), and on a deep set for the band check
+    for tag, lo, hi in (("sh", 0.44, 0.62), ("dp", 0.0, 0.40)):
+        rr = r2.uniform(lo, hi, 400)
+        aa = r2.uniform(0.0, 2 * np.pi, 400)
+        mx, my = rr * np.cos(aa), rr * np.sin(aa)
+        ms, mr = r2.normal(0, 1e-3, 400), np.full(400, 0.025)
+        R = O.ref_clear_inner(0, cx_, cy_, cidx, cm, ips, mx, my, ms, mr)
+        d.update({f"rf_c0{tag}_tx": mx, f"rf_c0{tag}_ty": my, f"rf_c0{tag}_ts": ms,
+                  f"rf_c0{tag}_tr": mr, f"rf_c0{tag}_x": R["x"], f"rf_c0{tag}_y": R["y"],
+                  f"rf_c0{tag}_s": R["str"], f"rf_c0{tag}_circ": np.array([R["circ_removed"]])})
+    d["rf_ct_pos"], d["rf_ct_sm"], d["rf_ct_ds"] = O.ref_cut_table()
+    # an open wall with a spur: a node shared by three panels
+    tnx = np.array([-1.0, -0.5, 0.0, 0.5, 1.0, 0.0, 0.0], np.float32)
+    tny = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 0.5, 1.0], np.float32)
+    tidx = np.array([0, 1, 1, 2, 2, 3, 3, 4, 2, 5, 5, 6], np.uint32)
+    d.update(rt_nx=tnx, rt_ny=tny, rt_idx=tidx)
+    d["rt_pnx"], d["rt_pny"] = O.ref_panel_normals(tnx, tny, tidx)
+    gx, gy = r2.uniform(-1.2, 1.2, 500), r2.uniform(-0.5, 1.2, 500)
+    d.update(rt_tx=gx, rt_ty=gy)
+    d["rt_dist"] = O.ref_nearest_distances(tnx, tny, tidx, gx, gy)
+    d["rt_refl_x"], d["rt_refl_y"] = O.ref_reflect(tnx, tny, tidx, gx, gy)
+    R = O.ref_clear_inner(1, tnx, tny, tidx, cm, ips, gx, gy)
+    d["rt_c1_x"], d["rt_c1_y"] = R["x"], R["y"]
 
     np.savez_compressed(OUT, **d)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(d)} arrays)")

@@ -1,0 +1,189 @@
+"""GPU parity of the particle/panel proximity kernels (src/Reflect.h) against the oracle.
+
+The kernels and the oracle both compute in IEEE double without fused multiply-adds in the
+reference's operation order, so nearest distances, reflected and pushed positions, trimmed
+strengths and move counts are compared bit for bit. The removed circulation is a sum over
+targets in a different order: relative 1e-13.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import omega2d_amd as o2d
+from helpers import circle_panels
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
+CM, IPS = 0.5 / np.sqrt(2.0 * np.pi), 0.05
+
+
+def _circle(npan):
+    nx, ny, idx = circle_panels(npan)
+    nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+    _, _, pnx, pny, _ = O.panel_geometry(nx, ny, idx)
+    return nx, ny, idx, pnx, pny
+
+
+def _spur():
+    g = GOLD
+    return (g["rt_nx"].astype(np.float64), g["rt_ny"].astype(np.float64), g["rt_idx"],
+            g["rt_pnx"], g["rt_pny"])
+
+
+def _targets(rng, n, rmax=0.8):
+    rho, a = rng.uniform(0.0, rmax, n), rng.uniform(0, 2 * np.pi, n)
+    return rho * np.cos(a), rho * np.sin(a)
+
+
+GEOMS = {"circle48": lambda: _circle(48), "circle129": lambda: _circle(129),
+         "spur": _spur, "one_panel": lambda: (np.array([-0.5, 0.5]), np.array([0.0, 0.0]),
+                                              np.array([0, 1], np.uint32), np.array([0.0]),
+                                              np.array([1.0]))}
+
+
+@pytest.mark.parametrize("geom", list(GEOMS))
+@pytest.mark.parametrize("nt", [1, 255, 257, 3000])
+def test_nearest_distances_bit_exact(geom, nt):
+    nx, ny, idx, _, _ = GEOMS[geom]()
+    tx, ty = _targets(np.random.default_rng(nt), nt)
+    got = o2d.nearest_distances(nx, ny, idx, tx, ty)
+    ref = O.nearest_distances(nx, ny, idx, tx, ty)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("geom", list(GEOMS))
+def test_reflect_bit_exact(geom):
+    nx, ny, idx, pnx, pny = GEOMS[geom]()
+    tx, ty = _targets(np.random.default_rng(11), 2500)
+    gx, gy, gn = o2d.reflect(nx, ny, idx, pnx, pny, tx, ty)
+    rx, ry, rn = O.reflect(nx, ny, idx, pnx, pny, tx, ty)
+    assert gn == rn
+    assert np.array_equal(gx, rx) and np.array_equal(gy, ry)
+
+
+@pytest.mark.parametrize("geom", list(GEOMS))
+@pytest.mark.parametrize("kind", ["particles", "field"])
+def test_clear_push_bit_exact(geom, kind):
+    nx, ny, idx, pnx, pny = GEOMS[geom]()
+    rng = np.random.default_rng(12)
+    tx, ty = _targets(rng, 2500)
+    ts, tr = (rng.normal(0, 1e-3, 2500), rng.uniform(0.015, 0.025, 2500)) \
+        if kind == "particles" else (None, None)
+    got = o2d.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, tx, ty, ts, tr)
+    ref = O.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, tx, ty, ts, tr)
+    assert ref["unsettled"] == 0
+    assert got["num_cropped"] == ref["num_cropped"]
+    assert np.array_equal(got["x"], ref["x"]) and np.array_equal(got["y"], ref["y"])
+    assert got["circ_removed"] == 0.0
+
+
+@pytest.mark.parametrize("shell", ["sh", "dp"])
+def test_clear_trim_bit_exact(shell):
+    g = GOLD
+    nx, ny, idx, pnx, pny = (g["rf_nx"].astype(np.float64), g["rf_ny"].astype(np.float64),
+                             g["rf_idx"], g["rf_pnx"], g["rf_pny"])
+    args = (g[f"rf_c0{shell}_tx"], g[f"rf_c0{shell}_ty"], g[f"rf_c0{shell}_ts"],
+            g[f"rf_c0{shell}_tr"])
+    got = o2d.clear_inner(0, nx, ny, idx, pnx, pny, CM, IPS, *args)
+    ref = O.clear_inner(0, nx, ny, idx, pnx, pny, CM, IPS, *args)
+    assert got["num_cropped"] == ref["num_cropped"]
+    for k in ("x", "y", "str"):
+        assert np.array_equal(got[k], ref[k])
+    assert abs(got["circ_removed"] - ref["circ_removed"]) <= 1e-13 * abs(ref["circ_removed"])
+
+
+def test_clear_at_bem_wake_scale():
+    """The C3 body (2048 panels) against 100,000 targets in and around it, method 1."""
+    nx, ny, idx, pnx, pny = _circle(2048)
+    rng = np.random.default_rng(33)
+    tx, ty = _targets(rng, 100_000, rmax=1.0)
+    got = o2d.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, tx, ty)
+    ref = O.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, tx, ty)
+    assert got["num_cropped"] == ref["num_cropped"] > 0
+    assert np.array_equal(got["x"], ref["x"]) and np.array_equal(got["y"], ref["y"])
+    d = o2d.nearest_distances(nx, ny, idx, tx, ty)
+    assert np.array_equal(d, O.nearest_distances(nx, ny, idx, tx, ty))
+
+
+def test_empty_inputs():
+    nx, ny, idx, pnx, pny = _circle(16)
+    z = np.zeros(0)
+    assert o2d.nearest_distances(nx, ny, idx, z, z).shape == (0,)
+    d = o2d.nearest_distances(z, z, np.zeros(0, np.uint32), np.ones(3), np.ones(3))
+    assert np.all(d == np.sqrt(np.finfo(np.float64).max))
+    x, y, n = o2d.reflect(z, z, np.zeros(0, np.uint32), z, z, np.ones(3), np.ones(3))
+    assert n == 0 and np.all(x == 1.0)
+    r = o2d.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, z, z)
+    assert r["num_cropped"] == 0 and r["circ_removed"] == 0.0
+
+
+def test_host_entry_errors():
+    nx, ny, idx, pnx, pny = _circle(16)
+    t = np.array([0.0]), np.array([0.0])
+    with pytest.raises(o2d.O2DError) as e:
+        o2d.clear_inner(2, nx, ny, idx, pnx, pny, CM, IPS, *t)
+    assert e.value.code == -1
+    bad = idx.copy()
+    bad[5] = 99
+    with pytest.raises(o2d.O2DError) as e:
+        o2d.nearest_distances(nx, ny, bad, *t)
+    assert e.value.code == -2
+    dup = idx.copy()
+    dup[1] = dup[0]  # panel 0 from a node to itself
+    with pytest.raises(o2d.O2DError) as e:
+        o2d.reflect(nx, ny, dup, pnx, pny, *t)
+    assert e.value.code == -1
+
+
+def test_unsettled_targets_are_reported_after_writing_results():
+    nx, ny = np.linspace(-2, 2, 9), np.zeros(9)
+    idx = np.stack([np.arange(8), np.arange(1, 9)], 1).astype(np.uint32).ravel()
+    pnx, pny = np.zeros(8), np.ones(8)
+    tx, ty = np.array([0.1, 0.2, 0.3]), np.array([-0.1, 0.3, 0.05])
+    with pytest.raises(o2d.O2DError) as e:
+        o2d.clear_inner(0, nx, ny, idx, pnx, pny, CM, IPS, tx, ty)
+    assert e.value.code == -7 and "2 targets" in str(e.value)
+    ref = O.clear_inner(0, nx, ny, idx, pnx, pny, CM, IPS, tx, ty, max_passes=64)
+    assert ref["unsettled"] == 2
+
+
+def test_device_entries_match_host_entries():
+    import torch
+    import omega2d_amd.device as dv
+    nx, ny, idx, pnx, pny = _spur()
+    rng = np.random.default_rng(44)
+    tx, ty = rng.uniform(-1.2, 1.2, 5000), rng.uniform(-0.5, 1.2, 5000)
+    ts, tr = rng.normal(0, 1e-3, 5000), np.full(5000, 0.02)
+    cu = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)  # noqa: E731
+    dnx, dny, dpnx, dpny = cu(nx), cu(ny), cu(pnx), cu(pny)
+    didx = cu(idx.astype(np.int32), torch.int32)
+    dist = dv.nearest_distances(dnx, dny, didx, cu(tx), cu(ty))
+    assert np.array_equal(dist.cpu().numpy(), o2d.nearest_distances(nx, ny, idx, tx, ty))
+    x, y = cu(tx), cu(ty)
+    c = dv.reflect(dnx, dny, didx, dpnx, dpny, x, y)
+    hx, hy, hn = o2d.reflect(nx, ny, idx, pnx, pny, tx, ty)
+    assert np.array_equal(x.cpu().numpy(), hx) and np.array_equal(y.cpu().numpy(), hy)
+    assert c.cpu().tolist() == [hn, 0, 0]
+    for method in (1, 0):
+        x, y, s, r = cu(tx), cu(ty), cu(ts), cu(tr)
+        circ, c = dv.clear_inner(method, dnx, dny, didx, dpnx, dpny, CM, IPS, x, y, s, r)
+        h = o2d.clear_inner(method, nx, ny, idx, pnx, pny, CM, IPS, tx, ty, ts, tr)
+        assert np.array_equal(x.cpu().numpy(), h["x"]) and np.array_equal(y.cpu().numpy(), h["y"])
+        assert np.array_equal(s.cpu().numpy(), h["str"])
+        assert c.cpu().tolist() == [h["num_cropped"], 0, 0]
+        assert float(circ.cpu()[0]) == h["circ_removed"]
+
+
+def test_device_entry_counts_bad_panels_without_reading_out_of_range():
+    import torch
+    import omega2d_amd.device as dv
+    nx, ny, idx, pnx, pny = _circle(32)
+    bad = idx.astype(np.int32).copy()
+    bad[7] = 1000  # clamped to the last node on the device and counted
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")  # noqa: E731
+    x, y = cu(np.array([0.0, 0.9])), cu(np.array([0.0, 0.0]))
+    c = dv.reflect(cu(nx), cu(ny), cu(bad), cu(pnx), cu(pny), x, y)
+    assert c.cpu().tolist()[2] == 1
