@@ -101,6 +101,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} "
                                "(run python -m omega2d_amd.build)")
+        # torch ships its own HIP runtime and asks for it by an unversioned name, so it cannot
+        # reuse a runtime this library loaded first: two runtimes would then share one HSA
+        # runtime and torch would find no GPU. Loading torch first makes this library bind to
+        # torch's runtime (same soname), so the process has exactly one.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)

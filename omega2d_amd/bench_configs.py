@@ -5,7 +5,8 @@
 C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update
 C3  circle of 2048 panels + 200k wake, V2 core: BEM assembly (vortex-only + augmented 2049^2,
     and vortex+source + augmented 4097^2) with HBM GB/s, LU factor + solve, RHS (points on
-    panels), panels on particles, particles on particles (split and plain)
+    panels), panels on particles, particles on particles (split and plain), and the
+    particle/panel proximity scans run after every move (clearing pass, nearest distances)
 C4  250k particles -> 4096 probes, Gaussian core: forward and adjoint (fp64), ratio
 C5  4M particles in a shear layer, RM core, one Ralston RK3 step in the split form with the
     fp64 position update; error of the split velocity against the fp64 form on 16,384 targets
@@ -25,7 +26,7 @@ import torch
 from . import device as dv
 from . import flops_per_pair, workloads as wl
 from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_PLAIN_PT,
-                    KT_NBODY_SPLIT, KT_PANEL_RHS, KT_PANELS, timing_enable, timing_read)
+                    KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS, timing_enable, timing_read)
 
 PEAK_FP32, PEAK_FP64, PEAK_HBM = 157.3e12, 78.6e12, 8.0e12
 
@@ -137,6 +138,21 @@ def c3():
                                     "Gint_s": round(float(nw) * nw / (ms * 1e-3) / 1e9, 1),
                                     "frac_peak": round(fp * float(nw) * nw / (ms * 1e-3) /
                                                        (PEAK_FP32 if prec == "split" else PEAK_FP64), 4)}
+    # proximity scans (src/Reflect.h): one clearing pass (method 1, the reference's cutoff
+    # 0.5/sqrt(2 pi) at spacing 0.005) and the nearest distances, every wake particle against
+    # every panel; 27 flops per (particle, panel) in the reference's count (src/Reflect.h:719)
+    a, b = w["idx"][0::2], w["idx"][1::2]
+    tx_, ty_ = w["nx"][b] - w["nx"][a], w["ny"][b] - w["ny"][a]
+    ln = np.sqrt(tx_ * tx_ + ty_ * ty_)
+    pnx, pny = T(-ty_ * (1.0 / ln)), T(tx_ * (1.0 / ln))
+    cm, ips = 0.5 / math.sqrt(2.0 * math.pi), 0.005
+    for name, fn in (("clear_inner", lambda: dv.clear_inner(1, nx, ny, idx, pnx, pny, cm, ips,
+                                                            x.clone(), y.clone())),
+                     ("nearest_distances", lambda: dv.nearest_distances(nx, ny, idx, x, y))):
+        ms, _ = kernel_ms(KT_NEAR, fn, reps=3)
+        res[name] = {"kernel_ms": round(ms, 4),
+                     "Gint_s": round(float(nw) * npan / (ms * 1e-3) / 1e9, 2),
+                     "frac_fp64": round(27.0 * nw * npan / (ms * 1e-3) / PEAK_FP64, 4)}
     us, vs_ = vel(x, y, s, r, x, y, r, "v2", "split")
     u6, v6 = vel(x, y, s, r, x, y, r, "v2", "fp64")
     res["split_err_vs_fp64"] = math.sqrt((rel(us, u6) ** 2 * float((u6 ** 2).sum())

@@ -1,8 +1,8 @@
 // Convection.h - velocity evaluation and Runge-Kutta advection of the gpu_hip environment: host
 // mirror of src/Convection.h:172-598 (find_vels, find_derivs, advect with Euler, Ralston RK2
-// and Ralston RK3). Positions advance in fp64 on the device (o2d_hip_rk_move_d). The
-// near-wall clean-up the reference runs after each move (clear_inner_layer, src/Reflect.h) is
-// not part of this path.
+// and Ralston RK3). Positions advance in fp64 on the device (o2d_hip_rk_move_d); after every
+// move the Lagrangian points are pushed out of the bodies' inner layer (clear_inner_layer,
+// method 1, src/Reflect.h) exactly where the reference does it.
 #pragma once
 
 #include <array>
@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "BEM.h"
+#include "Reflect.h"
 
 namespace omega2d {
 
@@ -74,65 +75,84 @@ class Convection {
     find_vels(fs, vort, bdry, fldpt);
   }
 
-  void advect(double time, double dt, const std::array<double, Dimensions>& fs,
+  // src/Convection.h:249-268; ips is the nominal particle spacing that sets the inner layer
+  void advect(double time, double dt, const std::array<double, Dimensions>& fs, S ips,
               std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
               std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
-    if (order == 1) advect_1st(time, dt, fs, vort, bdry, fldpt, bem);
-    else if (order == 2) advect_2nd_ralston(time, dt, fs, vort, bdry, fldpt, bem);
-    else if (order == 3) advect_3rd(time, dt, fs, vort, bdry, fldpt, bem);
+    if (order == 1) advect_1st(time, dt, fs, ips, vort, bdry, fldpt, bem);
+    else if (order == 2) advect_2nd_ralston(time, dt, fs, ips, vort, bdry, fldpt, bem);
+    else if (order == 3) advect_3rd(time, dt, fs, ips, vort, bdry, fldpt, bem);
     else throw std::runtime_error("convection order must be 1, 2 or 3");
   }
 
   // src/Convection.h:274-303
-  void advect_1st(double time, double dt, const std::array<double, Dimensions>& fs,
+  void advect_1st(double time, double dt, const std::array<double, Dimensions>& fs, S ips,
                   std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
                   std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
     find_derivs(time, fs, bem, bdry, vort, fldpt);
     for (auto* list : {&vort, &fldpt})
       for (auto& c : *list)
         if (auto* p = std::get_if<Points<S>>(&c)) move_points<S>(*p, dt, {1.0}, {p}, dev());
+    clear(bdry, vort, ips);
+    clear(bdry, fldpt, ips);
   }
 
   // src/Convection.h:390-466: stage at t + 2dt/3, u = u1/4 + 3 u2/4
   void advect_2nd_ralston(double time, double dt, const std::array<double, Dimensions>& fs,
-                          std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
-                          std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
+                          S ips, std::vector<Collection<S>>& vort,
+                          std::vector<Collection<S>>& bdry, std::vector<Collection<S>>& fldpt,
+                          BEM<S, I>& bem) {
     const double twothirds = 2.0 / 3.0;
     find_derivs(time, fs, bem, bdry, vort, fldpt);
     std::vector<Collection<S>> iv = vort, ifl = fldpt;
-    for (auto* list : {&iv, &ifl})
+    for (auto* list : {&iv, &ifl}) {
       for (auto& c : *list)
         if (auto* p = std::get_if<Points<S>>(&c)) move_points<S>(*p, twothirds * dt, {1.0}, {p}, dev());
+      clear(bdry, *list, ips);
+    }
     find_derivs(time + twothirds * dt, fs, bem, bdry, iv, ifl);
     combine(vort, {&iv}, dt, {0.25, 0.75});
     combine(fldpt, {&ifl}, dt, {0.25, 0.75});
+    clear(bdry, vort, ips);
+    clear(bdry, fldpt, ips);
   }
 
   // src/Convection.h:472-598: stages at t, t + dt/2, t + 3dt/4 (the third from x0 with u1),
   // weights 2/9, 3/9, 4/9
-  void advect_3rd(double time, double dt, const std::array<double, Dimensions>& fs,
+  void advect_3rd(double time, double dt, const std::array<double, Dimensions>& fs, S ips,
                   std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
                   std::vector<Collection<S>>& fldpt, BEM<S, I>& bem) {
     find_derivs(time, fs, bem, bdry, vort, fldpt);
     std::vector<Collection<S>> v1 = vort, f1 = fldpt;
-    for (auto* list : {&v1, &f1})
+    for (auto* list : {&v1, &f1}) {
       for (auto& c : *list)
         if (auto* p = std::get_if<Points<S>>(&c)) move_points<S>(*p, 0.5 * dt, {1.0}, {p}, dev());
+      clear(bdry, *list, ips);
+    }
     find_derivs(time + 0.5 * dt, fs, bem, bdry, v1, f1);
     std::vector<Collection<S>> v2 = vort, f2 = fldpt;
-    for (auto pr : {std::make_pair(&v2, &v1), std::make_pair(&f2, &f1)})
+    for (auto pr : {std::make_pair(&v2, &v1), std::make_pair(&f2, &f1)}) {
       for (size_t i = 0; i < pr.first->size(); ++i) {
         auto* p2 = std::get_if<Points<S>>(&(*pr.first)[i]);
         auto* p1 = std::get_if<Points<S>>(&(*pr.second)[i]);
         if (p2 && p1) move_points<S>(*p2, 0.75 * dt, {1.0}, {p1}, dev());
       }
+      clear(bdry, *pr.first, ips);
+    }
     find_derivs(time + 0.75 * dt, fs, bem, bdry, v2, f2);
     combine(vort, {&v1, &v2}, dt, {2.0 / 9.0, 3.0 / 9.0, 4.0 / 9.0});
     combine(fldpt, {&f1, &f2}, dt, {2.0 / 9.0, 3.0 / 9.0, 4.0 / 9.0});
+    clear(bdry, vort, ips);
+    clear(bdry, fldpt, ips);
   }
 
  private:
   int32_t dev() const { return env.get_device(); }
+
+  // the reference's push after every move: method 1, layer 0.5/sqrt(2 pi) spacings thick
+  void clear(std::vector<Collection<S>>& bdry, std::vector<Collection<S>>& pts, S ips) {
+    clear_inner_layer<S>(1, bdry, pts, S(0.5 / std::sqrt(2.0 * M_PI)), ips, env);
+  }
 
   void combine(std::vector<Collection<S>>& base, std::vector<std::vector<Collection<S>>*> more,
                double dt, const std::vector<double>& w) {

@@ -266,6 +266,8 @@ class Surfaces {
   S get_last_body_circ() const { return S(2.0) * vol * this_omega; }
   S get_last_body_circ_error() const { return S(2.0 * double(vol) * double(omega_error)); }
   S get_reabsorbed() const { return reabsorbed_gamma; }
+  // src/Surfaces.h:1053-1055: circulation taken from the particles by the clearing pass
+  void add_to_reabsorbed(S circ) { reabsorbed_gamma += circ; }
   S get_solved_omega() const { return solved_omega; }
 
   std::string to_string() const {

@@ -6,7 +6,8 @@
 //   o2d_host_driver <mode> <in.bin> <out.bin>
 //   mode: rk       one advection step of free particles (order, core, storage from the input)
 //         bem      BEM solve on a reactive circle + velocities of the wake particles
-//         bemstep  one advection step of the wake with the BEM solved at every stage
+//         bemstep  one advection step of the wake with the BEM solved at every stage and the
+//                  inner layer cleared after every move
 //
 // Input: int64 header [mode-specific counts], then float64 arrays (node coordinates as float32).
 #include <cstdint>
@@ -75,18 +76,18 @@ void run_rk(Reader& in, Writer& out, int64_t n, int order, int core, int prec) {
   env.set_precision(prec);
   Convection<S, double, Int> conv(env, order);
   BEM<S, Int> bem;
-  conv.advect(0.0, dt, {0.0, 0.0}, vort, bdry, fldpt, bem);
+  conv.advect(0.0, dt, {0.0, 0.0}, S(0.1), vort, bdry, fldpt, bem);
   const auto& p = std::get<Points<S>>(vort[0]);
   out.vec(p.get_pos()[0]);
   out.vec(p.get_pos()[1]);
 }
 
-// header: npan, nnodes, nwake, nunk, core, order; then fsx, fsy, dt, nodes x/y (float32),
-// idx (uint32 pairs), wake x, y, s, r
+// header: npan, nnodes, nwake, nunk, core, order; then fsx, fsy, dt, ips, nodes x/y
+// (float32), idx (uint32 pairs), wake x, y, s, r
 void run_bem(Reader& in, Writer& out, const std::string& mode) {
   const int64_t npan = in.i64(), nnodes = in.i64(), nw = in.i64(), nunk = in.i64(),
                 core = in.i64(), order = in.i64();
-  const double fsx = in.f64(), fsy = in.f64(), dt = in.f64();
+  const double fsx = in.f64(), fsy = in.f64(), dt = in.f64(), ips = in.f64();
   const auto nx = in.vec<float>(nnodes), ny = in.vec<float>(nnodes);
   const auto idx = in.vec<uint32_t>(2 * npan);
   auto wx = in.vec<double>(nw), wy = in.vec<double>(nw), ws = in.vec<double>(nw),
@@ -114,7 +115,7 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
     out.vec(p.get_vel()[0]);
     out.vec(p.get_vel()[1]);
   } else {
-    conv.advect(0.0, dt, fs, vort, bdry, fldpt, bem);
+    conv.advect(0.0, dt, fs, ips, vort, bdry, fldpt, bem);
     const auto& p = std::get<Points<double>>(vort[0]);
     out.vec(p.get_pos()[0]);
     out.vec(p.get_pos()[1]);

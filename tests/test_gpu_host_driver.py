@@ -135,7 +135,7 @@ def test_bem_solve_and_wake_velocity(tmp_path, nunk):
     w, nx, ny, idx = bem_case(nunk)
     nw = len(w["x"])
     header = [len(idx) // 2, len(nx), nw, nunk, CORE_ID["v2"], 2]
-    payload = [np.array([1.0, 0.0, 0.01]), nx, ny, idx, w["x"], w["y"], w["s"], w["r"]]
+    payload = [np.array([1.0, 0.0, 0.01, 0.1]), nx, ny, idx, w["x"], w["y"], w["s"], w["r"]]
     npan = len(idx) // 2
     nrow = npan * nunk + 1
     got = run_driver(tmp_path, "bem", header, payload, 2 * npan + 1 + nrow + 2 * nw)
@@ -160,18 +160,31 @@ def test_bem_solve_and_wake_velocity(tmp_path, nunk):
 
 
 def test_bem_rk2_step_with_wake(tmp_path):
+    """Ralston RK2 with the BEM solved at both stages and the inner layer cleared after each
+    move (method 1, src/Convection.h:413,464), as src/Convection.h:390-466 does it."""
     w, nx, ny, idx = bem_case(2, npan=96, nw=1000)
     nw = len(w["x"])
-    dt = 0.01
+    dt, ips = 0.01, 0.5  # a layer 0.1 thick: wake particles start from radius 0.52
     header = [len(idx) // 2, len(nx), nw, 2, CORE_ID["v2"], 2]
-    payload = [np.array([1.0, 0.0, dt]), nx, ny, idx, w["x"], w["y"], w["s"], w["r"]]
+    payload = [np.array([1.0, 0.0, dt, ips]), nx, ny, idx, w["x"], w["y"], w["s"], w["r"]]
     got = run_driver(tmp_path, "bemstep", header, payload, 2 * nw)
     ref = PyBem(nx, ny, idx, 2, "v2", (1.0, 0.0))
     x, y, s, r = w["x"], w["y"], w["s"], w["r"]
+    n64x, n64y = nx.astype(np.float64), ny.astype(np.float64)
+    _, _, pnx, pny, _ = O.panel_geometry(n64x, n64y, idx)
+    pushed = []
+
+    def clear(px, py):
+        c = O.clear_inner(1, n64x, n64y, idx, pnx, pny, 0.5 / np.sqrt(2.0 * np.pi), ips, px, py,
+                          s, r)
+        pushed.append(c["num_cropped"])
+        return c["x"], c["y"]
+
     ref.solve(x, y, s)
     u0 = ref.wake_vel(x, y, s, r)
-    x1, y1 = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * dt)
+    x1, y1 = clear(*O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * dt))
     ref.solve(x1, y1, s)
     u1 = ref.wake_vel(x1, y1, s, r)
-    ex, ey = O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt)
+    ex, ey = clear(*O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt))
+    assert min(pushed) > 0
     assert np.max(np.abs(got[:nw] - ex)) <= 1e-12 and np.max(np.abs(got[nw:] - ey)) <= 1e-12
