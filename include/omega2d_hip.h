@@ -94,7 +94,8 @@ int32_t o2d_hip_flops_per_pair(int32_t core, int32_t point_targets, int32_t pane
 #define O2D_KT_NBODY_SPLIT_PT 8 /* split fp32 kernel, point targets */
 #define O2D_KT_NBODY_PLAIN_PT 9 /* plain fp64 kernel, point targets */
 #define O2D_KT_NEAR 10          /* particle/panel proximity scan (distances, reflect, clear) */
-#define O2D_KT_COUNT 11
+#define O2D_KT_VORT 11          /* Gaussian vorticity of particles */
+#define O2D_KT_COUNT 12
 
 /* enable = 1: from now on record a HIP event pair on the launch stream around every timed
  * kernel launch (results are unaffected); enable = 0: stop recording. */
@@ -205,6 +206,19 @@ int o2d_hip_points_on_points_adjoint_dev_d(const o2d_env* env, int64_t ns, const
                                            const double* srad, int64_t nt, const double* tx,
                                            const double* ty, const double* gu, const double* gv,
                                            double* gx, double* gy, double* gs, double* gr);
+
+/* ---- Gaussian vorticity: points_affect_points_vorticity (src/InfluenceVort.h:47-126) ------ */
+
+/* tw_i += sum_j 2 sstr_j exp(-d_ij^2 / srad_j^2) / srad_j^2 at the nt targets (unscaled, as the
+ * reference's x86 loop; finalize_vels multiplies by 1/(2 pi)). Computed in fp64 whatever
+ * env->precision says; env->core is not used (the reference always uses the Gaussian here).
+ * Host arrays (_d) or device arrays (_dev_d). */
+int o2d_hip_points_vorticity_d(const o2d_env* env, int64_t ns, const double* sx,
+                               const double* sy, const double* sstr, const double* srad,
+                               int64_t nt, const double* tx, const double* ty, double* tw);
+int o2d_hip_points_vorticity_dev_d(const o2d_env* env, int64_t ns, const double* sx,
+                                   const double* sy, const double* sstr, const double* srad,
+                                   int64_t nt, const double* tx, const double* ty, double* tw);
 
 /* ---- particle/panel proximity: src/Reflect.h ---------------------------------------------- */
 /*

@@ -18,7 +18,8 @@ from ._cabi import (CORES, PRECISIONS, RESULTS, Env, O2DError, call, lib, make_e
 
 __all__ = ["Env", "O2DError", "make_env", "points_on_points", "panels_on_points",
            "points_on_panels", "bem_assemble", "bem_factor", "bem_solve", "rk_move",
-           "points_on_points_adjoint", "nearest_distances", "reflect", "clear_inner",
+           "points_on_points_adjoint", "points_vorticity", "nearest_distances", "reflect",
+           "clear_inner",
            "flops_per_pair", "release"]
 
 
@@ -158,6 +159,17 @@ def points_on_points_adjoint(sx, sy, ss, sr, tx, ty, gu, gv, *, core="gauss", de
     call("o2d_hip_points_on_points_adjoint_d", ctypes.byref(env), ns, _p(sx), _p(sy), _p(ss),
          _p(sr), len(tx), _p(tx), _p(ty), _p(gu), _p(gv), *[_p(o) for o in out])
     return tuple(out)
+
+
+def points_vorticity(sx, sy, ss, sr, tx, ty, *, device=0, tw=None):
+    """points_affect_points_vorticity (src/InfluenceVort.h:47-126): accumulates the unscaled
+    Gaussian vorticity sum into tw (fp64)."""
+    sx, sy, ss, sr, tx, ty = (_arr(a, np.float64) for a in (sx, sy, ss, sr, tx, ty))
+    tw = _out(tw, len(tx), np.float64)
+    env = make_env("gauss", "fp64", "vel", device)
+    call("o2d_hip_points_vorticity_d", ctypes.byref(env), len(sx), _p(sx), _p(sy), _p(ss),
+         _p(sr), len(tx), _p(tx), _p(ty), _p(tw))
+    return tw
 
 
 def nearest_distances(nx, ny, idx, tx, ty, *, device=0):

@@ -2,7 +2,8 @@
 
     python -m omega2d_amd.bench_configs [--configs c1,c3,c4,c5] [--out FILE]
 
-C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update
+C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update;
+    the Gaussian vorticity of the disc on itself
 C3  circle of 2048 panels + 200k wake, V2 core: BEM assembly (vortex-only + augmented 2049^2,
     and vortex+source + augmented 4097^2) with HBM GB/s, LU factor + solve, RHS (points on
     panels), panels on particles, particles on particles (split and plain), and the
@@ -26,7 +27,8 @@ import torch
 from . import device as dv
 from . import flops_per_pair, workloads as wl
 from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_PLAIN_PT,
-                    KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS, timing_enable, timing_read)
+                    KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS, KT_VORT, timing_enable,
+                    timing_read)
 
 PEAK_FP32, PEAK_FP64, PEAK_HBM = 157.3e12, 78.6e12, 8.0e12
 
@@ -91,6 +93,12 @@ def c1():
         res[prec + "_x"] = xs
     res["split_pos_err_vs_fp64"] = rel(res.pop("split_x"), res["fp64_x"])
     res.pop("fp64_x")
+    # Gaussian vorticity of the disc on itself (src/InfluenceVort.h), 13 flops per pair in the
+    # reference's count
+    ms, _ = kernel_ms(KT_VORT, lambda: dv.points_vorticity(x, y, s, r, x, y), reps=5)
+    res["vorticity"] = {"kernel_ms": round(ms, 4),
+                        "Gint_s": round(float(n) * n / (ms * 1e-3) / 1e9, 2),
+                        "frac_fp64": round(13.0 * n * n / (ms * 1e-3) / PEAK_FP64, 4)}
     return res
 
 

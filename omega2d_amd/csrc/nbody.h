@@ -44,6 +44,11 @@ int adjoint_dev(Device* d, hipStream_t s, int core, int64_t ns, const double* sx
                 const double* tx, const double* ty, const double* gu, const double* gv,
                 double* gx, double* gy, double* gs, double* gr);
 
+// Gaussian vorticity of particles at targets (fp64, +=), device pointers
+int vorticity_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const double* sy,
+                  const double* ss, const double* sr, int64_t nt, const double* tx,
+                  const double* ty, double* tw);
+
 // particle/panel proximity (mode 0 nearest distances, 1 reflect, 2 clear), device pointers
 int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult, double ips,
              int64_t nnodes, const double* nx, const double* ny, int64_t np,

@@ -12,7 +12,7 @@ from ._cabi import VELANDSHEAR, VELANDVORT, VELONLY, call, make_env
 
 __all__ = ["points_on_points", "panels_on_points", "points_on_panels", "bem_assemble",
            "bem_factor", "bem_solve", "rk_move", "points_on_points_adjoint", "morton_order",
-           "nearest_distances", "reflect", "clear_inner", "env_for"]
+           "points_vorticity", "nearest_distances", "reflect", "clear_inner", "env_for"]
 
 
 def _ptr(t):
@@ -145,6 +145,16 @@ def morton_order(x, y, x0, y0, inv_extent):
     call("o2d_hip_morton_order_dev_d", ctypes.byref(env), n, _ptr(x), _ptr(y), float(x0),
          float(y0), float(inv_extent), _ptr(keys), _ptr(order))
     return keys, order
+
+
+def points_vorticity(sx, sy, ss, sr, tx, ty, tw=None):
+    """Accumulates the unscaled Gaussian vorticity sum into tw (zeros when not given)."""
+    dev = _check(sx, sy, ss, sr, tx, ty, tw)
+    tw = torch.zeros(tx.numel(), dtype=torch.float64, device=dev) if tw is None else tw
+    env = env_for(dev, core="gauss")
+    call("o2d_hip_points_vorticity_dev_d", ctypes.byref(env), sx.numel(), _ptr(sx), _ptr(sy),
+         _ptr(ss), _ptr(sr), tx.numel(), _ptr(tx), _ptr(ty), _ptr(tw))
+    return tw
 
 
 def nearest_distances(nx, ny, idx, tx, ty, dist=None):

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "BEM.h"
+#include "InfluenceVort.h"
 #include "Reflect.h"
 
 namespace omega2d {
@@ -64,6 +65,24 @@ class Convection {
     }
     for (auto& b : bdry)
       if (auto* s = std::get_if<Surfaces<S>>(&b)) s->add_solved_rot_strengths(S(-1.0));
+  }
+
+  // src/Convection.h:134-166: vorticity (and zeroed velocities) on every Points target from
+  // the Points among the particles and the boundaries, finalized without a freestream
+  void find_vort(std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
+                 std::vector<Collection<S>>& targets) {
+    for (auto& t : targets) {
+      auto* pt = std::get_if<Points<S>>(&t);
+      if (!pt) continue;
+      pt->zero_vels();
+      auto& w = pt->get_vort();
+      std::fill(w.begin(), w.end(), S(0));
+      for (auto* list : {&vort, &bdry})
+        for (auto& s : *list)
+          if (auto* ps = std::get_if<Points<S>>(&s))
+            points_affect_points_vorticity<S, A>(*ps, *pt, env);
+      pt->finalize_vels({0.0, 0.0});
+    }
   }
 
   // src/Convection.h:229-243

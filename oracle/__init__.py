@@ -67,6 +67,7 @@ def lib():
         L.oc_points_on_points_adjoint.argtypes = [_int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                                   _vp, _vp, _vp, _vp, _vp, _vp]
         L.oc_morton_keys.argtypes = [_i64, _vp, _vp, _dbl, _dbl, _dbl, _vp]
+        L.oc_points_vorticity.argtypes = [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]
         L.oc_nearest_distances.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp]
         L.oc_reflect.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp]
         L.oc_cut_table.argtypes = [_dbl, _int, _vp, _vp, _vp]
@@ -103,6 +104,8 @@ def ref():
         R.ref_points_move3_d.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl,
                                          _dbl, _dbl, _dbl]
         R.ref_finalize_d.argtypes = [_i64, _vp, _vp, _dbl, _dbl]
+        R.ref_points_vorticity_d.argtypes = [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]
+        R.ref_points_vorticity_f.argtypes = [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]
         R.ref_nearest_distances_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp]
         R.ref_panel_normals_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _vp, _vp]
         R.ref_reflect_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp]
@@ -265,6 +268,16 @@ def morton_keys(x, y, x0, y0, inv_extent):
     keys = np.zeros(len(x), dtype=np.uint32)
     lib().oc_morton_keys(len(x), _p(x), _p(y), float(x0), float(y0), float(inv_extent), _p(keys))
     return keys
+
+
+def points_vorticity(sx, sy, ss, sr, tx, ty):
+    """Unscaled Gaussian vorticity sum at the targets (src/InfluenceVort.h:47-126)."""
+    sx, sy, ss, sr, tx, ty = map(_f64, (sx, sy, ss, sr, tx, ty))
+    tw = np.zeros(len(tx))
+    rc = lib().oc_points_vorticity(len(sx), _p(sx), _p(sy), _p(ss), _p(sr), len(tx), _p(tx),
+                                   _p(ty), _p(tw))
+    assert rc == 0, rc
+    return tw
 
 
 def nearest_distances(nx, ny, idx, tx, ty):
@@ -486,3 +499,15 @@ def ref_cut_table():
     rows = ref().ref_cut_table_d(256, _p(pos), _p(sm), _p(ds))
     assert rows > 0, rows
     return pos[:rows], sm[:rows], ds[:rows]
+
+
+def ref_points_vorticity(sx, sy, ss, sr, tx, ty, dtype=np.float64):
+    c = lambda a: np.ascontiguousarray(a, dtype=dtype)  # noqa: E731
+    sx, sy, ss, sr, tx, ty = map(c, (sx, sy, ss, sr, tx, ty))
+    tw = np.zeros(len(tx), dtype)
+    fn = ref().ref_points_vorticity_d if dtype == np.float64 else ref().ref_points_vorticity_f
+    with quiet():
+        rc = fn(len(sx), _p(sx), _p(sy), _p(ss), _p(sr), len(tx), _p(tx), _p(ty), _p(tw))
+    assert rc == 0, rc
+    return tw
+

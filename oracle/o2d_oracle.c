@@ -38,6 +38,8 @@
  *                          across a flat wall; reflected points lie outside a convex body.
  *   oc_cut_table           src/Reflect.h:333-394       pinned: golden table; the strength
  *                          fraction is the 2D Gaussian's cumulative weight (-> 1 at +3 radii).
+ *   oc_points_vorticity    src/InfluenceVort.h:47-126  pinned: golden vectors; closed-form
+ *                          single blob; the vorticity integrates to the circulation.
  *   oc_clear_inner         src/Reflect.h:487-727       pinned: golden vectors (method 1 push
  *                          and method 0 trim); pushed points sit at the cutoff layer of a flat
  *                          wall; the removed circulation equals the strength lost.
@@ -913,5 +915,24 @@ int oc_clear_inner(int method, int64_t nnodes, const double* nx, const double* n
   if (circ_removed) *circ_removed = circ;
   if (num_cropped) *num_cropped = ncrop;
   if (unsettled) *unsettled = nuns;
+  return 0;
+}
+
+/* points_affect_points_vorticity (src/InfluenceVort.h:47-126, the x86 loop :103-117): the
+ * unscaled Gaussian vorticity sum tw_i += sum_j 2 s_j exp(-d^2/r_j^2) / r_j^2 at every target;
+ * finalize_vels multiplies it by 1/(2 pi) (src/ElementBase.h:221-226). */
+int oc_points_vorticity(int64_t ns, const double* sx, const double* sy, const double* ss,
+                        const double* sr, int64_t nt, const double* tx, const double* ty,
+                        double* tw) {
+  for (int64_t i = 0; i < nt; ++i) {
+    double acc = 0.0;
+    for (int64_t j = 0; j < ns; ++j) {
+      const double dx = tx[i] - sx[j], dy = ty[i] - sy[j];
+      const double r2 = sr[j] * sr[j];
+      const double distsq = (dx * dx + dy * dy) / r2;
+      acc += 2.0 * ss[j] * exp(-distsq) / r2;
+    }
+    tw[i] += acc;
+  }
   return 0;
 }

@@ -81,6 +81,11 @@ int oc_clear_inner(int method, int64_t nnodes, const double* nx, const double* n
                    int max_passes, double* circ_removed, int64_t* num_cropped,
                    int64_t* unsettled);
 
+/* src/InfluenceVort.h:47-126: unscaled Gaussian vorticity of particles at targets (+=) */
+int oc_points_vorticity(int64_t ns, const double* sx, const double* sy, const double* ss,
+                        const double* sr, int64_t nt, const double* tx, const double* ty,
+                        double* tw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY. Compiled by oracle/Makefile against the reference headers where
 // they lie (src/Influence.h, src/Coefficients.h, src/RHS.h, src/Reflect.h, src/Points.h,
-// src/Surfaces.h),
+// src/Surfaces.h, src/InfluenceVort.h),
 // never copied. The shared library lands in oracle/_ref/ and is used only to write the golden
 // fixtures under tests/golden/, to validate the C restatement in oracle/o2d_oracle.c, and as
 // the "reference" CPU baseline timed beside the GPU numbers. Nothing in the product links it.
@@ -16,6 +16,7 @@
 #include "Coefficients.h"
 #include "RHS.h"
 #include "Reflect.h"
+#include "InfluenceVort.h"
 
 #include <cstdint>
 #include <cstring>
@@ -353,6 +354,36 @@ int ref_cut_table_d(int cap, double* pos, double* smult, double* dshift) {
     dshift[i] = std::get<2>(ct[i]);
   }
   return (int)ct.size();
+}
+
+// points_affect_points_vorticity<S,A> (src/InfluenceVort.h:47-126), x86 environment; targets
+// are particles when tr is given, field points otherwise (only their positions are read)
+int ref_points_vorticity_d(int64_t ns, const double* sx, const double* sy, const double* ss,
+                           const double* sr, int64_t nt, const double* tx, const double* ty,
+                           double* tw) {
+  Points<double> src = make_points<double>(ns, sx, sy, ss, sr, false);
+  std::vector<double> z(nt, 0.0), r(nt, 0.01);
+  Points<double> targ = make_points<double>(nt, tx, ty, z.data(), r.data(), false);
+  targ.zero_vels();
+  Vector<double>& w = targ.get_vort();
+  std::fill(w.begin(), w.end(), 0.0);
+  points_affect_points_vorticity<double, double>(src, targ, ExecEnv(true, direct, cpu_x86));
+  copy_out<double>(targ.get_vort(), tw);
+  return 0;
+}
+
+int ref_points_vorticity_f(int64_t ns, const float* sx, const float* sy, const float* ss,
+                           const float* sr, int64_t nt, const float* tx, const float* ty,
+                           float* tw) {
+  Points<float> src = make_points<float>(ns, sx, sy, ss, sr, false);
+  std::vector<float> z(nt, 0.0f), r(nt, 0.01f);
+  Points<float> targ = make_points<float>(nt, tx, ty, z.data(), r.data(), false);
+  targ.zero_vels();
+  Vector<float>& w = targ.get_vort();
+  std::fill(w.begin(), w.end(), 0.0f);
+  points_affect_points_vorticity<float, double>(src, targ, ExecEnv(true, direct, cpu_x86));
+  copy_out<float>(targ.get_vort(), tw);
+  return 0;
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 //         bem      BEM solve on a reactive circle + velocities of the wake particles
 //         bemstep  one advection step of the wake with the BEM solved at every stage and the
 //                  inner layer cleared after every move
+//         vort     Convection::find_vort of particles on themselves
 //
 // Input: int64 header [mode-specific counts], then float64 arrays (node coordinates as float32).
 #include <cstdint>
@@ -122,11 +123,23 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
   }
 }
 
+// header: n; then x, y, s, r
+void run_vort(Reader& in, Writer& out, int64_t n) {
+  const auto x = in.vec<double>(n), y = in.vec<double>(n), s = in.vec<double>(n),
+             r = in.vec<double>(n);
+  std::vector<Collection<double>> vort, bdry, fldpt;
+  vort.emplace_back(Points<double>(n, x.data(), y.data(), s.data(), r.data()));
+  ExecEnv env(true, direct, gpu_hip);
+  Convection<double, double, Int> conv(env, 2);
+  conv.find_vort(vort, bdry, vort);
+  out.vec(std::get<Points<double>>(vort[0]).get_vort());
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|bem|bemstep> <in.bin> <out.bin>\n";
+    std::cerr << "usage: o2d_host_driver <rk|bem|bemstep|vort> <in.bin> <out.bin>\n";
     return 2;
   }
   try {
@@ -140,6 +153,8 @@ int main(int argc, char** argv) {
       else run_rk<double>(in, out, n, (int)order, (int)core, (int)prec);
     } else if (mode == "bem" || mode == "bemstep") {
       run_bem(in, out, mode);
+    } else if (mode == "vort") {
+      run_vort(in, out, in.i64());
     } else {
       std::cerr << "unknown mode " << mode << "\n";
       return 2;

@@ -2,7 +2,7 @@
 
 Runs in a container that has the reference tree: it calls oracle/_ref/libo2d_ref.so (built by
 ``make -C oracle ref`` from src/Influence.h, src/Coefficients.h, src/RHS.h, src/Points.h,
-src/Surfaces.h, src/ElementBase.h, src/Reflect.h where they lie) on small seeded inputs and stores inputs and
+src/Surfaces.h, src/ElementBase.h, src/Reflect.h, src/InfluenceVort.h where they lie) on small seeded inputs and stores inputs and
 outputs. The fixture is data only: inputs and the reference's outputs.
 
     python tests/golden/make_golden.py
@@ -143,6 +143,16 @@ def main():
     d["rt_refl_x"], d["rt_refl_y"] = O.ref_reflect(tnx, tny, tidx, gx, gy)
     R = O.ref_clear_inner(1, tnx, tny, tidx, cm, ips, gx, gy)
     d["rt_c1_x"], d["rt_c1_y"] = R["x"], R["y"]
+
+    # Gaussian vorticity of particles (src/InfluenceVort.h), its own seed
+    r3 = np.random.default_rng(20241203)
+    vn, vt = 400, 350
+    vsx, vsy = r3.uniform(-1, 1, vn), r3.uniform(-1, 1, vn)
+    vss, vsr = r3.normal(0, 1, vn), r3.uniform(0.05, 0.2, vn)
+    vtx, vty = r3.uniform(-1.2, 1.2, vt), r3.uniform(-1.2, 1.2, vt)
+    d.update(vo_sx=vsx, vo_sy=vsy, vo_ss=vss, vo_sr=vsr, vo_tx=vtx, vo_ty=vty)
+    d["vo_w"] = O.ref_points_vorticity(vsx, vsy, vss, vsr, vtx, vty)
+    d["vo_wf"] = O.ref_points_vorticity(vsx, vsy, vss, vsr, vtx, vty, dtype=np.float32)
 
     np.savez_compressed(OUT, **d)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(d)} arrays)")

@@ -188,3 +188,16 @@ def test_bem_rk2_step_with_wake(tmp_path):
     ex, ey = clear(*O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt))
     assert min(pushed) > 0
     assert np.max(np.abs(got[:nw] - ex)) <= 1e-12 and np.max(np.abs(got[nw:] - ey)) <= 1e-12
+
+
+def test_find_vort_self(tmp_path):
+    """Convection::find_vort (src/Convection.h:134-166): Gaussian vorticity of the particles on
+    themselves, finalized (1/2pi), against the oracle."""
+    rng = np.random.default_rng(8)
+    n = 3000
+    x, y = rng.normal(0, 0.5, n), rng.normal(0, 0.5, n)
+    s, r = rng.normal(0, 1e-3, n), rng.uniform(0.02, 0.05, n)
+    got = run_driver(tmp_path, "vort", [n], [x, y, s, r], n)
+    ref = O.points_vorticity(x, y, s, r, x, y) * (0.5 / np.pi)
+    assert rel_l2(got, ref) <= 1e-12
+
