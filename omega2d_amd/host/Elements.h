@@ -10,9 +10,12 @@
 #include <cmath>
 #include <cstdint>
 #include <memory>
+#include <numeric>
 #include <optional>
 #include <string>
 #include <vector>
+
+#include "VtkXmlWriter.h"
 
 namespace omega2d {
 
@@ -77,6 +80,48 @@ class Points {
   }
   bool has_vort() const { return (bool)w; }
   bool has_shear() const { return (bool)e; }
+
+  // src/Points.h:556-740: one .vtu file (part_NN_FFFFF.vtu, fldpt_ for inert points) in `dir`
+  // (the working directory when empty); returns the file name
+  std::string write_vtk(size_t index, size_t frameno, double time,
+                        const std::string& dir = "") const {
+    const bool particles = E != inert;
+    const std::string name = vtu_name(particles ? "part_" : "fldpt_", index, frameno);
+    VtuFile f(dir.empty() ? name : dir + "/" + name);
+    vtu_time(f, time);
+    f.open("Piece", {{"NumberOfPoints", std::to_string(n)}, {"NumberOfCells", std::to_string(n)}});
+    f.open("Points");
+    f.data_array({{"NumberOfComponents", "3"}, {"Name", "position"}, {"type", "Float32"}},
+                 xyz_float(xs[0], xs[1]));
+    f.close();
+    f.open("Cells");
+    std::vector<int32_t> conn(n), off(n);
+    std::iota(conn.begin(), conn.end(), 0);
+    std::iota(off.begin(), off.end(), 1);
+    f.data_array({{"Name", "connectivity"}, {"type", "Int32"}}, conn);
+    f.data_array({{"Name", "offsets"}, {"type", "Int32"}}, off);
+    f.data_array({{"Name", "types"}, {"type", "UInt8"}}, std::vector<uint8_t>(n, 1));
+    f.close();
+    std::string scalars;
+    if (particles) scalars += "circulation,radius,";
+    if (w) scalars += "vorticity,";
+    VtkAttrs pd = {{"Vectors", "velocity"}};
+    if (!scalars.empty()) {
+      scalars.pop_back();
+      pd.emplace_back("Scalars", scalars);
+    }
+    f.open("PointData", pd);
+    if (particles) {
+      f.data_array({{"Name", "circulation"}, {"type", "Float32"}}, as_float(s));
+      f.data_array({{"Name", "radius"}, {"type", "Float32"}}, as_float(r));
+    }
+    if (w) f.data_array({{"Name", "vorticity"}, {"type", "Float32"}}, as_float(*w));
+    if (e) f.data_array({{"Name", "shearrate"}, {"type", "Float32"}}, as_float(*e));
+    f.data_array({{"NumberOfComponents", "3"}, {"Name", "velocity"}, {"type", "Float32"}},
+                 xyz_float(u[0], u[1]));
+    f.finish();
+    return name;
+  }
 
   // src/ElementBase.h:200-212
   void zero_vels() {
@@ -182,6 +227,35 @@ class Surfaces {
     }
     if (E != reactive) aug = false;
     return aug;
+  }
+
+  // src/Surfaces.h:1238-1379: one .vtu file (panel_NN_FFFFF.vtu) of the nodes, the panels as
+  // line cells and, unless inert, the vortex and source sheet strengths per panel
+  std::string write_vtk(size_t index, size_t frameno, double time,
+                        const std::string& dir = "") const {
+    const std::string name = vtu_name("panel_", index, frameno);
+    VtuFile f(dir.empty() ? name : dir + "/" + name);
+    vtu_time(f, time);
+    f.open("Piece", {{"NumberOfPoints", std::to_string(n)}, {"NumberOfCells", std::to_string(np)}});
+    f.open("Points");
+    f.data_array({{"NumberOfComponents", "3"}, {"Name", "position"}, {"type", "Float32"}},
+                 xyz_float(x[0], x[1]));
+    f.close();
+    f.open("Cells");
+    std::vector<int32_t> conn(idx.begin(), idx.end()), off(np);
+    for (size_t i = 0; i < np; ++i) off[i] = (int32_t)(2 * (i + 1));
+    f.data_array({{"Name", "connectivity"}, {"type", "Int32"}}, conn);
+    f.data_array({{"Name", "offsets"}, {"type", "Int32"}}, off);
+    f.data_array({{"Name", "types"}, {"type", "UInt8"}}, std::vector<uint8_t>(np, 3));
+    f.close();
+    if (E != inert) {
+      f.open("CellData", {{"Scalars", "vortex sheet strength,source sheet strength"}});
+      f.data_array({{"Name", "vortex sheet strength"}, {"type", "Float32"}}, as_float(*ps[0]));
+      f.data_array({{"Name", "source sheet strength"}, {"type", "Float32"}}, as_float(*ps[1]));
+      f.close();
+    }
+    f.finish();
+    return name;
   }
 
   // src/Surfaces.h:266-301: pop the solved rotation rate, de-interleave [vortex, source]

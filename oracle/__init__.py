@@ -112,6 +112,10 @@ def ref():
         R.ref_clear_inner_d.argtypes = [_int, _i64, _vp, _vp, _i64, _vp, _dbl, _dbl, _i64, _vp,
                                         _vp, _vp, _vp, _vp]
         R.ref_cut_table_d.argtypes = [_int, _vp, _vp, _vp]
+        R.ref_write_points_vtk_f.argtypes = [ctypes.c_char_p, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                             _vp, _int, _i64, _i64, _dbl, ctypes.c_char_p, _i64]
+        R.ref_write_panels_vtk_f.argtypes = [ctypes.c_char_p, _i64, _vp, _vp, _i64, _vp, _vp, _vp,
+                                             _i64, _i64, _dbl, ctypes.c_char_p, _i64]
         _ref = R
     return _ref
 
@@ -510,4 +514,32 @@ def ref_points_vorticity(sx, sy, ss, sr, tx, ty, dtype=np.float64):
         rc = fn(len(sx), _p(sx), _p(sy), _p(ss), _p(sr), len(tx), _p(tx), _p(ty), _p(tw))
     assert rc == 0, rc
     return tw
+
+
+def ref_write_points_vtk(directory, x, y, s, r, u, v, w=None, inert=False, index=0, frameno=0,
+                         time=0.0):
+    """Reference Points<float>::write_vtk inside `directory`; returns the file name."""
+    c = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)  # noqa: E731
+    x, y, s, r, u, v, w = map(c, (x, y, s, r, u, v, w))
+    name = ctypes.create_string_buffer(256)
+    with quiet():
+        rc = ref().ref_write_points_vtk_f(str(directory).encode(), len(x), _p(x), _p(y), _p(s),
+                                          _p(r), _p(u), _p(v), _p(w), int(bool(inert)), index,
+                                          frameno, float(time), name, 256)
+    assert rc == 0, rc
+    return name.value.decode()
+
+
+def ref_write_panels_vtk(directory, nx, ny, idx, vs, ss, index=0, frameno=0, time=0.0):
+    """Reference Surfaces<float>::write_vtk inside `directory`; returns the file name."""
+    nx, ny = _f32nodes(nx, ny)
+    idx = _u32(idx)
+    vs, ss = np.ascontiguousarray(vs, np.float32), np.ascontiguousarray(ss, np.float32)
+    name = ctypes.create_string_buffer(256)
+    with quiet():
+        rc = ref().ref_write_panels_vtk_f(str(directory).encode(), len(nx), _p(nx), _p(ny),
+                                          len(idx) // 2, _p(idx), _p(vs), _p(ss), index, frameno,
+                                          float(time), name, 256)
+    assert rc == 0, rc
+    return name.value.decode()
 

@@ -18,6 +18,8 @@
 #include "Reflect.h"
 #include "InfluenceVort.h"
 
+#include <unistd.h>
+
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -386,4 +388,47 @@ int ref_points_vorticity_f(int64_t ns, const float* sx, const float* sy, const f
   return 0;
 }
 
+// Points<float>::write_vtk (src/Points.h:556-740) run inside directory `dir`; particles (with
+// strengths and radii) unless inert != 0; w (vorticity) may be NULL. The file name is copied to
+// `name` (capacity cap). Float storage, as the application (STORE=float) writes its files.
+int ref_write_points_vtk_f(const char* dir, int64_t n, const float* x, const float* y,
+                           const float* s, const float* r, const float* u, const float* v,
+                           const float* w, int inert, int64_t index, int64_t frameno,
+                           double time, char* name, int64_t cap) {
+  Points<float> p = make_points<float>(n, x, y, inert ? nullptr : s, inert ? nullptr : r,
+                                       inert != 0);
+  p.zero_vels();
+  for (int64_t i = 0; i < n; ++i) { p.get_vel()[0][i] = u[i]; p.get_vel()[1][i] = v[i]; }
+  if (w) {
+    Vector<float>& pw = p.get_vort();
+    for (int64_t i = 0; i < n; ++i) pw[i] = w[i];
+  }
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof(cwd)) || chdir(dir) != 0) return -1;
+  const std::string fn = p.write_vtk((size_t)index, (size_t)frameno, time);
+  if (chdir(cwd) != 0) return -1;
+  std::snprintf(name, (size_t)cap, "%s", fn.c_str());
+  return 0;
+}
+
+// Surfaces<float>::write_vtk (src/Surfaces.h:1238-1379) of reactive panels with vortex and
+// source sheet strengths, inside directory `dir`
+int ref_write_panels_vtk_f(const char* dir, int64_t nnodes, const float* nx, const float* ny,
+                           int64_t np, const uint32_t* idx, const float* vs, const float* ss,
+                           int64_t index, int64_t frameno, double time, char* name, int64_t cap) {
+  HarnessSurfaces<float> surf = make_surfaces<float>(nnodes, nx, ny, np, idx, reactive, fixed,
+                                                     nullptr);
+  for (int64_t j = 0; j < np; ++j) {
+    surf.get_str()[j] = vs[j];
+    surf.get_src_str()[j] = ss[j];
+  }
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof(cwd)) || chdir(dir) != 0) return -1;
+  const std::string fn = surf.write_vtk((size_t)index, (size_t)frameno, time);
+  if (chdir(cwd) != 0) return -1;
+  std::snprintf(name, (size_t)cap, "%s", fn.c_str());
+  return 0;
+}
+
 }  // extern "C"
+

@@ -9,6 +9,8 @@
 //         bemstep  one advection step of the wake with the BEM solved at every stage and the
 //                  inner layer cleared after every move
 //         vort     Convection::find_vort of particles on themselves
+//         vtk      write particle, field-point and panel .vtu files into the directory given
+//                  in place of <out.bin> (no GPU work)
 //
 // Input: int64 header [mode-specific counts], then float64 arrays (node coordinates as float32).
 #include <cstdint>
@@ -135,16 +137,47 @@ void run_vort(Reader& in, Writer& out, int64_t n) {
   out.vec(std::get<Points<double>>(vort[0]).get_vort());
 }
 
+// header: n, nnodes, npan, index, frameno, pindex, pframeno; then time, ptime, x, y, s, r, u,
+// v, w (particles), nodes x/y (float32), idx (uint32 pairs), vortex and source strengths
+void run_vtk(Reader& in, const std::string& dir) {
+  const int64_t n = in.i64(), nnodes = in.i64(), npan = in.i64(), index = in.i64(),
+                frameno = in.i64(), pindex = in.i64(), pframeno = in.i64();
+  const double time = in.f64(), ptime = in.f64();
+  const auto x = in.vec<double>(n), y = in.vec<double>(n), s = in.vec<double>(n),
+             r = in.vec<double>(n), u = in.vec<double>(n), v = in.vec<double>(n),
+             w = in.vec<double>(n);
+  const auto nx = in.vec<float>(nnodes), ny = in.vec<float>(nnodes);
+  const auto idx = in.vec<uint32_t>(2 * npan);
+  const auto vs = in.vec<double>(npan), ss = in.vec<double>(npan);
+  Points<double> part(n, x.data(), y.data(), s.data(), r.data());
+  Points<double> fld(n, x.data(), y.data(), nullptr, nullptr);
+  for (auto* p : {&part, &fld}) {
+    p->get_vel()[0] = u;
+    p->get_vel()[1] = v;
+  }
+  part.get_vort() = w;
+  part.write_vtk((size_t)index, (size_t)frameno, time, dir);
+  fld.write_vtk((size_t)index + 1, (size_t)frameno, time, dir);
+  Surfaces<double> surf(nnodes, nx.data(), ny.data(), npan, idx.data(), reactive, fixed, nullptr);
+  surf.get_src_str() = ss;
+  surf.get_str() = vs;
+  surf.write_vtk((size_t)pindex, (size_t)pframeno, ptime, dir);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|bem|bemstep|vort> <in.bin> <out.bin>\n";
+    std::cerr << "usage: o2d_host_driver <rk|bem|bemstep|vort|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
     const std::string mode = argv[1];
     Reader in(argv[2]);
+    if (mode == "vtk") {
+      run_vtk(in, argv[3]);
+      return 0;
+    }
     Writer out(argv[3]);
     if (mode == "rk") {
       const int64_t n = in.i64(), order = in.i64(), core = in.i64(), storage = in.i64(),
@@ -155,6 +188,7 @@ int main(int argc, char** argv) {
       run_bem(in, out, mode);
     } else if (mode == "vort") {
       run_vort(in, out, in.i64());
+
     } else {
       std::cerr << "unknown mode " << mode << "\n";
       return 2;

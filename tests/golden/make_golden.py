@@ -2,7 +2,8 @@
 
 Runs in a container that has the reference tree: it calls oracle/_ref/libo2d_ref.so (built by
 ``make -C oracle ref`` from src/Influence.h, src/Coefficients.h, src/RHS.h, src/Points.h,
-src/Surfaces.h, src/ElementBase.h, src/Reflect.h, src/InfluenceVort.h where they lie) on small seeded inputs and stores inputs and
+src/Surfaces.h, src/ElementBase.h, src/Reflect.h, src/InfluenceVort.h where they lie) and, under
+tests/golden/vtk/, the .vtu files its own writers produce (src/VtkXmlWriter.h) on small seeded inputs and stores inputs and
 outputs. The fixture is data only: inputs and the reference's outputs.
 
     python tests/golden/make_golden.py
@@ -11,7 +12,9 @@ Core function: the reference compiles in Vatistas n=2 (src/CoreFunc.h:38), so ev
 particle-on-point vector uses that core.
 """
 import os
+import shutil
 import sys
+import tempfile
 
 import numpy as np
 
@@ -153,6 +156,29 @@ def main():
     d.update(vo_sx=vsx, vo_sy=vsy, vo_ss=vss, vo_sr=vsr, vo_tx=vtx, vo_ty=vty)
     d["vo_w"] = O.ref_points_vorticity(vsx, vsy, vss, vsr, vtx, vty)
     d["vo_wf"] = O.ref_points_vorticity(vsx, vsy, vss, vsr, vtx, vty, dtype=np.float32)
+
+    # VTU files written by the reference's own writers (src/Points.h:556-740,
+    # src/Surfaces.h:1238-1379, src/VtkXmlWriter.h), kept byte for byte under tests/golden/vtk/
+    r4 = np.random.default_rng(20241204)
+    kn = 37
+    kx, ky = r4.normal(0, 1, kn), r4.normal(0, 1, kn)
+    ks, kr = r4.normal(0, 1e-2, kn), r4.uniform(0.01, 0.02, kn)
+    ku, kv, kw = r4.normal(0, 1, kn), r4.normal(0, 1, kn), r4.normal(0, 10, kn)
+    d.update(vk_x=kx, vk_y=ky, vk_s=ks, vk_r=kr, vk_u=ku, vk_v=kv, vk_w=kw)
+    pnx, pny, pidx = circle_panels(12)
+    pvs, pss = r4.normal(0, 1, 12), r4.normal(0, 1, 12)
+    d.update(vk_pvs=pvs, vk_pss=pss)
+    vdir = os.path.join(HERE, "vtk")
+    os.makedirs(vdir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        names = [O.ref_write_points_vtk(tmp, kx, ky, ks, kr, ku, kv, kw, index=0, frameno=12,
+                                        time=0.125),
+                 O.ref_write_points_vtk(tmp, kx, ky, None, None, ku, kv, None, inert=True,
+                                        index=1, frameno=12, time=0.125),
+                 O.ref_write_panels_vtk(tmp, pnx, pny, pidx, pvs, pss, index=0, frameno=3,
+                                        time=2.5)]
+        for nm in names:
+            shutil.copyfile(os.path.join(tmp, nm), os.path.join(vdir, nm))
 
     np.savez_compressed(OUT, **d)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(d)} arrays)")
