@@ -201,6 +201,8 @@ int bem_assemble_dev(Device* d, hipStream_t s, int64_t nnodes, const double* nx,
 
 int bem_factor_dev(Device* d, hipStream_t s, int64_t n, const double* A, int64_t lda) {
   d->lu_ready = false;
+  d->inv_ready = false;
+  d->lu_solves = 0;
   void *lu, *piv, *info;
   O2D_TRY(slot(d, kLU, sizeof(double) * n * n, s, &lu));
   O2D_TRY(slot(d, kPivots, sizeof(rocblas_int) * n, s, &piv));
@@ -222,12 +224,44 @@ int bem_factor_dev(Device* d, hipStream_t s, int64_t n, const double* A, int64_t
   return 0;
 }
 
+// Solves A x = b with the kept factors. A matrix that is solved against again and again (a
+// body that does not move keeps its matrix for the whole run) is worth inverting: the second
+// solve with the same factors builds A^-1 from them (rocsolver getri, about the cost of one more
+// factorisation) and every solve from then on is one dgemv, an HBM-bound read of n^2 doubles
+// instead of two latency-bound triangular solves.
 int bem_solve_dev(Device* d, hipStream_t s, int64_t n, const double* b, double* x) {
   if (!d->lu_ready || d->lu_n != n)
     return fail(O2D_ERR_STATE, "no LU factors of size %lld (factor first)", (long long)n);
-  if (x != b) O2D_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   rocblas_handle h;
   O2D_TRY(blas_of(d, s, &h));
+  if (!d->inv_ready && d->lu_solves >= 1) {
+    void* inv;
+    O2D_TRY(slot(d, kInv, sizeof(double) * n * n, s, &inv));
+    O2D_HIP(hipMemcpyAsync(inv, d->buf[kLU], sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    if (rocsolver_dgetri(h, (rocblas_int)n, static_cast<double*>(inv), (rocblas_int)n,
+                         static_cast<rocblas_int*>(d->buf[kPivots]),
+                         static_cast<rocblas_int*>(d->buf[kInfo])) != rocblas_status_success)
+      return fail(O2D_ERR_HIP, "rocsolver_dgetri failed");
+    d->inv_ready = true;
+  }
+  ++d->lu_solves;
+  if (d->inv_ready) {
+    const double* src = b;
+    if (x == b) {  // dgemv needs distinct input and output vectors
+      void* t;
+      O2D_TRY(slot(d, kSolveTmp, sizeof(double) * n, s, &t));
+      O2D_HIP(hipMemcpyAsync(t, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      src = static_cast<const double*>(t);
+    }
+    const double one = 1.0, zero = 0.0;
+    if (rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success ||
+        rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n, &one,
+                      static_cast<const double*>(d->buf[kInv]), (rocblas_int)n, src, 1, &zero, x,
+                      1) != rocblas_status_success)
+      return fail(O2D_ERR_HIP, "rocblas_dgemv failed");
+    return 0;
+  }
+  if (x != b) O2D_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   if (rocsolver_dgetrs(h, rocblas_operation_none, (rocblas_int)n, 1, static_cast<double*>(d->buf[kLU]),
                        (rocblas_int)n, static_cast<const rocblas_int*>(d->buf[kPivots]), x,
                        (rocblas_int)n) != rocblas_status_success)

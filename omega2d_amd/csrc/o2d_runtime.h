@@ -29,6 +29,7 @@ enum Slot : int {
   kRkPtrs, kRkStage,
   kGradX, kGradY, kGradS, kGradR,
   kNearPart, kNodeNorm, kPanRec, kCutTable, kNearCounts,
+  kInv, kSolveTmp,
   kNumSlots
 };
 
@@ -42,6 +43,10 @@ struct Device {
   // LU factors of the last matrix given to o2d_hip_bem_factor_*
   int64_t lu_n = 0;
   bool lu_ready = false;
+  // solves made with the current factors; from the second one on, the explicit inverse
+  // (built once from the factors) turns each solve into one matrix-vector product
+  int64_t lu_solves = 0;
+  bool inv_ready = false;
   void* blas = nullptr;  // rocblas_handle
 };
 

@@ -70,9 +70,36 @@ def test_bem_factor_solve_matches_lapack(nunk):
     x = o2d.bem_solve(b)
     xr = np.linalg.solve(A, b)
     assert rel_l2(x, xr) <= 1e-12
-    # solve again with the same factors (the reference re-uses its solver until A changes)
-    x2 = o2d.bem_solve(2.0 * b)
-    assert rel_l2(x2, 2.0 * xr) <= 1e-12
+    # solve again with the same factors (the reference re-uses its solver until A changes): the
+    # second solve builds the explicit inverse, later ones are matrix-vector products
+    for k in range(4):
+        bk = rng.normal(size=A.shape[0])
+        assert rel_l2(o2d.bem_solve(bk), np.linalg.solve(A, bk)) <= 1e-12
+    # a new factorisation drops the inverse of the old matrix
+    A2 = A + np.diag(np.full(A.shape[0], 0.25))
+    o2d.bem_factor(A2)
+    for k in range(3):
+        bk = rng.normal(size=A.shape[0])
+        assert rel_l2(o2d.bem_solve(bk), np.linalg.solve(A2, bk)) <= 1e-12
+
+
+def test_bem_repeated_device_solves_in_place():
+    """Device entry with x aliasing b, on the inverse path, against LAPACK."""
+    import torch
+    import omega2d_amd.device as dv
+    npan = 300
+    nx, ny, idx = circle_panels(npan)
+    nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+    vol, _, _ = O.body_volume(nx, ny, idx)
+    A = O.bem_assemble(nx, ny, idx, 2, True, np.zeros(npan), np.zeros(npan), vol)
+    dA = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
+    dv.bem_factor(dA.view(A.shape[0], A.shape[0]))
+    rng = np.random.default_rng(3)
+    for k in range(4):
+        b = rng.normal(size=A.shape[0])
+        db = torch.from_numpy(b).cuda()
+        dv.bem_solve(db, db)
+        assert rel_l2(db.cpu().numpy(), np.linalg.solve(A, b)) <= 1e-12
 
 
 def test_bem_solve_without_factor_of_that_size_fails():
