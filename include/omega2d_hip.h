@@ -95,7 +95,9 @@ int32_t o2d_hip_flops_per_pair(int32_t core, int32_t point_targets, int32_t pane
 #define O2D_KT_NBODY_PLAIN_PT 9 /* plain fp64 kernel, point targets */
 #define O2D_KT_NEAR 10          /* particle/panel proximity scan (distances, reflect, clear) */
 #define O2D_KT_VORT 11          /* Gaussian vorticity of particles */
-#define O2D_KT_COUNT 12
+#define O2D_KT_TREE_BUILD 12    /* treecode: Morton orders, leaves, moments (all build kernels) */
+#define O2D_KT_TREE_EVAL 13     /* treecode: tree walk and evaluation */
+#define O2D_KT_COUNT 14
 
 /* enable = 1: from now on record a HIP event pair on the launch stream around every timed
  * kernel launch (results are unaffected); enable = 0: stop recording. */
@@ -206,6 +208,35 @@ int o2d_hip_points_on_points_adjoint_dev_d(const o2d_env* env, int64_t ns, const
                                            const double* srad, int64_t nt, const double* tx,
                                            const double* ty, const double* gu, const double* gv,
                                            double* gx, double* gy, double* gs, double* gr);
+
+/* ---- treecode summation: summation_t barneshut (src/ExecEnv.h:26-31) ---------------------- */
+
+/* Parameters of the O(N log N) particles-on-points sum. A cell of sources (radius r about its
+ * centre c) acts on a target z through its complex moments when r <= theta |z - c| and the
+ * core equals 1/d^2 for every pair to within core_tol (exactly, for the Gaussian, beyond its
+ * compact cutoff); otherwise it is opened, and leaves of 64 sources are summed directly with
+ * the exact core in fp64. The truncation error of a used cell is about (theta)^order of its
+ * contribution. order: 1..32; theta in (0, 1); core_tol >= 0 (ignored by the Gaussian). */
+typedef struct o2d_tree_opts {
+  double theta;
+  int32_t order;
+  int32_t pad;
+  double core_tol;
+} o2d_tree_opts;
+
+/* Same operator, arguments and accumulate semantics as o2d_hip_points_on_points_d (velocity
+ * only: env->results must be O2D_VELONLY; env->precision is not used, the sum is fp64). Host
+ * arrays (_d) or device arrays (_dev_d). */
+int o2d_hip_points_on_points_tree_d(const o2d_env* env, const o2d_tree_opts* opt, int64_t ns,
+                                    const double* sx, const double* sy, const double* sstr,
+                                    const double* srad, int64_t nt, const double* tx,
+                                    const double* ty, const double* trad, double* tu,
+                                    double* tv);
+int o2d_hip_points_on_points_tree_dev_d(const o2d_env* env, const o2d_tree_opts* opt,
+                                        int64_t ns, const double* sx, const double* sy,
+                                        const double* sstr, const double* srad, int64_t nt,
+                                        const double* tx, const double* ty, const double* trad,
+                                        double* tu, double* tv);
 
 /* ---- Gaussian vorticity: points_affect_points_vorticity (src/InfluenceVort.h:47-126) ------ */
 

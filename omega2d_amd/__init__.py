@@ -12,13 +12,14 @@ import ctypes
 
 import numpy as np
 
-from ._cabi import (CORES, PRECISIONS, RESULTS, Env, O2DError, call, lib, make_env,  # noqa: F401
+from ._cabi import (CORES, PRECISIONS, RESULTS, Env, O2DError, TreeOpts, call, lib,  # noqa: F401
+                    make_env,
                     CORE_RM, CORE_GAUSSIAN, CORE_WL, CORE_V2, CORE_V3, VELONLY, VELANDVORT,
                     VELANDSHEAR, PLAIN_FP64, SPLIT_FP32)
 
 __all__ = ["Env", "O2DError", "make_env", "points_on_points", "panels_on_points",
            "points_on_panels", "bem_assemble", "bem_factor", "bem_solve", "rk_move",
-           "points_on_points_adjoint", "points_vorticity", "nearest_distances", "reflect",
+           "points_on_points_adjoint", "points_on_points_tree", "points_vorticity", "nearest_distances", "reflect",
            "clear_inner",
            "flops_per_pair", "release"]
 
@@ -159,6 +160,21 @@ def points_on_points_adjoint(sx, sy, ss, sr, tx, ty, gu, gv, *, core="gauss", de
     call("o2d_hip_points_on_points_adjoint_d", ctypes.byref(env), ns, _p(sx), _p(sy), _p(ss),
          _p(sr), len(tx), _p(tx), _p(ty), _p(gu), _p(gv), *[_p(o) for o in out])
     return tuple(out)
+
+
+def points_on_points_tree(sx, sy, ss, sr, tx, ty, tr=None, *, core="gauss", theta=0.5,
+                          order=20, core_tol=1e-12, device=0, tu=None, tv=None):
+    """Treecode form of points_on_points (summation_t barneshut, src/ExecEnv.h:26-31): same
+    unscaled, accumulated velocity sum, fp64, with far cells replaced by their moments."""
+    sx, sy, ss, sr, tx, ty = (_arr(a, np.float64) for a in (sx, sy, ss, sr, tx, ty))
+    tr = None if tr is None else _arr(tr, np.float64)
+    nt = len(tx)
+    tu, tv = _out(tu, nt, np.float64), _out(tv, nt, np.float64)
+    env = make_env(core, "fp64", "vel", device)
+    opt = TreeOpts(float(theta), int(order), 0, float(core_tol))
+    call("o2d_hip_points_on_points_tree_d", ctypes.byref(env), ctypes.byref(opt), len(sx), _p(sx),
+         _p(sy), _p(ss), _p(sr), nt, _p(tx), _p(ty), _p(tr), _p(tu), _p(tv))
+    return tu, tv
 
 
 def points_vorticity(sx, sy, ss, sr, tx, ty, *, device=0, tw=None):

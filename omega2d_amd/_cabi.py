@@ -18,8 +18,8 @@ CORE_RM, CORE_GAUSSIAN, CORE_WL, CORE_V2, CORE_V3 = 1, 2, 3, 4, 5
 VELONLY, VELANDVORT, VELANDSHEAR = 1, 4, 5
 PLAIN_FP64, SPLIT_FP32 = 1, 2
 KT_NBODY_SPLIT, KT_NBODY_PLAIN, KT_PANELS, KT_PANEL_RHS, KT_BEM_ASSEMBLE, KT_ADJOINT, \
-    KT_SPLIT_PREP, KT_NBODY_SPLIT_PT, KT_NBODY_PLAIN_PT, KT_NEAR, KT_VORT = \
-    1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
+    KT_SPLIT_PREP, KT_NBODY_SPLIT_PT, KT_NBODY_PLAIN_PT, KT_NEAR, KT_VORT, KT_TREE_BUILD, \
+    KT_TREE_EVAL = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13
 
 CORES = {"rm": CORE_RM, "gauss": CORE_GAUSSIAN, "gaussian": CORE_GAUSSIAN, "wl": CORE_WL,
          "v2": CORE_V2, "v3": CORE_V3}
@@ -40,8 +40,15 @@ class O2DError(RuntimeError):
         self.code = code
 
 
+class TreeOpts(ctypes.Structure):
+    """o2d_tree_opts: theta, order, pad, core_tol."""
+    _fields_ = [("theta", ctypes.c_double), ("order", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("core_tol", ctypes.c_double)]
+
+
 _i64, _i32, _dbl, _vp = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
 _envp = ctypes.POINTER(Env)
+_optp = ctypes.POINTER(TreeOpts)
 
 # name -> argtypes (every function returns int unless listed in _RESTYPES)
 SIGNATURES = {
@@ -80,6 +87,10 @@ SIGNATURES = {
     "o2d_hip_points_on_points_adjoint_dev_d": [_envp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                                _vp, _vp, _vp, _vp, _vp, _vp],
     "o2d_hip_morton_order_dev_d": [_envp, _i64, _vp, _vp, _dbl, _dbl, _dbl, _vp, _vp],
+    "o2d_hip_points_on_points_tree_d": [_envp, _optp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                        _vp, _vp, _vp],
+    "o2d_hip_points_on_points_tree_dev_d": [_envp, _optp, _i64, _vp, _vp, _vp, _vp, _i64, _vp,
+                                            _vp, _vp, _vp, _vp],
     "o2d_hip_points_vorticity_d": [_envp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "o2d_hip_points_vorticity_dev_d": [_envp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "o2d_hip_nearest_distances_d": [_envp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp],

@@ -1,9 +1,11 @@
 """Per-configuration measurements beside the headline bench line (one JSON object per config).
 
-    python -m omega2d_amd.bench_configs [--configs c1,c3,c4,c5] [--out FILE]
+    python -m omega2d_amd.bench_configs [--configs c1,c2tree,c3,c4,c5] [--out FILE]
 
 C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update;
     the Gaussian vorticity of the disc on itself
+C2tree  the 1M blob workload with the treecode (summation_t barneshut): time, equivalent pair
+    rate and error against the fp64 direct sum, at two accuracy settings
 C3  circle of 2048 panels + 200k wake, V2 core: BEM assembly (vortex-only + augmented 2049^2,
     and vortex+source + augmented 4097^2) with HBM GB/s, LU factor + solve, RHS (points on
     panels), panels on particles, particles on particles (split and plain), and the
@@ -27,8 +29,8 @@ import torch
 from . import device as dv
 from . import flops_per_pair, workloads as wl
 from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_PLAIN_PT,
-                    KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS, KT_VORT, timing_enable,
-                    timing_read)
+                    KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS, KT_TREE_BUILD,
+                    KT_TREE_EVAL, KT_VORT, timing_enable, timing_read)
 
 PEAK_FP32, PEAK_FP64, PEAK_HBM = 157.3e12, 78.6e12, 8.0e12
 
@@ -254,12 +256,51 @@ def c5():
             "split_err_vs_fp64_16k": err}
 
 
+def c2tree():
+    """The 1M blob workload (1,048,576 particles on themselves + 65,536 tracers, Gaussian) with
+    the treecode: build and walk kernel times, the equivalent direct-sum pair rate, and the error
+    against the fp64 direct sum on 4,096 particle targets."""
+    w = wl.c2_blobs()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    tx, ty = T(w["tx"]), T(w["ty"])
+    n, nt = x.numel(), tx.numel()
+    pairs = float(n) * n + float(nt) * n
+    m = 4096
+    idx = torch.arange(0, n, n // m, device="cuda")[:m]
+    ru, rv = dv.points_on_points(x, y, s, r, x[idx].contiguous(), y[idx].contiguous(),
+                                 r[idx].contiguous(), core="gauss", precision="fp64")
+    res = {"config": "C2tree", "name": w["name"], "particles": n, "tracers": nt,
+           "equivalent_pairs": pairs}
+    for theta, order in ((0.5, 20), (0.4, 30)):
+        def step():
+            u, v = dv.points_on_points_tree(x, y, s, r, x, y, r, theta=theta, order=order)
+            dv.points_on_points_tree(x, y, s, r, tx, ty, theta=theta, order=order)
+            return u, v
+        u, v = step()
+        torch.cuda.synchronize()
+        t = timed(step, reps=3)
+        timing_read(KT_TREE_BUILD)
+        timing_read(KT_TREE_EVAL)
+        timing_enable(True)
+        step()
+        torch.cuda.synchronize()
+        timing_enable(False)
+        bms, _ = timing_read(KT_TREE_BUILD)
+        ems, _ = timing_read(KT_TREE_EVAL)
+        err = float(torch.sqrt(((u[idx] - ru) ** 2 + (v[idx] - rv) ** 2).sum() /
+                               (ru ** 2 + rv ** 2).sum()))
+        res[f"theta{theta}_order{order}"] = {
+            "ms_per_step": round(t * 1e3, 3), "build_ms": round(bms, 3), "walk_ms": round(ems, 3),
+            "equivalent_Gint_s": round(pairs / t / 1e9, 1), "err_vs_fp64_direct": err}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--configs", default="c1,c3,c4,c5")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
-    fns = {"c1": c1, "c3": c3, "c4": c4, "c5": c5}
+    fns = {"c1": c1, "c2tree": c2tree, "c3": c3, "c4": c4, "c5": c5}
     out = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()

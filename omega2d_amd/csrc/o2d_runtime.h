@@ -30,6 +30,7 @@ enum Slot : int {
   kGradX, kGradY, kGradS, kGradR,
   kNearPart, kNodeNorm, kPanRec, kCutTable, kNearCounts,
   kInv, kSolveTmp,
+  kTreeKeys, kTreeOrder, kTreeTOrder, kTreeSrc, kTreeCells, kTreeMom,
   kNumSlots
 };
 

@@ -1,0 +1,409 @@
+// treecode.hip - O(N log N) particles-on-points velocity, the summation the reference reserves
+// as summation_t barneshut (src/ExecEnv.h:26-31) behind the same operator as the direct sum.
+//
+// Tree: sources in Morton order, cut into leaves of kLeaf consecutive sources; each coarser
+// level groups kFan consecutive cells, so every cell is one contiguous Morton range and the tree
+// needs no pointers. Per cell: bounding-box centre c, radius r (covers every source), largest
+// radius term, and the complex moments M_k = sum_j s_j (z_j - c)^k, k < p (fp64). Parents get
+// their moments by the binomial shift of their children's.
+//
+// Evaluation: one wave per 64 Morton-ordered targets walks the tree from the root with a stack
+// in LDS. A cell is used through its moments when, for every lane of the wave,
+//   r <= theta |z - c|                      (truncation of the Laurent series)
+//   the core equals 1/d^2 for every pair within core_tol, from d_min = |z - c| - r: exactly
+//   beyond the Gaussian's compact cutoff (d^2 > 16 w, src/CoreFunc.h:120-122); for the other
+//   cores through the bound on |K d^2 - 1| (RM w/d^2, WL w^2/d^4, V2 w/(2 d^4), V3 w/(3 d^6)).
+// Otherwise the cell is opened, and a leaf is summed directly with the exact core in fp64.
+// With S = sum_k M_k / (z - c)^(k+1), the unscaled velocity is u - i v = -i S, i.e.
+// u = Im S, v = Re S.
+#include <cmath>
+#include <cstring>
+
+#include "nbody.h"
+#include "o2d_math.h"
+#include "o2d_runtime.h"
+
+namespace o2d {
+
+namespace {
+
+constexpr int kLeaf = 64;     // sources per leaf
+constexpr int kFan = 4;       // children per cell
+constexpr int kMaxP = 32;     // largest expansion order
+constexpr int kMaxLevels = 15;  // 2^31 sources make 14 levels
+constexpr int kTBlock = 256;  // 4 waves, each with its own 64 targets
+constexpr int kStack = 96;    // per-wave stack entries (depth <= (kFan - 1) * levels + 1)
+constexpr int kBlock = 256;
+
+struct Cell {
+  double cx, cy, r, wmax;
+  double x0, y0, x1, y1;  // bounding box of the cell's sources
+};
+
+struct Levels {
+  int n;                      // number of levels (level 0 = leaves, n - 1 = root)
+  int64_t off[kMaxLevels];    // first cell of each level in the cell array
+  int64_t cnt[kMaxLevels];    // cells per level
+};
+
+__global__ __launch_bounds__(kBlock) void tree_gather_kernel(
+    int64_t n, const uint32_t* __restrict__ order, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ s, const double* __restrict__ r,
+    int core, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ os,
+    double* __restrict__ ow) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = order[i];
+  ox[i] = x[j];
+  oy[i] = y[j];
+  if (os) os[i] = s[j];
+  if (ow) {
+    const double rr = r ? r[j] : 0.0;
+    ow[i] = core == O2D_CORE_V2   ? rterm<O2D_CORE_V2>(rr)
+            : core == O2D_CORE_V3 ? rterm<O2D_CORE_V3>(rr)
+                                  : rterm<O2D_CORE_RM>(rr);
+  }
+}
+
+// leaves: bounding box, centre, radius, largest radius term and moments of <= kLeaf sources
+__global__ __launch_bounds__(kBlock) void tree_leaf_kernel(
+    int64_t ns, int64_t nleaf, int p, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ s, const double* __restrict__ w, Cell* __restrict__ cell,
+    double2* __restrict__ mom) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nleaf) return;
+  const int64_t j0 = c * kLeaf, j1 = (j0 + kLeaf < ns) ? j0 + kLeaf : ns;
+  double x0 = x[j0], x1 = x[j0], y0 = y[j0], y1 = y[j0], wm = 0.0;
+  for (int64_t j = j0; j < j1; ++j) {
+    x0 = fmin(x0, x[j]);
+    x1 = fmax(x1, x[j]);
+    y0 = fmin(y0, y[j]);
+    y1 = fmax(y1, y[j]);
+    wm = fmax(wm, w[j]);
+  }
+  const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+  double r2 = 0.0;
+  double2 m[kMaxP];
+  for (int k = 0; k < p; ++k) m[k] = make_double2(0.0, 0.0);
+  for (int64_t j = j0; j < j1; ++j) {
+    const double dx = x[j] - cx, dy = y[j] - cy;
+    r2 = fmax(r2, dx * dx + dy * dy);
+    double pr = s[j], pi = 0.0;  // s (z - c)^k
+    for (int k = 0; k < p; ++k) {
+      m[k].x += pr;
+      m[k].y += pi;
+      const double nr = pr * dx - pi * dy;
+      pi = pr * dy + pi * dx;
+      pr = nr;
+    }
+  }
+  // rounding of the offsets: a relative margin keeps r an upper bound
+  cell[c] = Cell{cx, cy, sqrt(r2) * (1.0 + 1e-12), wm, x0, y0, x1, y1};
+  for (int k = 0; k < p; ++k) mom[c * p + k] = m[k];
+}
+
+// one coarser level: bounding-box union, covering radius, and moments shifted to the new centre
+// with M'_k = sum_{l <= k} C(k, l) M_l (c_child - c)^(k - l)
+__global__ __launch_bounds__(kBlock) void tree_up_kernel(int64_t nchild, int64_t ncell, int p,
+                                                         const Cell* __restrict__ child,
+                                                         const double2* __restrict__ cmom,
+                                                         Cell* __restrict__ cell,
+                                                         double2* __restrict__ mom) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncell) return;
+  const int64_t k0 = c * kFan, k1 = (k0 + kFan < nchild) ? k0 + kFan : nchild;
+  double x0 = child[k0].x0, y0 = child[k0].y0, x1 = child[k0].x1, y1 = child[k0].y1, wm = 0.0;
+  for (int64_t k = k0; k < k1; ++k) {
+    x0 = fmin(x0, child[k].x0);
+    y0 = fmin(y0, child[k].y0);
+    x1 = fmax(x1, child[k].x1);
+    y1 = fmax(y1, child[k].y1);
+    wm = fmax(wm, child[k].wmax);
+  }
+  const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+  double r = 0.0;
+  double2 m[kMaxP];
+  for (int k = 0; k < p; ++k) m[k] = make_double2(0.0, 0.0);
+  for (int64_t ch = k0; ch < k1; ++ch) {
+    const double dx = child[ch].cx - cx, dy = child[ch].cy - cy;
+    r = fmax(r, sqrt(dx * dx + dy * dy) + child[ch].r);
+    // powers of the shift d = c_child - c
+    double2 dp[kMaxP];
+    dp[0] = make_double2(1.0, 0.0);
+    for (int k = 1; k < p; ++k)
+      dp[k] = make_double2(dp[k - 1].x * dx - dp[k - 1].y * dy, dp[k - 1].x * dy + dp[k - 1].y * dx);
+    for (int k = 0; k < p; ++k) {
+      double binom = 1.0;  // C(k, l), built up along l
+      double ar = 0.0, ai = 0.0;
+      for (int l = 0; l <= k; ++l) {
+        const double2 ml = cmom[ch * p + l];
+        const double2 d = dp[k - l];
+        ar += binom * (ml.x * d.x - ml.y * d.y);
+        ai += binom * (ml.x * d.y + ml.y * d.x);
+        binom = binom * (double)(k - l) / (double)(l + 1);
+      }
+      m[k].x += ar;
+      m[k].y += ai;
+    }
+  }
+  cell[c] = Cell{cx, cy, r * (1.0 + 1e-12), wm, x0, y0, x1, y1};
+  for (int k = 0; k < p; ++k) mom[c * p + k] = m[k];
+}
+
+// may the moments of a cell stand for its sources at a target d_min away?
+template <int CORE>
+__device__ __forceinline__ bool core_far(double dm2, double w, double tol) {
+  if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+    return dm2 > 16.0 * w * (1.0 + 1e-12);
+  } else if constexpr (CORE == O2D_CORE_RM) {
+    return w <= tol * dm2;
+  } else if constexpr (CORE == O2D_CORE_WL) {
+    return w * w <= tol * dm2 * dm2;
+  } else if constexpr (CORE == O2D_CORE_V2) {
+    return w <= 2.0 * tol * dm2 * dm2;
+  } else {
+    return w <= 3.0 * tol * dm2 * dm2 * dm2;
+  }
+}
+
+struct TreeArgs {
+  Levels lv;
+  const Cell* cell;
+  const double2* mom;
+  const double* sx;
+  const double* sy;
+  const double* ss;
+  const double* sw;
+  int64_t ns;
+  int p;
+  double theta;
+  double tol;
+};
+
+template <int CORE>
+__global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t nt,
+                                                            const uint32_t* __restrict__ torder,
+                                                            const double* __restrict__ tx,
+                                                            const double* __restrict__ ty,
+                                                            const double* __restrict__ tr,
+                                                            double* __restrict__ tu,
+                                                            double* __restrict__ tv) {
+  __shared__ unsigned stack[kTBlock / 64][kStack];
+  const int wave = threadIdx.x / 64;
+  unsigned* st = stack[wave];
+  const int64_t i = (int64_t)blockIdx.x * kTBlock + threadIdx.x;
+  const bool valid = i < nt;
+  const uint32_t ti = valid ? torder[i] : 0u;
+  const double x = valid ? tx[ti] : 0.0, y = valid ? ty[ti] : 0.0;
+  const double wt = (valid && tr) ? rterm<CORE>(tr[ti]) : 0.0;
+  double u = 0.0, v = 0.0;     // direct part
+  double sr = 0.0, si = 0.0;   // S, the far part
+  const double th2 = a.theta * a.theta;
+  // a wave with no valid lane (the tail of the grid) has nothing to do
+  if (__any(valid)) {
+    int sp = 0;
+    const int root = a.lv.n - 1;
+    st[sp++] = (unsigned)root << 27;  // (level << 27) | index; the root is index 0
+    while (sp > 0) {
+      const unsigned e = st[--sp];
+      const int lev = (int)(e >> 27), idx = (int)(e & ((1u << 27) - 1));
+      const int64_t ci = a.lv.off[lev] + idx;
+      const Cell c = a.cell[ci];
+      const double dx = x - c.cx, dy = y - c.cy;
+      const double d2 = dx * dx + dy * dy;
+      bool acc = true;
+      if (valid) {
+        const double d = sqrt(d2);
+        const double dm = d - c.r;
+        acc = (c.r * c.r <= th2 * d2) && dm > 0.0 && core_far<CORE>(dm * dm, c.wmax + wt, a.tol);
+      }
+      if (__all(acc)) {
+        if (valid) {
+          // t = 1/(z - c); S += sum_k M_k t^(k+1) by Horner
+          const double q = 1.0 / d2;
+          const double tre = dx * q, tim = -dy * q;
+          const double2* m = a.mom + ci * a.p;
+          double hr = m[a.p - 1].x, hi = m[a.p - 1].y;
+          for (int k = a.p - 2; k >= 0; --k) {
+            const double nr = hr * tre - hi * tim + m[k].x;
+            hi = hr * tim + hi * tre + m[k].y;
+            hr = nr;
+          }
+          sr += hr * tre - hi * tim;
+          si += hr * tim + hi * tre;
+        }
+      } else if (lev == 0) {
+        if (valid) {
+          const int64_t j0 = (int64_t)idx * kLeaf;
+          const int64_t j1 = (j0 + kLeaf < a.ns) ? j0 + kLeaf : a.ns;
+          for (int64_t j = j0; j < j1; ++j) {
+            const double ex = x - a.sx[j], ey = y - a.sy[j];
+            const double e2 = __builtin_fma(ex, ex, ey * ey);
+            const double g = a.ss[j] * core_k64<CORE>(e2, a.sw[j] + wt);
+            u = __builtin_fma(-g, ey, u);
+            v = __builtin_fma(g, ex, v);
+          }
+        }
+      } else {
+        const int64_t first = (int64_t)idx * kFan;
+        const int64_t cnt = a.lv.cnt[lev - 1];
+        for (int k = kFan - 1; k >= 0; --k)
+          if (first + k < cnt) st[sp++] = ((unsigned)(lev - 1) << 27) | (unsigned)(first + k);
+      }
+    }
+  }
+  if (valid) {
+    tu[ti] += u + si;
+    tv[ti] += v + sr;
+  }
+}
+
+}  // namespace
+
+int tree_dev(Device* d, hipStream_t s, int core, double theta, int p, double tol, int64_t ns,
+             const double* sx, const double* sy, const double* ss, const double* sr, int64_t nt,
+             const double* tx, const double* ty, const double* tr, double* tu, double* tv) {
+  if (ns == 0 || nt == 0) return 0;
+  // levels
+  Levels lv{};
+  int64_t cnt = (ns + kLeaf - 1) / kLeaf, total = 0;
+  for (;;) {
+    if (lv.n == kMaxLevels) return fail(O2D_ERR_SIZE, "too many sources for the tree");
+    lv.off[lv.n] = total;
+    lv.cnt[lv.n] = cnt;
+    total += cnt;
+    ++lv.n;
+    if (cnt == 1) break;
+    cnt = (cnt + kFan - 1) / kFan;
+  }
+  if ((kFan - 1) * lv.n + 1 > kStack) return fail(O2D_ERR_SIZE, "tree too deep");
+  // buffers
+  void *ko, *to, *tk, *sbuf, *cb, *mb;
+  O2D_TRY(slot(d, kTreeKeys, sizeof(uint32_t) * (ns > nt ? ns : nt), s, &tk));
+  O2D_TRY(slot(d, kTreeOrder, sizeof(uint32_t) * ns, s, &ko));
+  O2D_TRY(slot(d, kTreeTOrder, sizeof(uint32_t) * nt, s, &to));
+  O2D_TRY(slot(d, kTreeSrc, sizeof(double) * 4 * ns, s, &sbuf));
+  O2D_TRY(slot(d, kTreeCells, sizeof(Cell) * total, s, &cb));
+  O2D_TRY(slot(d, kTreeMom, sizeof(double2) * p * total, s, &mb));
+  uint32_t* sorder = static_cast<uint32_t*>(ko);
+  uint32_t* torder = static_cast<uint32_t*>(to);
+  double* xs = static_cast<double*>(sbuf);
+  double *ys = xs + ns, *ssorted = ys + ns, *ws = ssorted + ns;
+  Cell* cells = static_cast<Cell*>(cb);
+  double2* mom = static_cast<double2*>(mb);
+  {
+    KernelTimer kt(O2D_KT_TREE_BUILD, s);
+    O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(tk),
+                         sorder));
+    tree_gather_kernel<<<(unsigned)((ns + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        ns, sorder, sx, sy, ss, sr, core, xs, ys, ssorted, ws);
+    O2D_LAUNCH_CHECK();
+    tree_leaf_kernel<<<(unsigned)((lv.cnt[0] + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        ns, lv.cnt[0], p, xs, ys, ssorted, ws, cells, mom);
+    O2D_LAUNCH_CHECK();
+    for (int l = 1; l < lv.n; ++l) {
+      tree_up_kernel<<<(unsigned)((lv.cnt[l] + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+          lv.cnt[l - 1], lv.cnt[l], p, cells + lv.off[l - 1], mom + lv.off[l - 1] * p,
+          cells + lv.off[l], mom + lv.off[l] * p);
+      O2D_LAUNCH_CHECK();
+    }
+    O2D_TRY(morton_order(d, s, nt, tx, ty, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(tk),
+                         torder));
+    kt.stop();
+  }
+  TreeArgs a{lv, cells, mom, xs, ys, ssorted, ws, ns, p, theta, tol};
+  const dim3 grid((unsigned)((nt + kTBlock - 1) / kTBlock));
+  KernelTimer kt(O2D_KT_TREE_EVAL, s);
+  switch (core) {
+    case O2D_CORE_RM:
+      tree_eval_kernel<O2D_CORE_RM><<<grid, kTBlock, 0, s>>>(a, nt, torder, tx, ty, tr, tu, tv);
+      break;
+    case O2D_CORE_GAUSSIAN:
+      tree_eval_kernel<O2D_CORE_GAUSSIAN><<<grid, kTBlock, 0, s>>>(a, nt, torder, tx, ty, tr, tu,
+                                                                   tv);
+      break;
+    case O2D_CORE_WL:
+      tree_eval_kernel<O2D_CORE_WL><<<grid, kTBlock, 0, s>>>(a, nt, torder, tx, ty, tr, tu, tv);
+      break;
+    case O2D_CORE_V2:
+      tree_eval_kernel<O2D_CORE_V2><<<grid, kTBlock, 0, s>>>(a, nt, torder, tx, ty, tr, tu, tv);
+      break;
+    default:
+      tree_eval_kernel<O2D_CORE_V3><<<grid, kTBlock, 0, s>>>(a, nt, torder, tx, ty, tr, tu, tv);
+      break;
+  }
+  kt.stop();
+  O2D_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace o2d
+
+using namespace o2d;
+
+extern "C" {
+
+static int check_tree(const o2d_env* env, const o2d_tree_opts* opt, int64_t ns, const void* sx,
+                      const void* sy, const void* ss, const void* sr, int64_t nt,
+                      const void* tx, const void* ty, const void* tu, const void* tv) {
+  O2D_TRY(check_env(env));
+  if (env->results != O2D_VELONLY)
+    return fail(O2D_ERR_RESULTS, "the treecode computes velocity only (results=%d)",
+                env->results);
+  if (!opt) return fail(O2D_ERR_ARG, "tree options are NULL");
+  if (!(opt->theta > 0.0 && opt->theta < 1.0))
+    return fail(O2D_ERR_ARG, "theta must lie in (0, 1) (got %g)", opt->theta);
+  if (opt->order < 1 || opt->order > kMaxP)
+    return fail(O2D_ERR_ARG, "expansion order must be 1..%d (got %d)", kMaxP, (int)opt->order);
+  if (!(opt->core_tol >= 0.0)) return fail(O2D_ERR_ARG, "core_tol must be >= 0");
+  if (ns < 0 || nt < 0) return fail(O2D_ERR_SIZE, "negative count");
+  if (ns > 0x7fffffffLL || nt > 0x7fffffffLL) return fail(O2D_ERR_SIZE, "count too large");
+  if (ns > 0 && (!sx || !sy || !ss || !sr)) return fail(O2D_ERR_ARG, "NULL source array");
+  if (nt > 0 && (!tx || !ty || !tu || !tv)) return fail(O2D_ERR_ARG, "NULL target array");
+  return 0;
+}
+
+int o2d_hip_points_on_points_tree_dev_d(const o2d_env* env, const o2d_tree_opts* opt,
+                                        int64_t ns, const double* sx, const double* sy,
+                                        const double* sstr, const double* srad, int64_t nt,
+                                        const double* tx, const double* ty, const double* trad,
+                                        double* tu, double* tv) {
+  clear_error();
+  O2D_TRY(check_tree(env, opt, ns, sx, sy, sstr, srad, nt, tx, ty, tu, tv));
+  if (ns == 0 || nt == 0) return 0;
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  return tree_dev(d, dev_stream(env), env->core, opt->theta, opt->order, opt->core_tol, ns, sx,
+                  sy, sstr, srad, nt, tx, ty, trad, tu, tv);
+}
+
+int o2d_hip_points_on_points_tree_d(const o2d_env* env, const o2d_tree_opts* opt, int64_t ns,
+                                    const double* sx, const double* sy, const double* sstr,
+                                    const double* srad, int64_t nt, const double* tx,
+                                    const double* ty, const double* trad, double* tu,
+                                    double* tv) {
+  clear_error();
+  O2D_TRY(check_tree(env, opt, ns, sx, sy, sstr, srad, nt, tx, ty, tu, tv));
+  if (ns == 0 || nt == 0) return 0;
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  hipStream_t s = stream_of(env, d);
+  double *dsx, *dsy, *dss, *dsr, *dtx, *dty, *dtr = nullptr, *du, *dv;
+  O2D_TRY(upload<double>(d, kSrcX, sx, ns, s, &dsx));
+  O2D_TRY(upload<double>(d, kSrcY, sy, ns, s, &dsy));
+  O2D_TRY(upload<double>(d, kSrcS, sstr, ns, s, &dss));
+  O2D_TRY(upload<double>(d, kSrcR, srad, ns, s, &dsr));
+  O2D_TRY(upload<double>(d, kTrgX, tx, nt, s, &dtx));
+  O2D_TRY(upload<double>(d, kTrgY, ty, nt, s, &dty));
+  if (trad) O2D_TRY(upload<double>(d, kTrgR, trad, nt, s, &dtr));
+  O2D_TRY(upload<double>(d, kOutU, tu, nt, s, &du));
+  O2D_TRY(upload<double>(d, kOutV, tv, nt, s, &dv));
+  O2D_TRY(tree_dev(d, s, env->core, opt->theta, opt->order, opt->core_tol, ns, dsx, dsy, dss,
+                   dsr, nt, dtx, dty, dtr, du, dv));
+  O2D_TRY(download<double>(tu, du, nt, s));
+  O2D_TRY(download<double>(tv, dv, nt, s));
+  O2D_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // extern "C"

@@ -8,11 +8,11 @@ import ctypes
 
 import torch
 
-from ._cabi import VELANDSHEAR, VELANDVORT, VELONLY, call, make_env
+from ._cabi import VELANDSHEAR, VELANDVORT, VELONLY, TreeOpts, call, make_env
 
 __all__ = ["points_on_points", "panels_on_points", "points_on_panels", "bem_assemble",
            "bem_factor", "bem_solve", "rk_move", "points_on_points_adjoint", "morton_order",
-           "points_vorticity", "nearest_distances", "reflect", "clear_inner", "env_for"]
+           "points_on_points_tree", "points_vorticity", "nearest_distances", "reflect", "clear_inner", "env_for"]
 
 
 def _ptr(t):
@@ -145,6 +145,21 @@ def morton_order(x, y, x0, y0, inv_extent):
     call("o2d_hip_morton_order_dev_d", ctypes.byref(env), n, _ptr(x), _ptr(y), float(x0),
          float(y0), float(inv_extent), _ptr(keys), _ptr(order))
     return keys, order
+
+
+def points_on_points_tree(sx, sy, ss, sr, tx, ty, tr=None, *, core="gauss", theta=0.5, order=20,
+                          core_tol=1e-12, tu=None, tv=None):
+    """Treecode velocity sum on device tensors (accumulates into tu, tv; zeros when absent)."""
+    dev = _check(sx, sy, ss, sr, tx, ty, tr, tu, tv)
+    nt = tx.numel()
+    tu = torch.zeros(nt, dtype=torch.float64, device=dev) if tu is None else tu
+    tv = torch.zeros(nt, dtype=torch.float64, device=dev) if tv is None else tv
+    env = env_for(dev, core=core)
+    opt = TreeOpts(float(theta), int(order), 0, float(core_tol))
+    call("o2d_hip_points_on_points_tree_dev_d", ctypes.byref(env), ctypes.byref(opt), sx.numel(),
+         _ptr(sx), _ptr(sy), _ptr(ss), _ptr(sr), nt, _ptr(tx), _ptr(ty), _ptr(tr), _ptr(tu),
+         _ptr(tv))
+    return tu, tv
 
 
 def points_vorticity(sx, sy, ss, sr, tx, ty, tw=None):

@@ -26,6 +26,12 @@ class ExecEnv {
   void set_internal(bool v) { m_internal = v; }
   bool is_internal() const { return m_internal; }
   void set_summation(summation_t s) { m_summ = s; }
+  summation_t get_summation() const { return m_summ; }
+  // treecode parameters (summation_t barneshut): opening angle, expansion order, core tolerance
+  void set_tree(double theta, int32_t order, double core_tol = 1e-12) {
+    m_tree = o2d_tree_opts{theta, order, 0, core_tol};
+  }
+  const o2d_tree_opts& get_tree() const { return m_tree; }
   void set_instrs(accel_t a) { m_accel = a; }
   accel_t get_instrs() const { return m_accel; }
 
@@ -68,6 +74,7 @@ class ExecEnv {
   int32_t m_core = O2D_CORE_V2;
   int32_t m_prec = O2D_PLAIN_FP64;
   int32_t m_device = 0;
+  o2d_tree_opts m_tree = {0.5, 20, 0, 1e-12};
 };
 
 enum results_t { velonly = 1, velandgrad = 2, psionly = 3, velandvort = 4, velandshear = 5 };

@@ -59,6 +59,30 @@ void points_affect_points(const Points<S>& src, Points<S>& targ, const ResultsTy
   S* tw = restype.compute_vort() ? targ.get_vort().data() : nullptr;
   S* te = restype.compute_shear() ? targ.get_shear().data() : nullptr;
   const S* tr = targ.is_inert() ? nullptr : targ.get_rad().data();
+  if (env.get_summation() == barneshut && restype.get_type() == velonly) {
+    // the treecode (fp64, double arrays), summation_t barneshut; vorticity and shear outputs
+    // stay on the direct sum below
+    const auto sxd = as_double(sx[0]), syd = as_double(sx[1]);
+    const auto ssd = as_double(src.get_str()), srd = as_double(src.get_rad());
+    const auto txd = as_double(tx[0]), tyd = as_double(tx[1]);
+    std::vector<double> trd;
+    if (tr) trd = as_double(targ.get_rad());
+    auto ud = as_double(tu[0]), vd = as_double(tu[1]);
+    o2d_check(o2d_hip_points_on_points_tree_d(&e, &env.get_tree(), (int64_t)src.get_n(),
+                                              sxd.data(), syd.data(), ssd.data(), srd.data(),
+                                              (int64_t)targ.get_n(), txd.data(), tyd.data(),
+                                              tr ? trd.data() : nullptr, ud.data(), vd.data()),
+              "o2d_hip_points_on_points_tree");
+    for (size_t i = 0; i < targ.get_n(); ++i) {
+      tu[0][i] = S(ud[i]);
+      tu[1][i] = S(vd[i]);
+    }
+    // the log reports the rate of the direct sum the treecode stands in for
+    const int per = o2d_hip_flops_per_pair(env.get_core(), targ.is_inert() ? 1 : 0, 0);
+    log_rate("points_affect_points", start,
+             (double)targ.get_n() * (2.0 + per * (double)src.get_n()));
+    return;
+  }
   int rc;
   if constexpr (std::is_same_v<S, float>) {
     rc = o2d_hip_points_on_points_f(&e, (int64_t)src.get_n(), sx[0].data(), sx[1].data(),

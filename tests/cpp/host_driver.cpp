@@ -5,6 +5,7 @@
 //
 //   o2d_host_driver <mode> <in.bin> <out.bin>
 //   mode: rk       one advection step of free particles (order, core, storage from the input)
+//         rktree   the same step with the treecode summation (summation_t barneshut)
 //         bem      BEM solve on a reactive circle + velocities of the wake particles
 //         bemstep  one advection step of the wake with the BEM solved at every stage and the
 //                  inner layer cleared after every move
@@ -66,7 +67,8 @@ struct Writer {
 
 // header: n, order, core, storage(0 double / 1 float), precision; then dt, x, y, s, r
 template <class S>
-void run_rk(Reader& in, Writer& out, int64_t n, int order, int core, int prec) {
+void run_rk(Reader& in, Writer& out, int64_t n, int order, int core, int prec,
+            summation_t summ = direct) {
   const double dt = in.f64();
   const auto x = in.vec<double>(n), y = in.vec<double>(n), s = in.vec<double>(n),
              r = in.vec<double>(n);
@@ -74,9 +76,10 @@ void run_rk(Reader& in, Writer& out, int64_t n, int order, int core, int prec) {
       rs(r.begin(), r.end());
   std::vector<Collection<S>> vort, bdry, fldpt;
   vort.emplace_back(Points<S>(n, xs.data(), ys.data(), ss.data(), rs.data()));
-  ExecEnv env(true, direct, gpu_hip);
+  ExecEnv env(true, summ, gpu_hip);
   env.set_core(core);
   env.set_precision(prec);
+  env.set_tree(0.4, 30);
   Convection<S, double, Int> conv(env, order);
   BEM<S, Int> bem;
   conv.advect(0.0, dt, {0.0, 0.0}, S(0.1), vort, bdry, fldpt, bem);
@@ -168,7 +171,7 @@ void run_vtk(Reader& in, const std::string& dir) {
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|bem|bemstep|vort|vtk> <in.bin> <out.bin|dir>\n";
+    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|vort|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
@@ -179,11 +182,12 @@ int main(int argc, char** argv) {
       return 0;
     }
     Writer out(argv[3]);
-    if (mode == "rk") {
+    if (mode == "rk" || mode == "rktree") {
       const int64_t n = in.i64(), order = in.i64(), core = in.i64(), storage = in.i64(),
                     prec = in.i64();
-      if (storage == 1) run_rk<float>(in, out, n, (int)order, (int)core, (int)prec);
-      else run_rk<double>(in, out, n, (int)order, (int)core, (int)prec);
+      const summation_t summ = mode == "rk" ? direct : barneshut;
+      if (storage == 1) run_rk<float>(in, out, n, (int)order, (int)core, (int)prec, summ);
+      else run_rk<double>(in, out, n, (int)order, (int)core, (int)prec, summ);
     } else if (mode == "bem" || mode == "bemstep") {
       run_bem(in, out, mode);
     } else if (mode == "vort") {

@@ -63,6 +63,25 @@ def test_rk_step_free_particles(tmp_path, order):
     assert np.max(np.abs(gx - ex)) <= 1e-14 and np.max(np.abs(gy - ey)) <= 1e-14
 
 
+def test_rk2_step_with_treecode_summation(tmp_path):
+    """Convection with ExecEnv summation barneshut: the host mirror's points_affect_points takes
+    the treecode (theta 0.4, order 30); one Ralston RK2 step against the oracle's direct one."""
+    rng = np.random.default_rng(9)
+    n = 6000
+    x, y = rng.normal(0, 0.5, n), rng.normal(0, 0.5, n)
+    s, r = rng.normal(0, 1.0 / n, n), np.full(n, 2e-3)
+    dt = 0.01
+    got = run_driver(tmp_path, "rktree", [n, 2, CORE_ID["gauss"], 0, 1],
+                     [np.float64(dt), x, y, s, r], 2 * n)
+    u0 = oracle_vel("gauss", x, y, s, r)
+    x1, y1 = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * dt)
+    u1 = oracle_vel("gauss", x1, y1, s, r)
+    ex, ey = O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt)
+    scale = dt * np.max(np.hypot(u0[0], u0[1]))
+    assert np.max(np.abs(got[:n] - ex)) <= 1e-9 * scale
+    assert np.max(np.abs(got[n:] - ey)) <= 1e-9 * scale
+
+
 def test_rk_step_float_storage(tmp_path):
     # the reference application's STORE=float through the _f entry point
     w = W.c1_disc(n=2000)
