@@ -205,6 +205,33 @@ def main():
     ka_ms, ka_n = timing_read(KT_ADJOINT)
     log(f"adjoint workload: fwd {ms_fwd:.3f} ms, adjoint {ms_adj:.3f} ms, ratio {ms_adj / ms_fwd:.2f}")
 
+    # ---- the same evaluation through the treecode (summation_t barneshut, theta 0.5, order 20):
+    # time per evaluation and error against the fp64 direct sum over every target
+    def tree_step():
+        pu.zero_()
+        pv.zero_()
+        qu.zero_()
+        qv.zero_()
+        dv.points_on_points_tree(sx, sy, ss, sr, sx, sy, sr, core="gauss", theta=0.5, order=20,
+                                 tu=pu, tv=pv)
+        dv.points_on_points_tree(sx, sy, ss, sr, tx, ty, None, core="gauss", theta=0.5, order=20,
+                                 tu=qu, tv=qv)
+
+    tree_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        tree_step()
+    torch.cuda.synchronize()
+    ms_tree = (time.perf_counter() - t0) * 1e3 / args.steps
+    tu_, tv_, tqu, tqv = (a.cpu().numpy() for a in (pu, pv, qu, qv))
+    err_tree = float(np.sqrt((np.sum((tu_ - fu) ** 2) + np.sum((tv_ - fv) ** 2)
+                              + np.sum((tqu - fqu) ** 2) + np.sum((tqv - fqv) ** 2))
+                             / (np.sum(fu ** 2) + np.sum(fv ** 2) + np.sum(fqu ** 2)
+                                + np.sum(fqv ** 2))))
+    log(f"treecode: {ms_tree:.2f} ms/step, {pairs / (ms_tree * 1e-3) / 1e9:.1f} equivalent "
+        f"G-int/s, err vs fp64 {err_tree:.3e}")
+
     # ---- roofline of the dominant kernel: the split launch with particle targets (N x N pairs,
     # 19 flops each in the reference's convention plus 2 per target), timed by the library's
     # HIP events around each such launch on its stream
@@ -255,6 +282,10 @@ def main():
                              "fwd_kernel_ms": round(kf_ms / max(kf_n, 1), 4),
                              "adjoint_kernel_ms": round(ka_ms / max(ka_n, 1), 4)},
         "split_prep_ms": round(prep_ms / max(prep_n, 1), 4),
+        "treecode": {"theta": 0.5, "order": 20, "ms_per_step": round(ms_tree, 3),
+                     "equivalent_value": round(pairs / (ms_tree * 1e-3) / 1e9, 1),
+                     "err_vs_fp64": err_tree,
+                     "note": "same evaluation, O(N log N) with fp64 moments; not the headline"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
