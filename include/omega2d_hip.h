@@ -172,6 +172,27 @@ int o2d_hip_bem_assemble_dev_d(const o2d_env* env, int64_t nnodes, const double*
                                const double* ny, int64_t np, const uint32_t* idx, int32_t nunk,
                                int32_t augmented, const double* urot_u, const double* urot_v,
                                double vol, double* A, int64_t lda);
+/* The block of a source body's panels acting on a DIFFERENT target body's panels
+ * (panels_on_panels_coeff with &src != &targ, src/Coefficients.h:155-573): rows are the target's
+ * equations (tnp * nunk, plus 1 if targ_aug), columns the source's unknowns (snp * nunk, plus 1 if
+ * src_aug), column-major with leading dimension lda, scaled by 1/(2 pi), two-way averaged and
+ * with no self term. targ_aug adds a bottom row of zeros (the lengths row counts only a body's own
+ * panels); src_aug adds the right column from urot_u/urot_v, the velocity of the source body
+ * rotating at unit rate on the target panels (computed by the caller as solve_bem does,
+ * src/BEMHelper.h:202-210), and a zero corner. Both bodies use the same nunk. */
+int o2d_hip_bem_assemble_pair_d(const o2d_env* env, int64_t snn, const double* snx,
+                                const double* sny, int64_t snp, const uint32_t* sidx,
+                                int64_t tnn, const double* tnx, const double* tny, int64_t tnp,
+                                const uint32_t* tidx, int32_t nunk, int32_t src_aug,
+                                int32_t targ_aug, const double* urot_u, const double* urot_v,
+                                double* A, int64_t lda);
+int o2d_hip_bem_assemble_pair_dev_d(const o2d_env* env, int64_t snn, const double* snx,
+                                    const double* sny, int64_t snp, const uint32_t* sidx,
+                                    int64_t tnn, const double* tnx, const double* tny,
+                                    int64_t tnp, const uint32_t* tidx, int32_t nunk,
+                                    int32_t src_aug, int32_t targ_aug, const double* urot_u,
+                                    const double* urot_v, double* A, int64_t lda);
+
 /* LU-factor the n x n column-major fp64 matrix on the device and keep the factors for
  * o2d_hip_bem_solve_d (the reference re-initialises its solver only when A changes,
  * src/BEM.h:182-195). A is a host array (_d) or device array (_dev_d, not modified). */

@@ -18,7 +18,7 @@ from ._cabi import (CORES, PRECISIONS, RESULTS, Env, O2DError, TreeOpts, call, l
                     VELANDSHEAR, PLAIN_FP64, SPLIT_FP32)
 
 __all__ = ["Env", "O2DError", "make_env", "points_on_points", "panels_on_points",
-           "points_on_panels", "bem_assemble", "bem_factor", "bem_solve", "rk_move",
+           "points_on_panels", "bem_assemble", "bem_assemble_pair", "bem_factor", "bem_solve", "rk_move",
            "points_on_points_adjoint", "points_on_points_tree", "points_vorticity", "nearest_distances", "reflect",
            "clear_inner",
            "flops_per_pair", "release"]
@@ -119,6 +119,24 @@ def bem_assemble(nx, ny, idx, nunk, augmented=False, urot_u=None, urot_v=None, v
     call("o2d_hip_bem_assemble_d", ctypes.byref(env), len(nx), _p(nx), _p(ny), npan, _p(idx),
          int(nunk), int(bool(augmented)), _p(uu), _p(uv), float(vol), _p(A), n)
     return A.reshape(n, n).T
+
+
+def bem_assemble_pair(snx, sny, sidx, tnx, tny, tidx, nunk, src_aug=False, targ_aug=False,
+                      urot_u=None, urot_v=None, *, device=0):
+    """Block of source surface s acting on a different target surface t
+    (src/Coefficients.h:155-573 with &src != &targ); returns A[row, col]."""
+    snx, sny, tnx, tny = (_arr(a, np.float64) for a in (snx, sny, tnx, tny))
+    sidx, tidx = _arr(sidx, np.uint32), _arr(tidx, np.uint32)
+    snp, tnp = len(sidx) // 2, len(tidx) // 2
+    nr, nc = tnp * nunk + int(bool(targ_aug)), snp * nunk + int(bool(src_aug))
+    uu = None if urot_u is None else _arr(urot_u, np.float64)
+    uv = None if urot_v is None else _arr(urot_v, np.float64)
+    A = np.zeros(nr * nc)
+    env = make_env("v2", "fp64", "vel", device)
+    call("o2d_hip_bem_assemble_pair_d", ctypes.byref(env), len(snx), _p(snx), _p(sny), snp,
+         _p(sidx), len(tnx), _p(tnx), _p(tny), tnp, _p(tidx), int(nunk), int(bool(src_aug)),
+         int(bool(targ_aug)), _p(uu), _p(uv), _p(A), nr)
+    return A.reshape(nc, nr).T.copy()
 
 
 def bem_factor(A, *, device=0):

@@ -76,6 +76,10 @@ SIGNATURES = {
                                _i64],
     "o2d_hip_bem_assemble_dev_d": [_envp, _i64, _vp, _vp, _i64, _vp, _i32, _i32, _vp, _vp, _dbl,
                                    _vp, _i64],
+    "o2d_hip_bem_assemble_pair_d": [_envp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp,
+                                    _i32, _i32, _i32, _vp, _vp, _vp, _i64],
+    "o2d_hip_bem_assemble_pair_dev_d": [_envp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+                                        _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i64],
     "o2d_hip_bem_factor_d": [_envp, _i64, _vp, _i64],
     "o2d_hip_bem_factor_dev_d": [_envp, _i64, _vp, _i64],
     "o2d_hip_bem_solve_d": [_envp, _i64, _vp, _vp],

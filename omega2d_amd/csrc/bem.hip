@@ -1,5 +1,6 @@
-// bem.hip - fp64 BEM influence matrix of a body acting on itself (src/Coefficients.h:155-573)
-// and its LU solve (the reference solves with GMRES, src/BEM.h:169-234).
+// bem.hip - fp64 BEM influence matrix blocks, of a body acting on itself or on another body
+// (src/Coefficients.h:155-573), and the LU solve of the assembled system (the reference solves
+// with GMRES, src/BEM.h:169-234).
 //
 // Matrix layout is the reference's: column-major, column j = source unknown, row i = target
 // equation; with two unknowns per panel the columns/rows interleave [vortex, source] and
@@ -11,6 +12,7 @@
 // (tangential/normal unit-rotation velocity) and corner (2 * volume) follow
 // src/Coefficients.h:464-537.
 #include <rocblas/rocblas.h>
+#include <algorithm>
 #include <rocsolver/rocsolver.h>
 
 #include "nbody.h"
@@ -70,14 +72,17 @@ __device__ __forceinline__ void unit_panel(const Geom& s, double tx, double ty, 
   b = us * tpy + vstar * tpx;
 }
 
-template <int NUNK>
-__global__ __launch_bounds__(kBlock) void assemble_kernel(int64_t np, const Geom* __restrict__ g,
+// one thread per (target panel i, source panel j); SELF: source and target are one surface,
+// whose diagonal is the self term pi (src/Coefficients.h:389-401)
+template <int NUNK, bool SELF>
+__global__ __launch_bounds__(kBlock) void assemble_kernel(int64_t nt, const Geom* __restrict__ gs,
+                                                          const Geom* __restrict__ gt,
                                                           double* __restrict__ A, int64_t lda) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // target panel (row)
   const int64_t j = blockIdx.y;                                   // source panel (column)
-  if (i >= np) return;
-  const Geom sj = g[j];
-  const Geom ti = g[i];
+  if (i >= nt) return;
+  const Geom sj = gs[j];
+  const Geom ti = gt[i];
   const double xi = 0.5 * (ti.x1 + ti.x0), yi = 0.5 * (ti.y1 + ti.y0);
   const double xj = 0.5 * (sj.x0 + sj.x1), yj = 0.5 * (sj.y0 + sj.y1);
   double a1, b1, a2, b2;
@@ -86,11 +91,12 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(int64_t np, const Geom
   const double fact = sj.len / ti.len;
   const double fac = 1.0 / (2.0 * kPi);
   const double ntx = -ti.ty, nty = ti.tx;
+  const bool diag = SELF && i == j;
   if constexpr (NUNK == 1) {
     double c = a1 * ti.tx + b1 * ti.ty;
     c -= (a2 * ti.tx + b2 * ti.ty) * fact;
     c *= 0.5;
-    if (i == j) c = kPi;
+    if (diag) c = kPi;
     A[j * lda + i] = c * fac;
   } else {
     double c11 = a1 * ti.tx + b1 * ti.ty;
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(int64_t np, const Geom
     c12 *= 0.5;
     c21 *= 0.5;
     c22 *= 0.5;
-    if (i == j) {
+    if (diag) {
       c11 = kPi;
       c12 = 0.0;
       c21 = 0.0;
@@ -118,6 +124,32 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(int64_t np, const Geom
     c1[0] = c21 * fac;
     c1[1] = c22 * fac;
   }
+}
+
+// augmentation of a block between two different bodies (src/Coefficients.h:464-537 with
+// &src != &targ): the bottom row is 0 (panel lengths count only a body's own panels), the right
+// column is the source body's unit-rotation velocity on the target panels, the corner is 0
+template <int NUNK>
+__global__ __launch_bounds__(kBlock) void augment_pair_kernel(int64_t ns, int64_t nt,
+                                                              const Geom* __restrict__ gt,
+                                                              int src_aug, int targ_aug,
+                                                              const double* __restrict__ uu,
+                                                              const double* __restrict__ uv,
+                                                              double* __restrict__ A,
+                                                              int64_t lda) {
+  const int64_t oldr = nt * NUNK, oldc = ns * NUNK;
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (targ_aug && k < oldc) A[k * lda + oldr] = 0.0;
+  if (src_aug && k < nt) {
+    const Geom q = gt[k];
+    if (NUNK == 1) {
+      A[oldc * lda + k] = uu[k] * q.tx + uv[k] * q.ty;
+    } else {
+      A[oldc * lda + 2 * k] = uu[k] * q.tx + uv[k] * q.ty;
+      A[oldc * lda + 2 * k + 1] = uu[k] * (-q.ty) + uv[k] * q.tx;
+    }
+  }
+  if (src_aug && targ_aug && k == 0) A[oldc * lda + oldr] = 0.0;
 }
 
 // bottom row (lengths), right column (unit-rotation tangential/normal velocity), corner
@@ -186,14 +218,48 @@ int bem_assemble_dev(Device* d, hipStream_t s, int64_t nnodes, const double* nx,
   if (np > 65535) return fail(O2D_ERR_SIZE, "at most 65535 panels per body (got %lld)", (long long)np);
   const dim3 grid((unsigned)((np + kBlock - 1) / kBlock), (unsigned)np);
   KernelTimer kt(O2D_KT_BEM_ASSEMBLE, s);
-  if (nunk == 1) assemble_kernel<1><<<grid, kBlock, 0, s>>>(np, g, A, lda);
-  else assemble_kernel<2><<<grid, kBlock, 0, s>>>(np, g, A, lda);
+  if (nunk == 1) assemble_kernel<1, true><<<grid, kBlock, 0, s>>>(np, g, g, A, lda);
+  else assemble_kernel<2, true><<<grid, kBlock, 0, s>>>(np, g, g, A, lda);
   kt.stop();
   O2D_LAUNCH_CHECK();
   if (augmented) {
     const int64_t m = np * nunk > np ? np * nunk : np;
     if (nunk == 1) augment_kernel<1><<<grid1(m), kBlock, 0, s>>>(np, g, urot_u, urot_v, vol, A, lda);
     else augment_kernel<2><<<grid1(m), kBlock, 0, s>>>(np, g, urot_u, urot_v, vol, A, lda);
+    O2D_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+int bem_assemble_pair_dev(Device* d, hipStream_t s, const double* snx, const double* sny,
+                          int64_t snp, const uint32_t* sidx, const double* tnx, const double* tny,
+                          int64_t tnp, const uint32_t* tidx, int nunk, int src_aug, int targ_aug,
+                          const double* urot_u, const double* urot_v, double* A, int64_t lda) {
+  if (snp == 0 || tnp == 0) return 0;
+  if (snp > 65535)
+    return fail(O2D_ERR_SIZE, "at most 65535 source panels per block (got %lld)", (long long)snp);
+  void *gsp, *gtp;
+  O2D_TRY(slot(d, kPanGeom, sizeof(Geom) * snp, s, &gsp));
+  O2D_TRY(slot(d, kPanGeom2, sizeof(Geom) * tnp, s, &gtp));
+  Geom* gs = static_cast<Geom*>(gsp);
+  Geom* gt = static_cast<Geom*>(gtp);
+  geom_kernel<<<grid1(snp), kBlock, 0, s>>>(snp, snx, sny, sidx, gs);
+  geom_kernel<<<grid1(tnp), kBlock, 0, s>>>(tnp, tnx, tny, tidx, gt);
+  O2D_LAUNCH_CHECK();
+  const dim3 grid((unsigned)((tnp + kBlock - 1) / kBlock), (unsigned)snp);
+  KernelTimer kt(O2D_KT_BEM_ASSEMBLE, s);
+  if (nunk == 1) assemble_kernel<1, false><<<grid, kBlock, 0, s>>>(tnp, gs, gt, A, lda);
+  else assemble_kernel<2, false><<<grid, kBlock, 0, s>>>(tnp, gs, gt, A, lda);
+  kt.stop();
+  O2D_LAUNCH_CHECK();
+  if (src_aug || targ_aug) {
+    const int64_t m = std::max(snp * nunk, tnp);
+    if (nunk == 1)
+      augment_pair_kernel<1><<<grid1(m), kBlock, 0, s>>>(snp, tnp, gt, src_aug, targ_aug, urot_u,
+                                                         urot_v, A, lda);
+    else
+      augment_pair_kernel<2><<<grid1(m), kBlock, 0, s>>>(snp, tnp, gt, src_aug, targ_aug, urot_u,
+                                                         urot_v, A, lda);
     O2D_LAUNCH_CHECK();
   }
   return 0;
@@ -328,6 +394,78 @@ int o2d_hip_bem_assemble_d(const o2d_env* env, int64_t nnodes, const double* nx,
                            static_cast<double*>(dA), lda));
   O2D_HIP(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * lda, sizeof(double) * n, n,
                            hipMemcpyDeviceToHost, s));
+  O2D_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int check_pair(const o2d_env* env, int64_t snn, const void* snx, const void* sny,
+                      int64_t snp, const void* sidx, int64_t tnn, const void* tnx,
+                      const void* tny, int64_t tnp, const void* tidx, int32_t nunk,
+                      int32_t src_aug, int32_t targ_aug, const void* uu, const void* uv,
+                      const void* A, int64_t lda) {
+  if (!env) return fail(O2D_ERR_ARG, "env is NULL");
+  if (nunk != 1 && nunk != 2) return fail(O2D_ERR_RESULTS, "nunk must be 1 or 2 (got %d)", nunk);
+  if (snp < 0 || tnp < 0 || snn < 0 || tnn < 0 || snp > 0x3fffffffLL || tnp > 0x3fffffffLL)
+    return fail(O2D_ERR_SIZE, "bad panel count");
+  if (lda < tnp * nunk + (targ_aug ? 1 : 0)) return fail(O2D_ERR_SIZE, "lda too small");
+  if ((snp > 0 && (!snx || !sny || !sidx)) || (tnp > 0 && (!tnx || !tny || !tidx)) || !A)
+    return fail(O2D_ERR_ARG, "NULL array");
+  if (src_aug && tnp > 0 && (!uu || !uv)) return fail(O2D_ERR_ARG, "src_aug needs urot_u/urot_v");
+  return 0;
+}
+
+int o2d_hip_bem_assemble_pair_dev_d(const o2d_env* env, int64_t snn, const double* snx,
+                                    const double* sny, int64_t snp, const uint32_t* sidx,
+                                    int64_t tnn, const double* tnx, const double* tny,
+                                    int64_t tnp, const uint32_t* tidx, int32_t nunk,
+                                    int32_t src_aug, int32_t targ_aug, const double* urot_u,
+                                    const double* urot_v, double* A, int64_t lda) {
+  clear_error();
+  O2D_TRY(check_pair(env, snn, snx, sny, snp, sidx, tnn, tnx, tny, tnp, tidx, nunk, src_aug,
+                     targ_aug, urot_u, urot_v, A, lda));
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  return bem_assemble_pair_dev(d, dev_stream(env), snx, sny, snp, sidx, tnx, tny, tnp, tidx,
+                               nunk, src_aug, targ_aug, urot_u, urot_v, A, lda);
+}
+
+int o2d_hip_bem_assemble_pair_d(const o2d_env* env, int64_t snn, const double* snx,
+                                const double* sny, int64_t snp, const uint32_t* sidx,
+                                int64_t tnn, const double* tnx, const double* tny, int64_t tnp,
+                                const uint32_t* tidx, int32_t nunk, int32_t src_aug,
+                                int32_t targ_aug, const double* urot_u, const double* urot_v,
+                                double* A, int64_t lda) {
+  clear_error();
+  O2D_TRY(check_pair(env, snn, snx, sny, snp, sidx, tnn, tnx, tny, tnp, tidx, nunk, src_aug,
+                     targ_aug, urot_u, urot_v, A, lda));
+  for (int64_t j = 0; j < 2 * snp; ++j)
+    if ((int64_t)sidx[j] >= snn) return fail(O2D_ERR_SIZE, "source node index out of range");
+  for (int64_t j = 0; j < 2 * tnp; ++j)
+    if ((int64_t)tidx[j] >= tnn) return fail(O2D_ERR_SIZE, "target node index out of range");
+  if (snp == 0 || tnp == 0) return 0;
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  hipStream_t s = stream_of(env, d);
+  const int64_t nr = tnp * nunk + (targ_aug ? 1 : 0);
+  const int64_t nc = snp * nunk + (src_aug ? 1 : 0);
+  double *dsx, *dsy, *dtx, *dty, *duu = nullptr, *duv = nullptr;
+  uint32_t *dsi, *dti;
+  O2D_TRY(upload<double>(d, kPanX, snx, snn, s, &dsx));
+  O2D_TRY(upload<double>(d, kPanY, sny, snn, s, &dsy));
+  O2D_TRY(upload<uint32_t>(d, kPanIdx, sidx, 2 * snp, s, &dsi));
+  O2D_TRY(upload<double>(d, kPan2X, tnx, tnn, s, &dtx));
+  O2D_TRY(upload<double>(d, kPan2Y, tny, tnn, s, &dty));
+  O2D_TRY(upload<uint32_t>(d, kPan2Idx, tidx, 2 * tnp, s, &dti));
+  if (src_aug) {
+    O2D_TRY(upload<double>(d, kPanVs, urot_u, tnp, s, &duu));
+    O2D_TRY(upload<double>(d, kPanSs, urot_v, tnp, s, &duv));
+  }
+  void* dA;
+  O2D_TRY(slot(d, kMatA, sizeof(double) * lda * nc, s, &dA));
+  O2D_TRY(bem_assemble_pair_dev(d, s, dsx, dsy, snp, dsi, dtx, dty, tnp, dti, nunk, src_aug,
+                                targ_aug, duu, duv, static_cast<double*>(dA), lda));
+  O2D_HIP(hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * lda,
+                           sizeof(double) * nr, nc, hipMemcpyDeviceToHost, s));
   O2D_HIP(hipStreamSynchronize(s));
   return 0;
 }

@@ -31,6 +31,7 @@ enum Slot : int {
   kNearPart, kNodeNorm, kPanRec, kCutTable, kNearCounts,
   kInv, kSolveTmp,
   kTreeKeys, kTreeOrder, kTreeTOrder, kTreeSrc, kTreeCells, kTreeMom,
+  kPan2X, kPan2Y, kPan2Idx, kPanGeom2,
   kNumSlots
 };
 

@@ -1,6 +1,7 @@
 // BEM.h - boundary-element system of the gpu_hip environment: host mirror of src/BEM.h
-// (BEM<S,I>: set_block/set_rhs/solve/get_str), src/Coefficients.h (panels_on_panels_coeff for a
-// body acting on itself), src/RHS.h (vels_to_rhs_panels) and src/BEMHelper.h (solve_bem).
+// (BEM<S,I>: set_block/set_rhs/solve/get_str), src/Coefficients.h (panels_on_panels_coeff of a
+// body on itself or on another body), src/RHS.h (vels_to_rhs_panels) and src/BEMHelper.h
+// (solve_bem over any number of bodies).
 // The matrix is assembled and LU-solved in fp64 on the device; the factors are kept until the
 // panels change, like the reference keeps its solver (src/BEM.h:182-195).
 #pragma once
@@ -75,24 +76,39 @@ class BEM {
   int32_t device = 0;
 };
 
-// self-influence block of one body (src/Coefficients.h:155-573), assembled on the device
+// one block of the influence matrix (src/Coefficients.h:155-573), assembled on the device: a
+// body on itself, or a source body on a different target body. For an augmented source the
+// target's panel velocities must hold the source's unit-rotation velocity (solve_bem sets them).
 template <class S>
 std::vector<double> panels_on_panels_coeff(const Surfaces<S>& src, Surfaces<S>& targ,
                                            const ExecEnv& env) {
-  if (&src != &targ)
-    throw std::runtime_error("the gpu_hip BEM assembles a body's self-influence block only");
   const int32_t nunk = (int32_t)targ.num_unknowns_per_panel();
-  const bool aug = targ.is_augmented();
-  const int64_t n = (int64_t)targ.get_num_rows();
-  std::vector<double> coeffs((size_t)(n * n), 0.0);
-  const auto nx = as_double(targ.get_pos()[0]), ny = as_double(targ.get_pos()[1]);
-  const auto uu = as_double(targ.get_vel()[0]), uv = as_double(targ.get_vel()[1]);
   const o2d_env e = env.c_env(velonly);
-  o2d_check(o2d_hip_bem_assemble_d(&e, (int64_t)targ.get_n(), nx.data(), ny.data(),
-                                   (int64_t)targ.get_npanels(), targ.get_idx().data(), nunk,
-                                   aug ? 1 : 0, uu.data(), uv.data(), double(targ.get_vol()),
-                                   coeffs.data(), n),
-            "o2d_hip_bem_assemble");
+  const auto tx = as_double(targ.get_pos()[0]), ty = as_double(targ.get_pos()[1]);
+  const auto uu = as_double(targ.get_vel()[0]), uv = as_double(targ.get_vel()[1]);
+  if (&src == &targ) {
+    const bool aug = targ.is_augmented();
+    const int64_t n = (int64_t)targ.get_num_rows();
+    std::vector<double> coeffs((size_t)(n * n), 0.0);
+    o2d_check(o2d_hip_bem_assemble_d(&e, (int64_t)targ.get_n(), tx.data(), ty.data(),
+                                     (int64_t)targ.get_npanels(), targ.get_idx().data(), nunk,
+                                     aug ? 1 : 0, uu.data(), uv.data(), double(targ.get_vol()),
+                                     coeffs.data(), n),
+              "o2d_hip_bem_assemble");
+    return coeffs;
+  }
+  if ((int32_t)src.num_unknowns_per_panel() != nunk)
+    throw std::runtime_error("BEM blocks between bodies need the same unknowns per panel");
+  const int64_t nr = (int64_t)targ.get_num_rows(), nc = (int64_t)src.get_num_rows();
+  std::vector<double> coeffs((size_t)(nr * nc), 0.0);
+  const auto sx = as_double(src.get_pos()[0]), sy = as_double(src.get_pos()[1]);
+  o2d_check(o2d_hip_bem_assemble_pair_d(&e, (int64_t)src.get_n(), sx.data(), sy.data(),
+                                        (int64_t)src.get_npanels(), src.get_idx().data(),
+                                        (int64_t)targ.get_n(), tx.data(), ty.data(),
+                                        (int64_t)targ.get_npanels(), targ.get_idx().data(), nunk,
+                                        src.is_augmented() ? 1 : 0, targ.is_augmented() ? 1 : 0,
+                                        uu.data(), uv.data(), coeffs.data(), nr),
+            "o2d_hip_bem_assemble_pair");
   return coeffs;
 }
 
@@ -117,7 +133,7 @@ std::vector<S> vels_to_rhs_panels(const Surfaces<S>& targ) {
   return rhs;
 }
 
-// src/BEMHelper.h:43-260 for boundaries that are Surfaces (one body each)
+// src/BEMHelper.h:43-260 for boundaries that are Surfaces
 template <class S, class A, class I>
 void solve_bem(double time, const std::array<double, Dimensions>& fs,
                std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
@@ -154,21 +170,26 @@ void solve_bem(double time, const std::array<double, Dimensions>& fs,
 
   if (!bem.is_A_current()) {
     bem.panels_changed();
+    // every (target, source) pair of bodies, src/BEMHelper.h:176-219
     for (auto& t : bdry) {
-      auto* surf = std::get_if<Surfaces<S>>(&t);
-      if (!surf) continue;
-      if (surf->is_augmented()) {
-        // velocity on the body's own panels from its unit-rotation strengths
-        surf->zero_vels();
-        surf->zero_strengths();
-        surf->add_unit_rot_strengths();
-        panels_affect_panels<S, A>(*surf, *surf, ResultsType(velonly), env);
-        surf->zero_strengths();
-        surf->finalize_vels({0.0, 0.0});
+      auto* targ = std::get_if<Surfaces<S>>(&t);
+      if (!targ) continue;
+      for (auto& sc : bdry) {
+        auto* src = std::get_if<Surfaces<S>>(&sc);
+        if (!src) continue;
+        if (src->is_augmented()) {
+          // velocity on the target's panels from the source's unit-rotation strengths
+          targ->zero_vels();
+          src->zero_strengths();
+          src->add_unit_rot_strengths();
+          panels_affect_panels<S, A>(*src, *targ, ResultsType(velonly), env);
+          src->zero_strengths();
+          targ->finalize_vels({0.0, 0.0});
+        }
+        const auto coeffs = panels_on_panels_coeff<S>(*src, *targ, env);
+        bem.set_block(targ->get_first_row(), targ->get_num_rows(), src->get_first_row(),
+                      src->get_num_rows(), coeffs);
       }
-      const auto coeffs = panels_on_panels_coeff<S>(*surf, *surf, env);
-      bem.set_block(surf->get_first_row(), surf->get_num_rows(), surf->get_first_row(),
-                    surf->get_num_rows(), coeffs);
     }
     bem.just_made_A();
   }

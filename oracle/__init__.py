@@ -61,6 +61,8 @@ def lib():
         L.oc_unit_rot_strengths.argtypes = [_i64, _vp, _vp, _i64, _vp, _dbl, _dbl, _vp, _vp]
         L.oc_bem_assemble.argtypes = [_i64, _vp, _vp, _i64, _vp, _int, _int, _vp, _vp, _dbl,
                                       _vp, _i64]
+        L.oc_bem_assemble_pair.argtypes = [_i64, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp,
+                                           _int, _int, _int, _vp, _vp, _vp, _i64]
         L.oc_vels_to_rhs.argtypes = [_i64, _int, _vp, _vp, _vp, _vp, _vp, _vp]
         L.oc_finalize.argtypes = [_i64, _vp, _vp, _dbl, _dbl]
         L.oc_rk_combine.argtypes = [_i64, _int, _vp, _vp, _vp, _dbl, _vp, _vp]
@@ -112,6 +114,8 @@ def ref():
         R.ref_clear_inner_d.argtypes = [_int, _i64, _vp, _vp, _i64, _vp, _dbl, _dbl, _i64, _vp,
                                         _vp, _vp, _vp, _vp]
         R.ref_cut_table_d.argtypes = [_int, _vp, _vp, _vp]
+        R.ref_bem_block_d.argtypes = [_i64, _vp, _vp, _i64, _vp, _int, _i64, _vp, _vp, _i64, _vp,
+                                      _int, _int, _vp, _vp, _vp, _vp, _vp]
         R.ref_write_points_vtk_f.argtypes = [ctypes.c_char_p, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
                                              _vp, _int, _i64, _i64, _dbl, ctypes.c_char_p, _i64]
         R.ref_write_panels_vtk_f.argtypes = [ctypes.c_char_p, _i64, _vp, _vp, _i64, _vp, _vp, _vp,
@@ -228,6 +232,24 @@ def bem_assemble(nx, ny, idx, nunk, augmented, urot_u=None, urot_v=None, vol=0.0
                                _p(uu), _p(uv), vol, _p(A), n)
     assert rc == 0, rc
     return A.reshape(n, n).T  # column-major storage -> A[row, col]
+
+
+def bem_assemble_pair(snx, sny, sidx, tnx, tny, tidx, nunk, src_aug, targ_aug, urot_u=None,
+                      urot_v=None):
+    """Block of source surface s on a different target surface t: rows = t, columns = s."""
+    snx, sny, tnx, tny = map(_f64, (snx, sny, tnx, tny))
+    sidx, tidx = _u32(sidx), _u32(tidx)
+    snp, tnp = len(sidx) // 2, len(tidx) // 2
+    nr = tnp * nunk + (1 if targ_aug else 0)
+    ncol = snp * nunk + (1 if src_aug else 0)
+    uu = _f64(urot_u if urot_u is not None else np.zeros(tnp))
+    uv = _f64(urot_v if urot_v is not None else np.zeros(tnp))
+    A = np.zeros(nr * ncol)
+    rc = lib().oc_bem_assemble_pair(len(snx), _p(snx), _p(sny), snp, _p(sidx), len(tnx), _p(tnx),
+                                    _p(tny), tnp, _p(tidx), int(nunk), int(bool(src_aug)),
+                                    int(bool(targ_aug)), _p(uu), _p(uv), _p(A), nr)
+    assert rc == 0, rc
+    return A.reshape(ncol, nr).T.copy()
 
 
 def vels_to_rhs(pu, pv, tang_x, tang_y, nunk, norm_bc=None):
@@ -542,4 +564,22 @@ def ref_write_panels_vtk(directory, nx, ny, idx, vs, ss, index=0, frameno=0, tim
                                           float(time), name, 256)
     assert rc == 0, rc
     return name.value.decode()
+
+
+def ref_bem_block(snx, sny, sidx, skind, tnx, tny, tidx, tkind, nunk):
+    """Reference block of one surface on another (solve_bem's steps): dict(A, urot_u, urot_v)."""
+    snx, sny = _f32nodes(snx, sny)
+    tnx, tny = _f32nodes(tnx, tny)
+    sidx, tidx = _u32(sidx), _u32(tidx)
+    snp, tnp = len(sidx) // 2, len(tidx) // 2
+    A = np.zeros((snp * nunk + 1) * (tnp * nunk + 1))
+    nr, nc = _i64(), _i64()
+    uu, uv = np.zeros(tnp), np.zeros(tnp)
+    with quiet():
+        rc = ref().ref_bem_block_d(len(snx), _p(snx), _p(sny), snp, _p(sidx), int(skind), len(tnx),
+                                   _p(tnx), _p(tny), tnp, _p(tidx), int(tkind), int(nunk), _p(A),
+                                   ctypes.byref(nr), ctypes.byref(nc), _p(uu), _p(uv))
+    assert rc == 0, rc
+    r, c = nr.value, nc.value
+    return dict(A=A[: r * c].reshape(c, r).T.copy(), urot_u=uu, urot_v=uv)
 

@@ -23,6 +23,9 @@
  *   oc_bem_assemble        src/Coefficients.h:155-573  pinned: golden matrices (vortex-only
  *                          and vortex+source, augmented and not); diagonal 1/2; bottom row =
  *                          panel lengths.
+ *   oc_bem_assemble_pair   src/Coefficients.h:155-573 (two different surfaces)  pinned:
+ *                          golden blocks of one body on another (vortex-only and
+ *                          vortex+source, with and without augmentation of either).
  *   oc_vels_to_rhs         src/RHS.h:54-115            pinned: golden RHS vectors.
  *   oc_finalize            src/ElementBase.h:215-234   pinned: compiled reference.
  *   oc_rk_combine          src/ElementBase.h:276-359   pinned: compiled reference (2 and 3
@@ -935,4 +938,90 @@ int oc_points_vorticity(int64_t ns, const double* sx, const double* sy, const do
     tw[i] += acc;
   }
   return 0;
+}
+
+/* panels_on_panels_coeff for two different surfaces (src/Coefficients.h:155-573 with
+ * &src != &targ): the block of target equations (rows) by source unknowns (columns), two-way
+ * averaged, scaled by 1/(2 pi), with no self term. targ_aug adds the bottom row, which is 0
+ * here because the lengths row counts only a body's own panels (:474-491); src_aug adds the
+ * right column, the source body's unit-rotation velocity urot_u/urot_v on the target panels
+ * projected on their tangents (and normals) (:509-531), and the corner, 0 here (:533-537). */
+int oc_bem_assemble_pair(int64_t snn, const double* snx, const double* sny, int64_t snp,
+                         const uint32_t* sidx, int64_t tnn, const double* tnx, const double* tny,
+                         int64_t tnp, const uint32_t* tidx, int nunk, int src_aug, int targ_aug,
+                         const double* urot_u, const double* urot_v, double* A, int64_t lda) {
+  if (nunk != 1 && nunk != 2) return -4;
+  const int64_t oldr = tnp * nunk, oldc = snp * nunk;
+  const int64_t nr = oldr + (targ_aug ? 1 : 0);
+  if (lda < nr) return -2;
+  double* tt0 = (double*)malloc(sizeof(double) * (tnp > 0 ? tnp : 1));
+  double* tt1 = (double*)malloc(sizeof(double) * (tnp > 0 ? tnp : 1));
+  double* tn0 = (double*)malloc(sizeof(double) * (tnp > 0 ? tnp : 1));
+  double* tn1 = (double*)malloc(sizeof(double) * (tnp > 0 ? tnp : 1));
+  double* ta = (double*)malloc(sizeof(double) * (tnp > 0 ? tnp : 1));
+  double* st0 = (double*)malloc(sizeof(double) * (snp > 0 ? snp : 1));
+  double* st1 = (double*)malloc(sizeof(double) * (snp > 0 ? snp : 1));
+  double* sa = (double*)malloc(sizeof(double) * (snp > 0 ? snp : 1));
+  int rc = oc_panel_geometry(tnn, tnx, tny, tnp, tidx, tt0, tt1, tn0, tn1, ta);
+  if (!rc) rc = oc_panel_geometry(snn, snx, sny, snp, sidx, st0, st1, NULL, NULL, sa);
+  if (rc) goto done;
+  const double fac = 1.0 / (2.0 * O2D_PI);
+  for (int64_t j = 0; j < snp; ++j) {
+    const uint32_t sa_ = sidx[2 * j], sb_ = sidx[2 * j + 1];
+    const double sx0 = snx[sa_], sy0 = sny[sa_], sx1 = snx[sb_], sy1 = sny[sb_];
+    for (int64_t i = 0; i < tnp; ++i) {
+      const uint32_t ta_ = tidx[2 * i], tb_ = tidx[2 * i + 1];
+      const double tx0 = tnx[ta_], ty0 = tny[ta_], tx1 = tnx[tb_], ty1 = tny[tb_];
+      const double xi = 0.5 * (tx1 + tx0), yi = 0.5 * (ty1 + ty0);
+      const double fact = sa[j] / ta[i];
+      if (nunk == 2) {
+        double vu, vv, su, sv;
+        panel_1vos(sx0, sy0, sx1, sy1, 1.0, 1.0, xi, yi, &vu, &vv, &su, &sv);
+        double c11 = vu * tt0[i] + vv * tt1[i];
+        double c12 = vu * tn0[i] + vv * tn1[i];
+        double c21 = su * tt0[i] + sv * tt1[i];
+        double c22 = su * tn0[i] + sv * tn1[i];
+        panel_1vos(tx0, ty0, tx1, ty1, 1.0, 1.0, 0.5 * (sx0 + sx1), 0.5 * (sy0 + sy1), &vu, &vv,
+                   &su, &sv);
+        c11 = 0.5 * (c11 - fact * (vu * tt0[i] + vv * tt1[i]));
+        c12 = 0.5 * (c12 - fact * (vu * tn0[i] + vv * tn1[i]));
+        c21 = 0.5 * (c21 - fact * (su * tt0[i] + sv * tt1[i]));
+        c22 = 0.5 * (c22 - fact * (su * tn0[i] + sv * tn1[i]));
+        A[(2 * j) * lda + 2 * i] = c11 * fac;
+        A[(2 * j) * lda + 2 * i + 1] = c12 * fac;
+        A[(2 * j + 1) * lda + 2 * i] = c21 * fac;
+        A[(2 * j + 1) * lda + 2 * i + 1] = c22 * fac;
+      } else {
+        double vu, vv;
+        panel_1v(sx0, sy0, sx1, sy1, 1.0, xi, yi, &vu, &vv);
+        double c = vu * tt0[i] + vv * tt1[i];
+        panel_1v(tx0, ty0, tx1, ty1, 1.0, 0.5 * (sx0 + sx1), 0.5 * (sy0 + sy1), &vu, &vv);
+        c = 0.5 * (c - (vu * tt0[i] + vv * tt1[i]) * fact);
+        A[j * lda + i] = c * fac;
+      }
+    }
+  }
+  if (targ_aug)
+    for (int64_t j = 0; j < oldc; ++j) A[j * lda + oldr] = 0.0;
+  if (src_aug) {
+    for (int64_t i = 0; i < tnp; ++i) {
+      if (nunk == 1) {
+        A[oldc * lda + i] = urot_u[i] * tt0[i] + urot_v[i] * tt1[i];
+      } else {
+        A[oldc * lda + 2 * i] = urot_u[i] * tt0[i] + urot_v[i] * tt1[i];
+        A[oldc * lda + 2 * i + 1] = urot_u[i] * tn0[i] + urot_v[i] * tn1[i];
+      }
+    }
+    if (targ_aug) A[oldc * lda + oldr] = 0.0;
+  }
+done:
+  free(tt0);
+  free(tt1);
+  free(tn0);
+  free(tn1);
+  free(ta);
+  free(st0);
+  free(st1);
+  free(sa);
+  return rc;
 }

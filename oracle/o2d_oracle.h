@@ -44,6 +44,12 @@ int oc_bem_assemble(int64_t nnodes, const double* nx, const double* ny, int64_t 
                     const uint32_t* idx, int nunk, int augmented, const double* urot_u,
                     const double* urot_v, double vol, double* A, int64_t lda);
 
+/* the block of a source surface acting on a different target surface (no self term) */
+int oc_bem_assemble_pair(int64_t snn, const double* snx, const double* sny, int64_t snp,
+                         const uint32_t* sidx, int64_t tnn, const double* tnx, const double* tny,
+                         int64_t tnp, const uint32_t* tidx, int nunk, int src_aug, int targ_aug,
+                         const double* urot_u, const double* urot_v, double* A, int64_t lda);
+
 int oc_vels_to_rhs(int64_t np, int nunk, const double* pu, const double* pv,
                    const double* tang_x, const double* tang_y, const double* norm_bc,
                    double* rhs);

@@ -167,6 +167,58 @@ int bem_coeffs(int64_t nnodes, const float* nx, const float* ny, int64_t np, con
 }
 
 // RHS from panel-centre velocities (src/RHS.h:54-115) for a reactive surface with given normal BCs
+// One off-diagonal block, source surface on a different target surface, exactly as solve_bem
+// builds it (src/BEMHelper.h:184-216): when the source body is augmented, the velocity of its
+// unit-rotation strengths at the target's collocation points, then panels_on_panels_coeff.
+// kind: 0 = no Body, 2 = a named rotating body (augmented). Returns the target velocities in
+// urot_u/urot_v (zeros when the source is not augmented).
+template <class S, class A>
+int bem_block(int64_t snn, const float* snx, const float* sny, int64_t snp, const uint32_t* sidx,
+              int skind, int64_t tnn, const float* tnx, const float* tny, int64_t tnp,
+              const uint32_t* tidx, int tkind, int nunk, S* coeffs, int64_t* nrows_out,
+              int64_t* ncols_out, S* urot_u, S* urot_v) {
+  std::shared_ptr<Body> sb, tb;
+  if (skind == 2) { sb = std::make_shared<Body>(); sb->set_name("rotor_a"); }
+  if (tkind == 2) { tb = std::make_shared<Body>(); tb->set_name("rotor_b"); }
+  HarnessSurfaces<S> src = make_surfaces<S>(snn, snx, sny, snp, sidx, reactive,
+                                            sb ? bodybound : fixed, sb);
+  HarnessSurfaces<S> targ = make_surfaces<S>(tnn, tnx, tny, tnp, tidx, reactive,
+                                             tb ? bodybound : fixed, tb);
+  if (nunk == 1) {
+    src.set_vortex_only();
+    targ.set_vortex_only();
+  }
+  src.transform(0.0);
+  targ.transform(0.0);
+  targ.zero_vels();
+  if (src.is_augmented()) {
+    src.zero_strengths();
+    src.add_unit_rot_strengths();
+    ElementPacket<float> fp = targ.represent_as_particles(0.0001, -1.0);
+    ElementPacket<S> sp(std::vector<S>(fp.x.begin(), fp.x.end()), fp.idx,
+                        std::vector<S>(fp.val.begin(), fp.val.end()), fp.nelem, fp.ndim);
+    Points<S> temppts(sp, active, lagrangian, nullptr, 0.0001f);
+    temppts.zero_vels();
+    const Surfaces<S>& csrc = src;
+    panels_affect_points<S, A>(csrc, temppts, ResultsType(velonly), ExecEnv(true, direct, cpu_x86));
+    for (size_t d = 0; d < Dimensions; ++d)
+      for (int64_t i = 0; i < tnp; ++i) targ.get_vel()[d][i] += temppts.get_vel()[d][i];
+    src.zero_strengths();
+    targ.finalize_vels(std::array<double, Dimensions>({0.0, 0.0}));
+  }
+  for (int64_t i = 0; i < tnp; ++i) {
+    urot_u[i] = targ.get_vel()[0][i];
+    urot_v[i] = targ.get_vel()[1][i];
+  }
+  const Surfaces<S>& csrc = src;
+  Vector<S> c = panels_on_panels_coeff<S>(csrc, static_cast<Surfaces<S>&>(targ));
+  *nrows_out = targ.get_num_rows();
+  *ncols_out = src.get_num_rows();
+  if ((int64_t)c.size() != *nrows_out * *ncols_out) return -1;
+  std::memcpy(coeffs, c.data(), c.size() * sizeof(S));
+  return 0;
+}
+
 template <class S>
 int rhs_panels(int64_t nnodes, const float* nx, const float* ny, int64_t np, const uint32_t* idx,
                int nunk, const S* pu, const S* pv, const float* norm_bc, S* rhs) {
@@ -428,6 +480,15 @@ int ref_write_panels_vtk_f(const char* dir, int64_t nnodes, const float* nx, con
   if (chdir(cwd) != 0) return -1;
   std::snprintf(name, (size_t)cap, "%s", fn.c_str());
   return 0;
+}
+
+int ref_bem_block_d(int64_t snn, const float* snx, const float* sny, int64_t snp,
+                    const uint32_t* sidx, int skind, int64_t tnn, const float* tnx,
+                    const float* tny, int64_t tnp, const uint32_t* tidx, int tkind, int nunk,
+                    double* coeffs, int64_t* nrows, int64_t* ncols, double* urot_u,
+                    double* urot_v) {
+  return bem_block<double, double>(snn, snx, sny, snp, sidx, skind, tnn, tnx, tny, tnp, tidx,
+                                   tkind, nunk, coeffs, nrows, ncols, urot_u, urot_v);
 }
 
 }  // extern "C"

@@ -10,6 +10,8 @@
 //         bemstep  one advection step of the wake with the BEM solved at every stage and the
 //                  inner layer cleared after every move
 //         vort     Convection::find_vort of particles on themselves
+//         bem2     BEM solve on two bodies (a ground circle and a second, named body) plus the
+//                  velocities of the wake particles
 //         vtk      write particle, field-point and panel .vtu files into the directory given
 //                  in place of <out.bin> (no GPU work)
 //
@@ -128,6 +130,47 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
   }
 }
 
+// header: npanA, nnodesA, npanB, nnodesB, nwake, nunk, core; then fsx, fsy, nodes of A
+// (float32 x, y), idxA, nodes of B, idxB, wake x, y, s, r
+void run_bem2(Reader& in, Writer& out) {
+  const int64_t npa = in.i64(), nna = in.i64(), npb = in.i64(), nnb = in.i64(), nw = in.i64(),
+                nunk = in.i64(), core = in.i64();
+  const double fsx = in.f64(), fsy = in.f64();
+  const auto ax = in.vec<float>(nna), ay = in.vec<float>(nna);
+  const auto ai = in.vec<uint32_t>(2 * npa);
+  const auto bx = in.vec<float>(nnb), by = in.vec<float>(nnb);
+  const auto bi = in.vec<uint32_t>(2 * npb);
+  auto wx = in.vec<double>(nw), wy = in.vec<double>(nw), ws = in.vec<double>(nw),
+       wr = in.vec<double>(nw);
+  auto ground = std::make_shared<Body>();
+  auto other = std::make_shared<Body>();
+  other->name = "cylinder";
+  Surfaces<double> sa(nna, ax.data(), ay.data(), npa, ai.data(), reactive, bodybound, ground);
+  Surfaces<double> sb(nnb, bx.data(), by.data(), npb, bi.data(), reactive, bodybound, other);
+  if (nunk == 1) {
+    sa.set_vortex_only();
+    sb.set_vortex_only();
+  }
+  std::vector<Collection<double>> vort, bdry, fldpt;
+  vort.emplace_back(Points<double>(nw, wx.data(), wy.data(), ws.data(), wr.data()));
+  bdry.emplace_back(sa);
+  bdry.emplace_back(sb);
+  ExecEnv env(true, direct, gpu_hip);
+  env.set_core((int32_t)core);
+  Convection<double, double, Int> conv(env, 2);
+  BEM<double, Int> bem;
+  conv.find_derivs(0.0, {fsx, fsy}, bem, bdry, vort, fldpt);
+  for (auto& c : bdry) {
+    const auto& surf = std::get<Surfaces<double>>(c);
+    out.vec(surf.get_str());
+    out.vec(surf.get_src_str());
+    out.f64(surf.get_solved_omega());
+  }
+  const auto& p = std::get<Points<double>>(vort[0]);
+  out.vec(p.get_vel()[0]);
+  out.vec(p.get_vel()[1]);
+}
+
 // header: n; then x, y, s, r
 void run_vort(Reader& in, Writer& out, int64_t n) {
   const auto x = in.vec<double>(n), y = in.vec<double>(n), s = in.vec<double>(n),
@@ -171,7 +214,7 @@ void run_vtk(Reader& in, const std::string& dir) {
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|vort|vtk> <in.bin> <out.bin|dir>\n";
+    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bem2|vort|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
@@ -192,6 +235,8 @@ int main(int argc, char** argv) {
       run_bem(in, out, mode);
     } else if (mode == "vort") {
       run_vort(in, out, in.i64());
+    } else if (mode == "bem2") {
+      run_bem2(in, out);
 
     } else {
       std::cerr << "unknown mode " << mode << "\n";

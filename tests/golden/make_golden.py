@@ -157,6 +157,18 @@ def main():
     d["vo_w"] = O.ref_points_vorticity(vsx, vsy, vss, vsr, vtx, vty)
     d["vo_wf"] = O.ref_points_vorticity(vsx, vsy, vss, vsr, vtx, vty, dtype=np.float32)
 
+    # BEM blocks between two different bodies (src/Coefficients.h with &src != &targ, built as
+    # src/BEMHelper.h:184-216 builds them); body kind 2 = rotating (augmented), 0 = no body
+    ax_, ay_, ai_ = circle_panels(24)
+    bx_, by_, bi_ = circle_panels(16, diam=0.6, cx=1.6, cy=0.3)
+    d.update(bp_ax=ax_, bp_ay=ay_, bp_ai=ai_, bp_bx=bx_, bp_by=by_, bp_bi=bi_)
+    for nunk in (1, 2):
+        for sk, tk in ((2, 2), (2, 0), (0, 2), (0, 0)):
+            for tag, geo in (("ab", (ax_, ay_, ai_, bx_, by_, bi_)), ("ba", (bx_, by_, bi_, ax_, ay_, ai_))):
+                R = O.ref_bem_block(geo[0], geo[1], geo[2], sk, geo[3], geo[4], geo[5], tk, nunk)
+                key = f"bp_{tag}_{nunk}_{sk}{tk}"
+                d[key + "_A"], d[key + "_uu"], d[key + "_uv"] = R["A"], R["urot_u"], R["urot_v"]
+
     # VTU files written by the reference's own writers (src/Points.h:556-740,
     # src/Surfaces.h:1238-1379, src/VtkXmlWriter.h), kept byte for byte under tests/golden/vtk/
     r4 = np.random.default_rng(20241204)

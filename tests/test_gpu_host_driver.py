@@ -220,3 +220,98 @@ def test_find_vort_self(tmp_path):
     ref = O.points_vorticity(x, y, s, r, x, y) * (0.5 / np.pi)
     assert rel_l2(got, ref) <= 1e-12
 
+
+class MultiBem:
+    """solve_bem + find_vels for several reactive bodies (src/BEMHelper.h:43-260), restated on
+    the oracle: self blocks, blocks between bodies (with the source body's unit-rotation
+    velocity on the target's collocation points), one augmented row and column per body."""
+
+    def __init__(self, bodies, nunk, core, fs):
+        self.b = []
+        for nx, ny, idx in bodies:
+            nx, ny = nx.astype(np.float64), ny.astype(np.float64)
+            tx, ty, nnx, nny, area = O.panel_geometry(nx, ny, idx)
+            vol, cx, cy = O.body_volume(nx, ny, idx)
+            rvs, rss = O.unit_rot_strengths(nx, ny, idx, cx, cy)
+            a, b = idx[0::2], idx[1::2]
+            mx = (0.5 * nx[b] + 0.5 * nx[a] + 1e-4 * nnx).astype(np.float32)
+            my = (0.5 * ny[b] + 0.5 * ny[a] + 1e-4 * nny).astype(np.float32)
+            self.b.append(dict(nx=nx, ny=ny, idx=idx, tx=tx, ty=ty, vol=vol, rvs=rvs, rss=rss,
+                               mx=mx, my=my, np=len(idx) // 2))
+        self.nunk, self.core, self.fs = nunk, core, fs
+
+    def rows(self, k):
+        return self.b[k]["np"] * self.nunk + 1
+
+    def solve(self, wx, wy, ws):
+        n = sum(self.rows(k) for k in range(len(self.b)))
+        A = np.zeros((n, n))
+        rhs = np.zeros(n)
+        r0 = np.cumsum([0] + [self.rows(k) for k in range(len(self.b))])
+        for t, T in enumerate(self.b):
+            pu, pv = O.points_on_panels(wx, wy, ws, T["nx"], T["ny"], T["idx"])
+            pu, pv = O.finalize(pu, pv, self.fs)
+            rhs[r0[t]:r0[t + 1] - 1] = O.vels_to_rhs(pu, pv, T["tx"], T["ty"], self.nunk,
+                                                     np.zeros(T["np"]))
+            for s_, S in enumerate(self.b):
+                uu, uv = O.panels_on_points(S["nx"], S["ny"], S["idx"], S["rvs"], S["rss"],
+                                            T["mx"], T["my"])
+                uu, uv = O.finalize(uu, uv)
+                if s_ == t:
+                    blk = O.bem_assemble(T["nx"], T["ny"], T["idx"], self.nunk, True, uu, uv,
+                                         T["vol"])
+                else:
+                    blk = O.bem_assemble_pair(S["nx"], S["ny"], S["idx"], T["nx"], T["ny"],
+                                              T["idx"], self.nunk, True, True, uu, uv)
+                A[r0[t]:r0[t + 1], r0[s_]:r0[s_ + 1]] = blk
+        sol = np.linalg.solve(A, rhs)
+        for k, B in enumerate(self.b):
+            part = sol[r0[k]:r0[k + 1]]
+            B["omega"] = part[-1]
+            if self.nunk == 2:
+                B["vs"], B["ss"] = part[:-1][0::2].copy(), part[:-1][1::2].copy()
+            else:
+                B["vs"], B["ss"] = part[:-1].copy(), np.zeros(B["np"])
+
+    def wake_vel(self, wx, wy, ws, wr):
+        u, v = O.points_on_points(self.core, wx, wy, ws, wr, wx, wy, wr)
+        for B in self.b:
+            pu, pv = O.panels_on_points(B["nx"], B["ny"], B["idx"], B["vs"] + B["omega"] * B["rvs"],
+                                        B["ss"] + B["omega"] * B["rss"], wx, wy)
+            u, v = u + pu, v + pv
+        return O.finalize(u, v, self.fs)
+
+
+@pytest.mark.parametrize("nunk", [1, 2])
+def test_bem_two_bodies_and_wake_velocity(tmp_path, nunk):
+    """A ground circle and a second cylinder: the host mirror's solve_bem assembles the four
+    blocks (two self, two between the bodies) and solves; strengths, rotation rates and the
+    wake velocities against the oracle restatement."""
+    ax, ay, ai = circle_panels(96)
+    bx, by, bi = circle_panels(48, diam=0.6, cx=1.6, cy=0.3)
+    rng = np.random.default_rng(21)
+    nw = 800
+    r = np.sqrt(rng.uniform(0.6 ** 2, 2.5 ** 2, nw))
+    th = rng.uniform(0, 2 * np.pi, nw)
+    wx, wy = r * np.cos(th), r * np.sin(th)
+    far = np.hypot(wx - 1.6, wy - 0.3) > 0.4
+    wx, wy = wx[far], wy[far]
+    nw = len(wx)
+    ws, wr = rng.normal(0, 2e-4, nw), np.full(nw, 0.005)
+    header = [len(ai) // 2, len(ax), len(bi) // 2, len(bx), nw, nunk, CORE_ID["v2"]]
+    payload = [np.array([1.0, 0.0]), ax, ay, ai, bx, by, bi, wx, wy, ws, wr]
+    na, nb = len(ai) // 2, len(bi) // 2
+    got = run_driver(tmp_path, "bem2", header, payload, 2 * na + 1 + 2 * nb + 1 + 2 * nw)
+    ref = MultiBem([(ax, ay, ai), (bx, by, bi)], nunk, "v2", (1.0, 0.0))
+    ref.solve(wx, wy, ws)
+    o = 0
+    for B in ref.b:
+        gvs, gss, gom = got[o:o + B["np"]], got[o + B["np"]:o + 2 * B["np"]], got[o + 2 * B["np"]]
+        o += 2 * B["np"] + 1
+        assert rel_l2(gvs, B["vs"]) <= 1e-10
+        if nunk == 2:
+            assert rel_l2(gss, B["ss"]) <= 1e-10
+        assert abs(gom - B["omega"]) <= 1e-10
+    eu, ev = ref.wake_vel(wx, wy, ws, wr)
+    assert rel_l2(got[o:o + nw], eu) <= 1e-10 and rel_l2(got[o + nw:], ev) <= 1e-10
+

@@ -56,6 +56,22 @@ def test_bem_assemble(nunk, augmented, npan):
 
 
 @pytest.mark.parametrize("nunk", [1, 2])
+@pytest.mark.parametrize("src_aug,targ_aug", [(True, True), (True, False), (False, True),
+                                              (False, False)])
+def test_bem_block_between_bodies(nunk, src_aug, targ_aug):
+    ax, ay, ai = circle_panels(300)
+    bx, by, bi = circle_panels(129, diam=0.6, cx=1.6, cy=0.3)
+    ax, ay, bx, by = (v.astype(np.float64) for v in (ax, ay, bx, by))
+    rng = np.random.default_rng(nunk)
+    uu, uv = rng.normal(size=129), rng.normal(size=129)
+    got = o2d.bem_assemble_pair(ax, ay, ai, bx, by, bi, nunk, src_aug, targ_aug, uu, uv)
+    ref = O.bem_assemble_pair(ax, ay, ai, bx, by, bi, nunk, src_aug, targ_aug, uu, uv)
+    assert got.shape == ref.shape == (129 * nunk + targ_aug, 300 * nunk + src_aug)
+    assert rel_l2(got, ref) <= 1e-12
+    assert np.max(np.abs(got - ref)) <= 1e-13
+
+
+@pytest.mark.parametrize("nunk", [1, 2])
 def test_bem_factor_solve_matches_lapack(nunk):
     npan = 200
     nx, ny, idx = circle_panels(npan)

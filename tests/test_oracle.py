@@ -91,6 +91,23 @@ def test_bem_assemble_golden(nunk, kind):
     assert np.max(np.abs(A - Aref)) <= 1e-15
 
 
+@pytest.mark.parametrize("nunk", [1, 2])
+@pytest.mark.parametrize("kinds", ["22", "20", "02", "00"])
+@pytest.mark.parametrize("tag", ["ab", "ba"])
+def test_bem_block_between_bodies_golden(nunk, kinds, tag):
+    """Block of one body on another (src/Coefficients.h with &src != &targ), augmented or not."""
+    g = GOLD
+    a = (g["bp_ax"].astype(np.float64), g["bp_ay"].astype(np.float64), g["bp_ai"])
+    b = (g["bp_bx"].astype(np.float64), g["bp_by"].astype(np.float64), g["bp_bi"])
+    src, targ = (a, b) if tag == "ab" else (b, a)
+    key = f"bp_{tag}_{nunk}_{kinds}"
+    ref = g[key + "_A"]
+    got = O.bem_assemble_pair(*src, *targ, nunk, kinds[0] == "2", kinds[1] == "2",
+                              g[key + "_uu"], g[key + "_uv"])
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref))
+
+
 def test_bem_internal_flow_not_augmented():
     # reversed panel order bounds internal flow: negative volume, no augmentation for ground
     g = GOLD
