@@ -53,7 +53,7 @@ extern "C" {
 /* precision forms */
 #define O2D_PLAIN_FP64 1 /* fp64 pair arithmetic and accumulation */
 #define O2D_SPLIT_FP32 2 /* fp32 pair arithmetic on coordinates rebased to each source tile's
-                            fp64 centroid, fp64 accumulation per target (velocity only) */
+                            fp64 centroid, fp64 accumulation per target; all result types */
 
 /* execution environment: the gpu_hip counterpart of ExecEnv (src/ExecEnv.h:45-108) plus the
  * run-time choices the reference makes at compile time */

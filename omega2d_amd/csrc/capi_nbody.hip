@@ -55,8 +55,6 @@ static int check_pp(const o2d_env* env, int64_t ns, const void* sx, const void* 
     return fail(O2D_ERR_ARG, "results=VELANDVORT needs tw");
   if (nt > 0 && env->results == O2D_VELANDSHEAR && !te)
     return fail(O2D_ERR_ARG, "results=VELANDSHEAR needs te");
-  if (env->precision == O2D_SPLIT_FP32 && env->results != O2D_VELONLY)
-    return fail(O2D_ERR_RESULTS, "the split fp32 form computes velocity only");
   return 0;
 }
 

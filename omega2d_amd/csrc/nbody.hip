@@ -150,11 +150,12 @@ struct alignas(16) GroupInfo {
 };
 
 // pair loop over one group; CHECKED = false drops the Gaussian's in-core test (the caller has
-// shown that every pair of this group is beyond the compact cutoff, where K = 1/d^2)
-template <int CORE, bool TR, int TPL, int UNROLL, bool CHECKED>
+// shown that every pair of this group is beyond the compact cutoff, where K = 1/d^2). RES adds
+// the vorticity or shear sum into fw (factors from core_kw32).
+template <int CORE, int RES, bool TR, int TPL, int UNROLL, bool CHECKED>
 __device__ __forceinline__ void group_pairs(const float4* __restrict__ q0, const float* ex,
                                             const float* ey, const float* pw, float* fu,
-                                            float* fv) {
+                                            float* fv, float* fw) {
 #pragma unroll UNROLL
   for (int k = 0; k < kGroup; ++k) {
     const float4 q = q0[k];
@@ -165,9 +166,16 @@ __device__ __forceinline__ void group_pairs(const float4* __restrict__ q0, const
       float kk;
       if constexpr (CORE == O2D_CORE_GAUSSIAN && !CHECKED) {
         kk = __builtin_amdgcn_rcpf(__builtin_fmaf(dx, dx, dy * dy));
-      } else {
+        // beyond the cutoff the shear factor is K and the vorticity factor is dropped
+        if constexpr (RES == O2D_VELANDSHEAR) fw[t] += __builtin_fabsf(q.z) * kk;
+      } else if constexpr (RES == O2D_VELONLY) {
         const float w = TR ? q.w + pw[t] : q.w;
         kk = core_k32<CORE>(dx, dy, w);
+      } else {
+        const float w = TR ? q.w + pw[t] : q.w;
+        float aux;
+        kk = core_kw32<CORE, RES>(dx, dy, w, aux);
+        fw[t] = __builtin_fmaf(RES == O2D_VELANDVORT ? q.z : __builtin_fabsf(q.z), aux, fw[t]);
       }
       const float gk = q.z * kk;
       fu[t] = __builtin_fmaf(-gk, dy, fu[t]);
@@ -178,20 +186,21 @@ __device__ __forceinline__ void group_pairs(const float4* __restrict__ q0, const
 
 // sources prepared by the split prep: fp32 (x - c, y - c, strength, radius term) in Morton
 // order, padded to whole tiles; grp[g] describes group g
-template <int CORE, bool TR, int TPL, int UNROLL>
+template <int CORE, int RES, bool TR, int TPL, int UNROLL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(O2D_WAVES32, 8)))
 void nbody32_kernel(
     const float4* __restrict__ src, const GroupInfo* __restrict__ grp, int64_t ntiles,
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty,
     const double* __restrict__ tr, int64_t tiles_per_chunk, double* __restrict__ ou,
-    double* __restrict__ ov, int direct) {
+    double* __restrict__ ov, double* __restrict__ ow, int direct) {
   constexpr int kGroups = kTile / kGroup;
+  constexpr bool kAux = RES != O2D_VELONLY;
   __shared__ float4 tile[kTile];
   __shared__ GroupInfo tgrp[kGroups];
   const int tid = threadIdx.x;
   const int64_t tbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
 
-  double px[TPL], py[TPL], au[TPL], av[TPL];
+  double px[TPL], py[TPL], au[TPL], av[TPL], aw[TPL];
   float pw[TPL];
 #pragma unroll
   for (int t = 0; t < TPL; ++t) {
@@ -200,7 +209,7 @@ void nbody32_kernel(
     px[t] = tx[ii];
     py[t] = ty[ii];
     pw[t] = TR ? (float)rterm<CORE>(tr[ii]) : 0.0f;
-    au[t] = av[t] = 0.0;
+    au[t] = av[t] = aw[t] = 0.0;
   }
 
   const int64_t tb = (int64_t)blockIdx.y * tiles_per_chunk;
@@ -214,13 +223,14 @@ void nbody32_kernel(
 #pragma unroll 1
     for (int g = 0; g < kGroups; ++g) {
       const GroupInfo cg = tgrp[g];
-      float ex[TPL], ey[TPL], fu[TPL], fv[TPL];
+      float ex[TPL], ey[TPL], fu[TPL], fv[TPL], fw[TPL];
 #pragma unroll
       for (int t = 0; t < TPL; ++t) {
         ex[t] = (float)(px[t] - cg.cx);
         ey[t] = (float)(py[t] - cg.cy);
         fu[t] = 0.0f;
         fv[t] = 0.0f;
+        fw[t] = 0.0f;
       }
       if constexpr (CORE == O2D_CORE_GAUSSIAN) {
         // every pair of the group is outside the compact core (d^2 > 16 w) when the target's
@@ -234,16 +244,17 @@ void nbody32_kernel(
           far = far && (e2 * (1.0f - 1e-5f) > lim * lim * (1.0f + 1e-5f));
         }
         if (__all(far))
-          group_pairs<CORE, TR, TPL, 16, false>(tile + g * kGroup, ex, ey, pw, fu, fv);
+          group_pairs<CORE, RES, TR, TPL, 16, false>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
         else
-          group_pairs<CORE, TR, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv);
+          group_pairs<CORE, RES, TR, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
       } else {
-        group_pairs<CORE, TR, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv);
+        group_pairs<CORE, RES, TR, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
       }
 #pragma unroll
       for (int t = 0; t < TPL; ++t) {
         au[t] += (double)fu[t];
         av[t] += (double)fv[t];
+        if (kAux) aw[t] += (double)fw[t];
       }
     }
   }
@@ -255,10 +266,12 @@ void nbody32_kernel(
     if (direct) {
       ou[i] += au[t];
       ov[i] += av[t];
+      if (kAux) ow[i] += aw[t];
     } else {
       const int64_t o = (int64_t)blockIdx.y * nt + i;
       ou[o] = au[t];
       ov[o] = av[t];
+      if (kAux) ow[o] = aw[t];
     }
   }
 }
@@ -520,17 +533,33 @@ void dispatch64(int res, bool tr_present, dim3 grid, hipStream_t s, int64_t ns, 
 #undef O2D_L64
 }
 
-template <int CORE>
-void dispatch32(bool tr_present, dim3 grid, hipStream_t s, const float4* src, const GroupInfo* grp,
-                int64_t ntiles, int64_t nt, const double* tx, const double* ty, const double* tr,
-                int64_t per, double* ou, double* ov, int direct) {
+template <int CORE, int RES>
+void launch32(bool tr_present, dim3 grid, hipStream_t s, const float4* src, const GroupInfo* grp,
+              int64_t ntiles, int64_t nt, const double* tx, const double* ty, const double* tr,
+              int64_t per, double* ou, double* ov, double* ow, int direct) {
   constexpr int kUnroll = 16;
   if (tr_present)
-    nbody32_kernel<CORE, true, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
-        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
+    nbody32_kernel<CORE, RES, true, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
+        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, ow, direct);
   else
-    nbody32_kernel<CORE, false, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
-        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, direct);
+    nbody32_kernel<CORE, RES, false, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
+        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, ow, direct);
+}
+
+template <int CORE>
+void dispatch32(int res, bool tr_present, dim3 grid, hipStream_t s, const float4* src,
+                const GroupInfo* grp, int64_t ntiles, int64_t nt, const double* tx,
+                const double* ty, const double* tr, int64_t per, double* ou, double* ov,
+                double* ow, int direct) {
+  if (res == O2D_VELONLY)
+    launch32<CORE, O2D_VELONLY>(tr_present, grid, s, src, grp, ntiles, nt, tx, ty, tr, per, ou,
+                                ov, ow, direct);
+  else if (res == O2D_VELANDVORT)
+    launch32<CORE, O2D_VELANDVORT>(tr_present, grid, s, src, grp, ntiles, nt, tx, ty, tr, per,
+                                   ou, ov, ow, direct);
+  else
+    launch32<CORE, O2D_VELANDSHEAR>(tr_present, grid, s, src, grp, ntiles, nt, tx, ty, tr, per,
+                                    ou, ov, ow, direct);
 }
 
 template <int CORE>
@@ -618,9 +647,7 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
     return 0;
   }
 
-  // split form: velocity only
-  if (res != O2D_VELONLY)
-    return fail(O2D_ERR_RESULTS, "the split fp32 form computes velocity only (results=%d)", res);
+  // split form
   if (ns > 0x7fffffffLL) return fail(O2D_ERR_SIZE, "too many sources for the split form");
   const int64_t ntiles = (ns + kTile - 1) / kTile;
   const int64_t npad = ntiles * kTile;
@@ -649,26 +676,28 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);
   ChunkPlan cp = plan_chunks(d, twgs, ntiles);
   const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
-  double *ou = tu, *ov = tv;
+  double *ou = tu, *ov = tv, *ow = tw;
   const int direct = cp.nchunk == 1;
   if (!direct) {
     void* p;
-    O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * cp.nchunk * nt, s, &p));
+    const int nf = res == O2D_VELONLY ? 2 : 3;
+    O2D_TRY(slot(d, kPartials, sizeof(double) * nf * cp.nchunk * nt, s, &p));
     ou = static_cast<double*>(p);
     ov = ou + cp.nchunk * nt;
+    ow = res == O2D_VELONLY ? nullptr : ov + cp.nchunk * nt;
   }
   KernelTimer kt(trp ? O2D_KT_NBODY_SPLIT : O2D_KT_NBODY_SPLIT_PT, s);
   switch (core) {
-    case O2D_CORE_RM: dispatch32<O2D_CORE_RM>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
-    case O2D_CORE_GAUSSIAN: dispatch32<O2D_CORE_GAUSSIAN>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
-    case O2D_CORE_WL: dispatch32<O2D_CORE_WL>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
-    case O2D_CORE_V2: dispatch32<O2D_CORE_V2>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
-    default: dispatch32<O2D_CORE_V3>(trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, direct); break;
+    case O2D_CORE_RM: dispatch32<O2D_CORE_RM>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_GAUSSIAN: dispatch32<O2D_CORE_GAUSSIAN>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_WL: dispatch32<O2D_CORE_WL>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_V2: dispatch32<O2D_CORE_V2>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
+    default: dispatch32<O2D_CORE_V3>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
   }
   kt.stop();
   O2D_LAUNCH_CHECK();
   if (!direct) {
-    reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, ou, ov, nullptr, tu, tv, nullptr);
+    reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, ou, ov, ow, tu, tv, tw);
     O2D_LAUNCH_CHECK();
   }
   return 0;

@@ -211,4 +211,94 @@ __device__ __forceinline__ float core_k32(float dx, float dy, float w) {
   }
 }
 
+// ---- fp32 split-precision vorticity and shear factors --------------------------------------
+//
+// With bbb = 2 dK/d(d^2), the reference's per-pair vorticity (bbb dx^2 + K) - (-bbb dy^2 - K)
+// (src/Kernels.h:139-179) is bbb d^2 + 2K, and its per-pair shear
+// sqrt((bbb dx dy)^2 + (bbb (dx^2 - dy^2))^2 / 4) (src/Kernels.h:185-231) is |bbb| d^2 / 2, since
+// dx^2 dy^2 + (dx^2 - dy^2)^2 / 4 = d^4 / 4. Each core's pair of factors is written here in a
+// form free of cancellation:
+//   RM  K = 1/(d^2+w):           vort 2 w K^2,          shear d^2 K^2
+//   WL  K = (dd+w)/dd^2, dd=d^2+w: vort 4 w^2 / dd^3,   shear (d^2 + 3w) d^2 / dd^3
+//   V2  K = (d^4+w)^-1/2:        vort 2 w K^3,          shear d^4 K^3
+//   V3  K = (d^6+w)^-1/3:        vort 2 w K^4,          shear d^6 K^4
+//   Gaussian, x = d^2/w:         vort 2 e^-x / w,       shear |(1+x) e^-x - 1| / (x w)
+//     (for x < 0.001 the reference's K is the constant 1/w, which makes its shear (1 - e^-x)/w)
+// Beyond the Gaussian's compact cutoff (x > 16) K = 1/d^2, the shear factor is K and the
+// vorticity factor 2 e^-x / w is below 2.3e-7 of the in-core value 2/w; the split form drops it
+// (the reference's own fp32 pair arithmetic rounds the same term at that level).
+// Returns K; aux receives the vorticity (RES == O2D_VELANDVORT) or shear factor, to be scaled by
+// the source strength (shear by its magnitude).
+
+// (1+x) e^-x - 1 = -x^2 (1/2 - x/3 + x^2/8 - x^3/30 + ...) for small x, as |.| / x
+__device__ __forceinline__ float gauss_shear_series(float x) {
+  // coefficients (m - 1) / m! of x^(m-2), m = 2..12, alternating; truncation below 1e-8
+  // relative for x < 1
+  float p = __builtin_fmaf(x, 11.0f / 479001600.0f, -10.0f / 39916800.0f);
+  p = __builtin_fmaf(x, p, 9.0f / 3628800.0f);
+  p = __builtin_fmaf(x, p, -8.0f / 362880.0f);
+  p = __builtin_fmaf(x, p, 7.0f / 40320.0f);
+  p = __builtin_fmaf(x, p, -6.0f / 5040.0f);
+  p = __builtin_fmaf(x, p, 5.0f / 720.0f);
+  p = __builtin_fmaf(x, p, -4.0f / 120.0f);
+  p = __builtin_fmaf(x, p, 3.0f / 24.0f);
+  p = __builtin_fmaf(x, p, -2.0f / 6.0f);
+  p = __builtin_fmaf(x, p, 0.5f);
+  return x * p;
+}
+
+template <int CORE, int RES>
+__device__ __forceinline__ float core_kw32(float dx, float dy, float w, float& aux) {
+  const float d2 = __builtin_fmaf(dx, dx, dy * dy);
+  if constexpr (CORE == O2D_CORE_RM) {
+    const float k = __builtin_amdgcn_rcpf(d2 + w);
+    aux = (RES == O2D_VELANDVORT ? 2.0f * w : d2) * k * k;
+    return k;
+  } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+    const float rd2 = __builtin_amdgcn_rcpf(d2);
+    if (d2 > 16.0f * w) {
+      aux = RES == O2D_VELANDVORT ? 0.0f : rd2;
+      return rd2;
+    }
+    const float cf = __builtin_amdgcn_rcpf(w);
+    const float x = d2 * cf;
+    const float e = __builtin_amdgcn_exp2f(-x * 1.4426950408889634f);
+    float k;
+    if (x < 0.001f) {
+      k = cf;
+      // 1 - e^-x = x (1 - x/2) to fp32 precision here
+      aux = RES == O2D_VELANDVORT ? 2.0f * cf * e : cf * x * __builtin_fmaf(-0.5f, x, 1.0f);
+    } else {
+      k = one_minus_expm(x) * rd2;
+      if constexpr (RES == O2D_VELANDVORT)
+        aux = 2.0f * cf * e;
+      else
+        aux = cf * (x < 1.0f ? gauss_shear_series(x)
+                             : __builtin_fabsf(__builtin_fmaf(1.0f + x, e, -1.0f)) *
+                                   __builtin_amdgcn_rcpf(x));
+    }
+    return k;
+  } else if constexpr (CORE == O2D_CORE_WL) {
+    const float dd = d2 + w;
+    const float r = __builtin_amdgcn_rcpf(dd);
+    const float r3 = r * r * r;
+    aux = (RES == O2D_VELANDVORT ? 4.0f * w * w : __builtin_fmaf(3.0f, w, d2) * d2) * r3;
+    return (dd + w) * r * r;
+  } else if constexpr (CORE == O2D_CORE_V2) {
+    const float k = __builtin_amdgcn_rsqf(__builtin_fmaf(d2, d2, w));
+    aux = (RES == O2D_VELANDVORT ? 2.0f * w : d2 * d2) * k * k * k;
+    return k;
+  } else {
+    // the exp2/log2 estimate of v^-1/3 carries the log's error times |log2 v| / 3; one Newton
+    // step y (4 - v y^3) / 3 brings it to fp32 rounding before the fourth power
+    const float d6 = d2 * d2 * d2;
+    const float v = d6 + w;
+    const float k0 = __builtin_amdgcn_exp2f(-0.33333333333f * __builtin_amdgcn_logf(v));
+    const float k = k0 * __builtin_fmaf(-v, k0 * k0 * k0, 4.0f) * 0.33333334f;
+    const float k2 = k * k;
+    aux = (RES == O2D_VELANDVORT ? 2.0f * w : d6) * k2 * k2;
+    return k;
+  }
+}
+
 }  // namespace o2d

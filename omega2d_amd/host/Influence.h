@@ -108,8 +108,10 @@ template <class S, class A>
 void panels_affect_points(const Surfaces<S>& src, Points<S>& targ, const ResultsType& restype,
                           const ExecEnv& env) {
   require_gpu_hip(env);
-  if (!restype.compute_vel() || restype.get_type() != velonly)
-    throw std::runtime_error("surfaces compute velocity only");
+  // src/Influence.h:427-428: only streamfunction and gradients are refused; for the other
+  // result types panels add velocity and leave vorticity and shear untouched
+  if (restype.compute_psi() || restype.compute_grad())
+    throw std::runtime_error("surfaces cannot compute streamfunction or velocity gradients");
   const auto start = Clock::now();
   const o2d_env e = env.c_env(velonly);
   const auto nx = as_double(src.get_pos()[0]), ny = as_double(src.get_pos()[1]);
@@ -140,7 +142,9 @@ template <class S, class A>
 void points_affect_panels(const Points<S>& src, Surfaces<S>& targ, const ResultsType& restype,
                           const ExecEnv& env) {
   require_gpu_hip(env);
-  if (restype.get_type() != velonly) throw std::runtime_error("panels receive velocity only");
+  // src/Influence.h:638-639, as for panels_affect_points: velocity for every other result type
+  if (restype.compute_psi() || restype.compute_grad())
+    throw std::runtime_error("points cannot compute streamfunction or velocity gradients");
   const auto start = Clock::now();
   const o2d_env e = env.c_env(velonly);
   const auto sx = as_double(src.get_pos()[0]), sy = as_double(src.get_pos()[1]);

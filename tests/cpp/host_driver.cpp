@@ -122,6 +122,17 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
     const auto& p = std::get<Points<double>>(vort[0]);
     out.vec(p.get_vel()[0]);
     out.vec(p.get_vel()[1]);
+  } else if (mode == "bemvort") {
+    // velocity and vorticity of the flow after the solve (src/Simulation.cpp:412), the wake
+    // summed in the split form
+    env.set_precision(O2D_SPLIT_FP32);
+    Convection<double, double, Int> sconv(env, (int)order);
+    sconv.find_derivs(0.0, fs, bem, bdry, vort, fldpt);
+    sconv.find_vels(fs, vort, bdry, vort, velandvort, true);
+    auto& p = std::get<Points<double>>(vort[0]);
+    out.vec(p.get_vel()[0]);
+    out.vec(p.get_vel()[1]);
+    out.vec(p.get_vort());
   } else {
     conv.advect(0.0, dt, fs, ips, vort, bdry, fldpt, bem);
     const auto& p = std::get<Points<double>>(vort[0]);
@@ -214,7 +225,7 @@ void run_vtk(Reader& in, const std::string& dir) {
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bem2|vort|vtk> <in.bin> <out.bin|dir>\n";
+    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bem2|vort|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
@@ -231,7 +242,7 @@ int main(int argc, char** argv) {
       const summation_t summ = mode == "rk" ? direct : barneshut;
       if (storage == 1) run_rk<float>(in, out, n, (int)order, (int)core, (int)prec, summ);
       else run_rk<double>(in, out, n, (int)order, (int)core, (int)prec, summ);
-    } else if (mode == "bem" || mode == "bemstep") {
+    } else if (mode == "bem" || mode == "bemstep" || mode == "bemvort") {
       run_bem(in, out, mode);
     } else if (mode == "vort") {
       run_vort(in, out, in.i64());

@@ -65,11 +65,10 @@ def test_argument_errors_are_reported_not_raised():
     assert _pp(ctypes.byref(ok), tu=None) == -1
     # vorticity requested without its output array
     assert _pp(ctypes.byref(_cabi.make_env("rm", "fp64", "vort"))) == -1
-    # the split form computes velocity only
-    assert _pp(ctypes.byref(_cabi.make_env("rm", "split", "vort")),
-               tw=np.ones(4)) == -4
+    # the split form also needs the vorticity array
+    assert _pp(ctypes.byref(_cabi.make_env("rm", "split", "vort"))) == -1
     msg = _cabi.lib().o2d_hip_last_error_string().decode()
-    assert "velocity only" in msg
+    assert "needs tw" in msg
 
 
 def test_panel_index_out_of_range_rejected_on_host():
@@ -121,8 +120,8 @@ def test_python_binding_raises_on_error():
     import omega2d_amd as o2d
     with pytest.raises(o2d.O2DError) as ei:
         o2d.points_on_points(np.ones(2), np.ones(2), np.ones(2), np.ones(2), np.ones(2),
-                             np.ones(2), core="rm", precision="split", results="vort")
-    assert ei.value.code == -4
+                             np.ones(2), core="rm", precision="split", results=3)
+    assert ei.value.code == -4  # streamfunction is not a result of this path
     with pytest.raises(TypeError):
         o2d.points_on_points(np.ones(2, np.int64), np.ones(2), np.ones(2), np.ones(2), np.ones(2),
                              np.ones(2))

@@ -178,6 +178,25 @@ def test_bem_solve_and_wake_velocity(tmp_path, nunk):
         assert np.max(np.abs(np.abs(gvs) - 2 * np.abs(np.sin(th)))) < 0.1
 
 
+def test_bem_velocity_and_vorticity_split(tmp_path):
+    """find_vels with velandvort after the solve (src/Simulation.cpp:412), the wake in the split
+    form: the panels add velocity only and leave the vorticity to the particles
+    (src/Influence.h:422-614)."""
+    w, nx, ny, idx = bem_case(2)
+    nw = len(w["x"])
+    header = [len(idx) // 2, len(nx), nw, 2, CORE_ID["v2"], 2]
+    payload = [np.array([1.0, 0.0, 0.01, 0.1]), nx, ny, idx, w["x"], w["y"], w["s"], w["r"]]
+    got = run_driver(tmp_path, "bemvort", header, payload, 3 * nw)
+    gu, gv, gw = got[:nw], got[nw:2 * nw], got[2 * nw:]
+    ref = PyBem(nx, ny, idx, 2, "v2", (1.0, 0.0))
+    ref.solve(w["x"], w["y"], w["s"])
+    eu, ev = ref.wake_vel(w["x"], w["y"], w["s"], w["r"])
+    assert rel_l2(gu, eu) <= 1e-6 and rel_l2(gv, ev) <= 1e-6
+    _, _, ew = O.points_on_points("v2", w["x"], w["y"], w["s"], w["r"], w["x"], w["y"], w["r"],
+                                  restype="vort")
+    assert rel_l2(gw, ew * (0.5 / np.pi)) <= 1e-6
+
+
 def test_bem_rk2_step_with_wake(tmp_path):
     """Ralston RK2 with the BEM solved at both stages and the inner layer cleared after each
     move (method 1, src/Convection.h:413,464), as src/Convection.h:390-466 does it."""

@@ -102,6 +102,60 @@ def test_split_fp32_error_bound(core, particle_targets):
     assert err <= 1e-6, (core, err)
 
 
+def _clustered(rng, n, sig):
+    sx = np.concatenate([0.3 + 0.2 * rng.normal(size=n // 2), rng.uniform(-2, 2, n // 2)])
+    sy = np.concatenate([-0.1 + 0.2 * rng.normal(size=n // 2), rng.uniform(-2, 2, n // 2)])
+    ss = rng.normal(0, 1e-3, n) + 2e-4
+    return sx, sy, ss, np.full(n, sig)
+
+
+@pytest.mark.parametrize("core", CORES)
+@pytest.mark.parametrize("results", ["vort", "shear"])
+@pytest.mark.parametrize("particle_targets", [False, True])
+def test_split_fp32_vorticity_and_shear(core, results, particle_targets):
+    """Split-form vorticity and shear sums (src/Kernels.h:139-231) against the fp64 oracle, at
+    the split form's 1e-6 bar; velocity alongside them at the same bar."""
+    rng = np.random.default_rng(13)
+    sx, sy, ss, sr = _clustered(rng, 12000, 0.01)
+    tx, ty = sx[::5].copy(), sy[::5].copy()
+    tr = sr[::5].copy() if particle_targets else None
+    got = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr, core=core, results=results,
+                               precision="split")
+    ref = O.points_on_points(core, sx, sy, ss, sr, tx, ty, tr, results)
+    errs = [rel_l2(g, r) for g, r in zip(got, ref)]
+    print(core, results, particle_targets, errs)
+    assert max(errs) <= 1e-6, (core, results, errs)
+
+
+@pytest.mark.parametrize("results", ["vort", "shear"])
+def test_split_fp32_gaussian_far_field_vorticity_and_shear(results):
+    """Gaussian cores far smaller than the spacing (most groups take the far path, where the
+    shear factor is 1/d^2 and the vorticity factor, below e^-16, is dropped); also chunked."""
+    rng = np.random.default_rng(17)
+    sx, sy, ss, sr = _clustered(rng, 40000, 2e-3)
+    for tx, ty, tr in ((sx[::9].copy(), sy[::9].copy(), sr[::9].copy()),
+                       (sx[:50].copy(), sy[:50].copy(), None)):
+        got = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr, core="gauss", results=results,
+                                   precision="split")
+        ref = O.points_on_points("gauss", sx, sy, ss, sr, tx, ty, tr, results)
+        errs = [rel_l2(g, r) for g, r in zip(got, ref)]
+        print(results, len(tx), errs)
+        assert max(errs) <= 1e-6, (results, errs)
+
+
+def test_split_fp32_self_vorticity_is_core_value():
+    """One particle on itself: velocity 0, vorticity 2/w of its own core (RM: w = 2 sigma^2 for
+    a particle target), shear 0."""
+    x, y, s, r = (np.array([0.3]), np.array([-0.2]), np.array([2.0]), np.array([0.05]))
+    u, v, w = o2d.points_on_points(x, y, s, r, x, y, r, core="rm", results="vort",
+                                   precision="split")
+    assert u[0] == 0.0 and v[0] == 0.0
+    assert abs(w[0] - 2.0 * 2.0 / (2 * 0.05 ** 2)) <= 1e-6 * abs(w[0])
+    _, _, e = o2d.points_on_points(x, y, s, r, x, y, r, core="gauss", results="shear",
+                                   precision="split")
+    assert e[0] == 0.0
+
+
 def test_float_storage_entry_point():
     rng = np.random.default_rng(5)
     sx, sy, ss, sr = (a.astype(np.float32) for a in particles(rng, 2000))
