@@ -18,6 +18,7 @@
 #include "nbody.h"
 #include "o2d_math.h"
 #include "o2d_runtime.h"
+#include "panel_math.h"
 
 namespace o2d {
 
@@ -56,20 +57,11 @@ __global__ __launch_bounds__(kBlock) void geom_kernel(int64_t np, const double* 
 }
 
 // unit-strength vortex panel (a along the panel rotation, b across): the velocity of
-// kernelu_1v_0p / kernelu_1vos_0p is (a, b) for the vortex part and (b, -a) for the source part
+// kernelu_1v_0p / kernelu_1vos_0p is (a, b) for the vortex part and (b, -a) for the source part;
+// far pairs take the atanh series of panel_math.h
 __device__ __forceinline__ void unit_panel(const Geom& s, double tx, double ty, double& a,
                                            double& b) {
-  const double dx0 = tx - s.x0, dy0 = ty - s.y0;
-  const double dx1 = tx - s.x1, dy1 = ty - s.y1;
-  const double px = s.x1 - s.x0, py = s.y1 - s.y0;
-  const double rij2 = __builtin_fma(dx0, dx0, dy0 * dy0);
-  const double rij12 = __builtin_fma(dx1, dx1, dy1 * dy1);
-  const double mult = rsq64(__builtin_fma(px, px, py * py));
-  const double us = atan2(__builtin_fma(py, dx0, -px * dy0), __builtin_fma(dy1, dy0, dx1 * dx0));
-  const double vstar = 0.5 * log(rij2 * rcp64(rij12));
-  const double tpx = px * mult, tpy = py * mult;
-  a = us * tpx - vstar * tpy;
-  b = us * tpy + vstar * tpx;
+  unit_panel_vel(s.x0, s.y0, s.x1, s.y1, tx, ty, a, b);
 }
 
 // one thread per (target panel i, source panel j); SELF: source and target are one surface,
