@@ -65,89 +65,108 @@ __global__ __launch_bounds__(kBlock) void tree_gather_kernel(
   }
 }
 
-// leaves: bounding box, centre, radius, largest radius term and moments of <= kLeaf sources
+// wave-wide reductions (every lane receives the result)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+  return v;
+}
+
+// leaves, one wave each (lane j holds source j of the leaf): bounding box, centre, radius,
+// largest radius term and the moments M_k = sum_j s_j (z_j - c)^k of <= kLeaf sources, each
+// moment a wave sum; lane k keeps M_k
 __global__ __launch_bounds__(kBlock) void tree_leaf_kernel(
     int64_t ns, int64_t nleaf, int p, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ s, const double* __restrict__ w, Cell* __restrict__ cell,
     double2* __restrict__ mom) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c >= nleaf) return;
-  const int64_t j0 = c * kLeaf, j1 = (j0 + kLeaf < ns) ? j0 + kLeaf : ns;
-  double x0 = x[j0], x1 = x[j0], y0 = y[j0], y1 = y[j0], wm = 0.0;
-  for (int64_t j = j0; j < j1; ++j) {
-    x0 = fmin(x0, x[j]);
-    x1 = fmax(x1, x[j]);
-    y0 = fmin(y0, y[j]);
-    y1 = fmax(y1, y[j]);
-    wm = fmax(wm, w[j]);
-  }
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  if (c >= nleaf) return;  // whole waves leave together
+  const int64_t j = c * kLeaf + lane;
+  const bool in = j < ns;
+  const double xj = in ? x[j] : 0.0, yj = in ? y[j] : 0.0;
+  const double x0 = wave_min(in ? xj : INFINITY), x1 = wave_max(in ? xj : -INFINITY);
+  const double y0 = wave_min(in ? yj : INFINITY), y1 = wave_max(in ? yj : -INFINITY);
+  const double wm = wave_max(in ? w[j] : 0.0);
   const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
-  double r2 = 0.0;
-  double2 m[kMaxP];
-  for (int k = 0; k < p; ++k) m[k] = make_double2(0.0, 0.0);
-  for (int64_t j = j0; j < j1; ++j) {
-    const double dx = x[j] - cx, dy = y[j] - cy;
-    r2 = fmax(r2, dx * dx + dy * dy);
-    double pr = s[j], pi = 0.0;  // s (z - c)^k
-    for (int k = 0; k < p; ++k) {
-      m[k].x += pr;
-      m[k].y += pi;
-      const double nr = pr * dx - pi * dy;
-      pi = pr * dy + pi * dx;
-      pr = nr;
+  const double dx = xj - cx, dy = yj - cy;
+  const double r2 = wave_max(in ? dx * dx + dy * dy : 0.0);
+  double pr = in ? s[j] : 0.0, pi = 0.0;  // s (z - c)^k
+  double mr = 0.0, mi = 0.0;
+  for (int k = 0; k < p; ++k) {
+    const double sr = wave_sum(pr), si = wave_sum(pi);
+    if (lane == k) {
+      mr = sr;
+      mi = si;
     }
+    const double nr = pr * dx - pi * dy;
+    pi = pr * dy + pi * dx;
+    pr = nr;
   }
   // rounding of the offsets: a relative margin keeps r an upper bound
-  cell[c] = Cell{cx, cy, sqrt(r2) * (1.0 + 1e-12), wm, x0, y0, x1, y1};
-  for (int k = 0; k < p; ++k) mom[c * p + k] = m[k];
+  if (lane == 0) cell[c] = Cell{cx, cy, sqrt(r2) * (1.0 + 1e-12), wm, x0, y0, x1, y1};
+  if (lane < p) mom[c * p + lane] = make_double2(mr, mi);
 }
 
-// one coarser level: bounding-box union, covering radius, and moments shifted to the new centre
-// with M'_k = sum_{l <= k} C(k, l) M_l (c_child - c)^(k - l)
+// one coarser level, one wave per cell: bounding-box union, covering radius, and moments
+// shifted to the new centre, lane k forming M'_k = sum_children sum_{l <= k} C(k, l) M_l d^(k-l)
+// with d = c_child - c (the children's moments staged in LDS)
 __global__ __launch_bounds__(kBlock) void tree_up_kernel(int64_t nchild, int64_t ncell, int p,
                                                          const Cell* __restrict__ child,
                                                          const double2* __restrict__ cmom,
                                                          Cell* __restrict__ cell,
                                                          double2* __restrict__ mom) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c >= ncell) return;
-  const int64_t k0 = c * kFan, k1 = (k0 + kFan < nchild) ? k0 + kFan : nchild;
+  __shared__ double2 sm[kBlock / 64][kFan * kMaxP];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x / 64;
+  const int64_t c = (int64_t)blockIdx.x * (kBlock / 64) + wv;
+  if (c >= ncell) return;  // whole waves leave together; no block-wide barrier below
+  const int64_t k0 = c * kFan;
+  const int nch = (int)((k0 + kFan < nchild) ? kFan : nchild - k0);
+  for (int e = lane; e < nch * p; e += 64) sm[wv][e] = cmom[k0 * p + e];
   double x0 = child[k0].x0, y0 = child[k0].y0, x1 = child[k0].x1, y1 = child[k0].y1, wm = 0.0;
-  for (int64_t k = k0; k < k1; ++k) {
-    x0 = fmin(x0, child[k].x0);
-    y0 = fmin(y0, child[k].y0);
-    x1 = fmax(x1, child[k].x1);
-    y1 = fmax(y1, child[k].y1);
-    wm = fmax(wm, child[k].wmax);
+  for (int k = 0; k < nch; ++k) {
+    x0 = fmin(x0, child[k0 + k].x0);
+    y0 = fmin(y0, child[k0 + k].y0);
+    x1 = fmax(x1, child[k0 + k].x1);
+    y1 = fmax(y1, child[k0 + k].y1);
+    wm = fmax(wm, child[k0 + k].wmax);
   }
   const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
-  double r = 0.0;
-  double2 m[kMaxP];
-  for (int k = 0; k < p; ++k) m[k] = make_double2(0.0, 0.0);
-  for (int64_t ch = k0; ch < k1; ++ch) {
-    const double dx = child[ch].cx - cx, dy = child[ch].cy - cy;
-    r = fmax(r, sqrt(dx * dx + dy * dy) + child[ch].r);
-    // powers of the shift d = c_child - c
-    double2 dp[kMaxP];
-    dp[0] = make_double2(1.0, 0.0);
-    for (int k = 1; k < p; ++k)
-      dp[k] = make_double2(dp[k - 1].x * dx - dp[k - 1].y * dy, dp[k - 1].x * dy + dp[k - 1].y * dx);
-    for (int k = 0; k < p; ++k) {
-      double binom = 1.0;  // C(k, l), built up along l
-      double ar = 0.0, ai = 0.0;
-      for (int l = 0; l <= k; ++l) {
-        const double2 ml = cmom[ch * p + l];
-        const double2 d = dp[k - l];
-        ar += binom * (ml.x * d.x - ml.y * d.y);
-        ai += binom * (ml.x * d.y + ml.y * d.x);
-        binom = binom * (double)(k - l) / (double)(l + 1);
+  // the staged moments are read by other lanes of the same wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double r = 0.0, ar = 0.0, ai = 0.0;
+  for (int ch = 0; ch < nch; ++ch) {
+    const double dx = child[k0 + ch].cx - cx, dy = child[k0 + ch].cy - cy;
+    r = fmax(r, sqrt(dx * dx + dy * dy) + child[k0 + ch].r);
+    if (lane < p) {
+      const int k = lane;
+      // l from k down to 0: C(k, l) M_l d^(k - l), with d^(k-l) and C(k, l) by recurrence
+      double dpr = 1.0, dpi = 0.0, binom = 1.0;
+      for (int l = k; l >= 0; --l) {
+        const double2 ml = sm[wv][ch * p + l];
+        ar += binom * (ml.x * dpr - ml.y * dpi);
+        ai += binom * (ml.x * dpi + ml.y * dpr);
+        const double nr = dpr * dx - dpi * dy;
+        dpi = dpr * dy + dpi * dx;
+        dpr = nr;
+        binom = binom * (double)l / (double)(k - l + 1);
       }
-      m[k].x += ar;
-      m[k].y += ai;
     }
   }
-  cell[c] = Cell{cx, cy, r * (1.0 + 1e-12), wm, x0, y0, x1, y1};
-  for (int k = 0; k < p; ++k) mom[c * p + k] = m[k];
+  if (lane == 0) cell[c] = Cell{cx, cy, r * (1.0 + 1e-12), wm, x0, y0, x1, y1};
+  if (lane < p) mom[c * p + lane] = make_double2(ar, ai);
 }
 
 // may the moments of a cell stand for its sources at a target d_min away?
@@ -298,11 +317,12 @@ int tree_dev(Device* d, hipStream_t s, int core, double theta, int p, double tol
     tree_gather_kernel<<<(unsigned)((ns + kBlock - 1) / kBlock), kBlock, 0, s>>>(
         ns, sorder, sx, sy, ss, sr, core, xs, ys, ssorted, ws);
     O2D_LAUNCH_CHECK();
-    tree_leaf_kernel<<<(unsigned)((lv.cnt[0] + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        ns, lv.cnt[0], p, xs, ys, ssorted, ws, cells, mom);
+    constexpr int kCellsPerBlock = kBlock / 64;  // one wave per cell
+    tree_leaf_kernel<<<(unsigned)((lv.cnt[0] + kCellsPerBlock - 1) / kCellsPerBlock), kBlock, 0,
+                       s>>>(ns, lv.cnt[0], p, xs, ys, ssorted, ws, cells, mom);
     O2D_LAUNCH_CHECK();
     for (int l = 1; l < lv.n; ++l) {
-      tree_up_kernel<<<(unsigned)((lv.cnt[l] + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      tree_up_kernel<<<(unsigned)((lv.cnt[l] + kCellsPerBlock - 1) / kCellsPerBlock), kBlock, 0, s>>>(
           lv.cnt[l - 1], lv.cnt[l], p, cells + lv.off[l - 1], mom + lv.off[l - 1] * p,
           cells + lv.off[l], mom + lv.off[l] * p);
       O2D_LAUNCH_CHECK();
