@@ -199,6 +199,22 @@ struct TreeArgs {
   double tol;
 };
 
+// makes an in-wave LDS write visible to the other lanes of the wave before they read it, and
+// keeps later writes behind earlier reads
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct alignas(16) LeafSrc {
+  double x, y, s, w;
+};
+
+// Every lane of a wave walks the same cells. The popped stack entry is made wave-uniform
+// (readfirstlane), so cell records come in through scalar loads; a leaf's sources and a cell's
+// moments are loaded once per wave, one element per lane (coalesced), into the wave's LDS and
+// read back as broadcasts, instead of one dependent load per pair or per coefficient.
 template <int CORE>
 __global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t nt,
                                                             const uint32_t* __restrict__ torder,
@@ -207,9 +223,14 @@ __global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t 
                                                             const double* __restrict__ tr,
                                                             double* __restrict__ tu,
                                                             double* __restrict__ tv) {
-  __shared__ unsigned stack[kTBlock / 64][kStack];
-  const int wave = threadIdx.x / 64;
+  constexpr int kWaves = kTBlock / 64;
+  __shared__ unsigned stack[kWaves][kStack];
+  __shared__ LeafSrc lsrc[kWaves][kLeaf];
+  __shared__ double2 lmom[kWaves][kMaxP];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   unsigned* st = stack[wave];
+  LeafSrc* ls = lsrc[wave];
+  double2* lm = lmom[wave];
   const int64_t i = (int64_t)blockIdx.x * kTBlock + threadIdx.x;
   const bool valid = i < nt;
   const uint32_t ti = valid ? torder[i] : 0u;
@@ -218,56 +239,86 @@ __global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t 
   double u = 0.0, v = 0.0;     // direct part
   double sr = 0.0, si = 0.0;   // S, the far part
   const double th2 = a.theta * a.theta;
+  const int p = a.p;
   // a wave with no valid lane (the tail of the grid) has nothing to do
   if (__any(valid)) {
     int sp = 0;
     const int root = a.lv.n - 1;
-    st[sp++] = (unsigned)root << 27;  // (level << 27) | index; the root is index 0
+    if (lane == 0) st[0] = (unsigned)root << 27;  // (level << 27) | index; the root is index 0
+    sp = 1;
+    wave_lds_sync();
     while (sp > 0) {
-      const unsigned e = st[--sp];
+      const unsigned e = __builtin_amdgcn_readfirstlane(st[--sp]);
       const int lev = (int)(e >> 27), idx = (int)(e & ((1u << 27) - 1));
       const int64_t ci = a.lv.off[lev] + idx;
       const Cell c = a.cell[ci];
       const double dx = x - c.cx, dy = y - c.cy;
       const double d2 = dx * dx + dy * dy;
-      bool acc = true;
-      if (valid) {
-        const double d = sqrt(d2);
-        const double dm = d - c.r;
-        acc = (c.r * c.r <= th2 * d2) && dm > 0.0 && core_far<CORE>(dm * dm, c.wmax + wt, a.tol);
-      }
+      const double dm = sqrt(d2) - c.r;  // distance to the nearest possible source of the cell
+      // every pair with this cell is beyond the core's cutoff (or within core_tol of 1/d^2)
+      const bool cfar = !valid || (dm > 0.0 && core_far<CORE>(dm * dm, c.wmax + wt, a.tol));
+      const bool acc = cfar && (!valid || c.r * c.r <= th2 * d2);
       if (__all(acc)) {
+        if (lane < p) lm[lane] = a.mom[ci * p + lane];
+        wave_lds_sync();
         if (valid) {
           // t = 1/(z - c); S += sum_k M_k t^(k+1) by Horner
           const double q = 1.0 / d2;
           const double tre = dx * q, tim = -dy * q;
-          const double2* m = a.mom + ci * a.p;
-          double hr = m[a.p - 1].x, hi = m[a.p - 1].y;
-          for (int k = a.p - 2; k >= 0; --k) {
-            const double nr = hr * tre - hi * tim + m[k].x;
-            hi = hr * tim + hi * tre + m[k].y;
+          double hr = lm[p - 1].x, hi = lm[p - 1].y;
+#pragma unroll 4
+          for (int k = p - 2; k >= 0; --k) {
+            const double2 mk = lm[k];
+            const double nr = hr * tre - hi * tim + mk.x;
+            hi = hr * tim + hi * tre + mk.y;
             hr = nr;
           }
           sr += hr * tre - hi * tim;
           si += hr * tim + hi * tre;
         }
+        wave_lds_sync();
       } else if (lev == 0) {
-        if (valid) {
-          const int64_t j0 = (int64_t)idx * kLeaf;
-          const int64_t j1 = (j0 + kLeaf < a.ns) ? j0 + kLeaf : a.ns;
-          for (int64_t j = j0; j < j1; ++j) {
-            const double ex = x - a.sx[j], ey = y - a.sy[j];
+        const int64_t j0 = (int64_t)idx * kLeaf;
+        const int nj = (int)((j0 + kLeaf < a.ns) ? kLeaf : a.ns - j0);
+        if (lane < nj) {
+          const int64_t j = j0 + lane;
+          ls[lane] = LeafSrc{a.sx[j], a.sy[j], a.ss[j], a.sw[j]};
+        }
+        wave_lds_sync();
+        if (CORE == O2D_CORE_GAUSSIAN && __all(cfar)) {
+          // the whole leaf is beyond the compact cutoff for every lane: K = 1/d^2
+          if (valid) {
+#pragma unroll 8
+            for (int k = 0; k < nj; ++k) {
+              const LeafSrc q = ls[k];
+              const double ex = x - q.x, ey = y - q.y;
+              const double g = q.s * rcp64(__builtin_fma(ex, ex, ey * ey));
+              u = __builtin_fma(-g, ey, u);
+              v = __builtin_fma(g, ex, v);
+            }
+          }
+        } else if (valid) {
+#pragma unroll 4
+          for (int k = 0; k < nj; ++k) {
+            const LeafSrc q = ls[k];
+            const double ex = x - q.x, ey = y - q.y;
             const double e2 = __builtin_fma(ex, ex, ey * ey);
-            const double g = a.ss[j] * core_k64<CORE>(e2, a.sw[j] + wt);
+            const double g = q.s * core_k64<CORE>(e2, q.w + wt);
             u = __builtin_fma(-g, ey, u);
             v = __builtin_fma(g, ex, v);
           }
         }
+        wave_lds_sync();
       } else {
         const int64_t first = (int64_t)idx * kFan;
         const int64_t cnt = a.lv.cnt[lev - 1];
-        for (int k = kFan - 1; k >= 0; --k)
-          if (first + k < cnt) st[sp++] = ((unsigned)(lev - 1) << 27) | (unsigned)(first + k);
+        // lane k writes child kFan-1-k, so the first child ends on top of the stack
+        int nc = 0;
+        for (int k = 0; k < kFan; ++k) nc += (first + k < cnt) ? 1 : 0;
+        if (lane < nc)
+          st[sp + lane] = ((unsigned)(lev - 1) << 27) | (unsigned)(first + nc - 1 - lane);
+        sp += nc;
+        wave_lds_sync();
       }
     }
   }
