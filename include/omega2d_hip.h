@@ -341,6 +341,29 @@ int o2d_hip_morton_order_dev_d(const o2d_env* env, int64_t n, const double* x, c
                                double x0, double y0, double inv_extent, uint32_t* keys,
                                uint32_t* order);
 
+/* ---- the reference's external velocity solver hook ------------------------------------- */
+
+/* external_vel_solver_f_ / _tr_f_ / _d_: the exact signatures the reference declares for an
+ * external particle velocity solver (src/Influence.h:31-38) and calls from points_affect_points
+ * when built with EXTERNAL_VEL_SOLVE and run with a non-internal ExecEnv
+ * (src/Influence.h:74-98): counts by pointer, float or double arrays, point targets (_f_, _d_)
+ * or particle targets with radii tr (_tr_f_). Linking the reference against this library
+ * routes that call to o2d_hip_points_on_points_f/_d. Core, precision and device come from
+ * O2D_EXT_CORE (rm|gauss|wl|v2|v3, default v2), O2D_EXT_PRECISION (split|fp64; default split
+ * for the float entries, fp64 for the double one) and O2D_EXT_DEVICE (default 0), read on the
+ * first call. tu, tv accumulate the unscaled velocity. Returns the flop count
+ * nt (2 + flops per pair * ns) of the reference's convention; negative on an error, whose
+ * reason is printed to stderr. */
+float external_vel_solver_f_(int* ns, const float* sx, const float* sy, const float* ss,
+                             const float* sr, int* nt, const float* tx, const float* ty, float* tu,
+                             float* tv);
+float external_vel_solver_tr_f_(int* ns, const float* sx, const float* sy, const float* ss,
+                                const float* sr, int* nt, const float* tx, const float* ty,
+                                const float* tr, float* tu, float* tv);
+float external_vel_solver_d_(int* ns, const double* sx, const double* sy, const double* ss,
+                             const double* sr, int* nt, const double* tx, const double* ty,
+                             double* tu, double* tv);
+
 #ifdef __cplusplus
 }
 #endif

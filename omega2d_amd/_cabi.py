@@ -106,8 +106,16 @@ SIGNATURES = {
     "o2d_hip_clear_inner_dev_d": [_envp, _i32, _dbl, _dbl, _i64, _vp, _vp, _i64, _vp, _vp, _vp,
                                   _i64, _vp, _vp, _vp, _vp, _vp, _vp],
 }
+# the reference's external solver hook (counts by pointer, return value = flop count)
+_ip = ctypes.POINTER(ctypes.c_int)
+EXTERNAL_SIGNATURES = {
+    "external_vel_solver_f_": [_ip, _vp, _vp, _vp, _vp, _ip, _vp, _vp, _vp, _vp],
+    "external_vel_solver_tr_f_": [_ip, _vp, _vp, _vp, _vp, _ip, _vp, _vp, _vp, _vp, _vp],
+    "external_vel_solver_d_": [_ip, _vp, _vp, _vp, _vp, _ip, _vp, _vp, _vp, _vp],
+}
 _RESTYPES = {"o2d_hip_last_error_string": ctypes.c_char_p, "o2d_hip_abi_version": _i32,
-             "o2d_hip_flops_per_pair": _i32}
+             "o2d_hip_flops_per_pair": _i32, "external_vel_solver_f_": ctypes.c_float,
+             "external_vel_solver_tr_f_": ctypes.c_float, "external_vel_solver_d_": ctypes.c_float}
 
 _lib = None
 
@@ -128,7 +136,7 @@ def lib():
         except ImportError:
             pass
         L = ctypes.CDLL(LIB_PATH)
-        for name, args in SIGNATURES.items():
+        for name, args in {**SIGNATURES, **EXTERNAL_SIGNATURES}.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = _RESTYPES.get(name, ctypes.c_int)

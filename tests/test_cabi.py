@@ -14,9 +14,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "omega2d_hip.h")
 
 
-def declared():
+def declared(prefix=r"o2d_hip_[a-z0-9_]+"):
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(o2d_hip_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(" + prefix + r")\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -26,6 +26,44 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_cabi.SIGNATURES), "ctypes table and header disagree"
+    ext = declared(r"external_vel_solver_[a-z_]+")
+    assert ext == ["external_vel_solver_d_", "external_vel_solver_f_", "external_vel_solver_tr_f_"]
+    for n in ext:
+        assert hasattr(L, n), n
+    assert set(ext) == set(_cabi.EXTERNAL_SIGNATURES)
+
+
+_EXT_SCRIPT = """
+import ctypes, sys
+import numpy as np
+from omega2d_amd import _cabi
+L = _cabi.lib()
+a = np.ones(4, dtype=np.float32)
+p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+ns, nt = ctypes.c_int(int(sys.argv[1])), ctypes.c_int(int(sys.argv[2]))
+print(L.external_vel_solver_f_(ctypes.byref(ns), p(a), p(a), p(a), p(a), ctypes.byref(nt), p(a),
+                               p(a), p(a), p(a)))
+"""
+
+
+@pytest.mark.parametrize("env,ns,nt,expect", [
+    ({}, 0, 3, 0.0),                                  # nothing to sum: no device needed
+    ({}, 3, 0, 0.0),
+    ({"O2D_EXT_CORE": "v7"}, 3, 3, -1.0),             # rejected before any device call
+    ({"O2D_EXT_PRECISION": "fp16"}, 3, 3, -1.0),
+    ({"O2D_EXT_DEVICE": "x"}, 3, 3, -1.0),
+    ({}, -1, 3, -1.0),
+])
+def test_external_hook_configuration_and_counts(env, ns, nt, expect):
+    import subprocess
+    import sys
+    e = dict(os.environ, PYTHONPATH=ROOT, **env)
+    r = subprocess.run([sys.executable, "-c", _EXT_SCRIPT, str(ns), str(nt)], env=e,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert float(r.stdout.split()[-1]) == expect
+    if expect < 0:
+        assert "external_vel_solver" in r.stderr
 
 
 def test_abi_version_and_flop_convention():
