@@ -15,7 +15,10 @@
  *    _d takes double storage, _f float storage (the reference application's STORE=float);
  *    both compute in the precision env->precision selects. Device mirrors are kept by the
  *    library per device and reused across calls.
- *  - o2d_hip_*_dev_d: device pointers, enqueued on env->stream (asynchronous).
+ *  - o2d_hip_*_dev_d: device pointers, enqueued on env->stream (asynchronous). The library's
+ *    scratch buffers (sort keys, partial sums, staged panels, LU factors) are per device and
+ *    shared by every call on that device, so device-pointer calls on different streams of one
+ *    device must not overlap in time: order them (one stream, or events between streams).
  */
 #ifndef OMEGA2D_HIP_H
 #define OMEGA2D_HIP_H

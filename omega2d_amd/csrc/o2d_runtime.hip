@@ -52,8 +52,9 @@ int slot(Device* d, Slot which, size_t bytes, hipStream_t s, void** out) {
   if (bytes == 0) bytes = 256;
   if (d->cap[which] < bytes) {
     if (d->buf[which]) {
-      O2D_HIP(hipStreamSynchronize(s));
-      O2D_HIP(hipStreamSynchronize(d->stream));
+      // the old buffer may still be read by work queued on any stream of this device
+      (void)s;
+      O2D_HIP(hipDeviceSynchronize());
       O2D_HIP(hipFree(d->buf[which]));
       d->buf[which] = nullptr;
       d->cap[which] = 0;
@@ -156,7 +157,8 @@ int o2d_hip_release(void) {
     Device* d = &g_dev[i];
     if (!d->ready) continue;
     O2D_HIP(hipSetDevice(i));
-    O2D_HIP(hipStreamSynchronize(d->stream));
+    // work on caller streams may still read the buffers
+    O2D_HIP(hipDeviceSynchronize());
     release_blas(d);
     for (int s = 0; s < kNumSlots; ++s) {
       if (d->buf[s]) O2D_HIP(hipFree(d->buf[s]));
