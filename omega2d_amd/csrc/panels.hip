@@ -27,11 +27,11 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kPTile = 256;  // panels per LDS stage (panels on points)
-constexpr int kPanTile = 8;  // panels per workgroup (points on panels)
+#ifndef O2D_PANTILE
+#define O2D_PANTILE 8
+#endif
+constexpr int kPanTile = O2D_PANTILE;  // panels per workgroup (points on panels)
 
-struct Panel {
-  double x0, y0, x1, y1, vs, ss;
-};
 
 template <bool SRC>
 __global__ __launch_bounds__(kBlock) void panels_on_points_kernel(
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void panels_on_points_kernel(
     const uint32_t* __restrict__ idx, const double* __restrict__ vs, const double* __restrict__ ss,
     int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty, int64_t chunk,
     double* __restrict__ ou, double* __restrict__ ov, int direct) {
-  __shared__ Panel tile[kPTile];
+  __shared__ PanelPre tile[kPTile];
   const int tid = threadIdx.x;
   const int64_t i = (int64_t)blockIdx.x * kBlock + tid;
   const int64_t ii = i < nt ? i : nt - 1;
@@ -49,26 +49,22 @@ __global__ __launch_bounds__(kBlock) void panels_on_points_kernel(
   const int64_t je = jb + chunk < np ? jb + chunk : np;
   for (int64_t j0 = jb; j0 < je; j0 += kPTile) {
     const int64_t j = j0 + tid;
-    Panel p;
+    PanelPre p{};
     if (j < je) {
       const uint32_t a = idx[2 * j], b = idx[2 * j + 1];
-      p.x0 = nx[a];
-      p.y0 = ny[a];
-      p.x1 = nx[b];
-      p.y1 = ny[b];
-      p.vs = vs[j];
-      p.ss = SRC ? ss[j] : 0.0;
+      p = panel_pre(nx[a], ny[a], nx[b], ny[b], vs[j], SRC ? ss[j] : 0.0);
     }
     __syncthreads();
     tile[tid] = p;
     __syncthreads();
     const int nk = (int)(je - j0 < kPTile ? je - j0 : kPTile);
     for (int k = 0; k < nk; ++k) {
-      const Panel q = tile[k];
-      double u, v;
-      panel_vel(q.x0, q.y0, q.x1, q.y1, q.vs, q.ss, px, py, u, v);
-      au += u;
-      av += v;
+      const PanelPre& q = tile[k];
+      double lr, li;
+      panel_log(q, px, py, lr, li);
+      // v + i u = W L
+      au += __builtin_fma(q.wr, li, q.wi * lr);
+      av += __builtin_fma(q.wr, lr, -q.wi * li);
     }
   }
   if (i >= nt) return;
@@ -109,38 +105,40 @@ __global__ __launch_bounds__(kBlock) void points_on_panels_kernel(
     const double* __restrict__ ny, const uint32_t* __restrict__ idx, int64_t chunk,
     double* __restrict__ part) {
   __shared__ double red[kBlock / 64][2 * kPanTile];
+  __shared__ PanelPre pp[kPanTile];
   const int tid = threadIdx.x;
   const int64_t p0 = (int64_t)blockIdx.x * kPanTile;
-  double x0[kPanTile], y0[kPanTile], x1[kPanTile], y1[kPanTile];
-#pragma unroll
-  for (int k = 0; k < kPanTile; ++k) {
-    const int64_t p = p0 + k < np ? p0 + k : np - 1;
+  if (tid < kPanTile) {
+    const int64_t p = p0 + tid < np ? p0 + tid : np - 1;
     const uint32_t a = idx[2 * p], b = idx[2 * p + 1];
-    x0[k] = nx[a];
-    y0[k] = ny[a];
-    x1[k] = nx[b];
-    y1[k] = ny[b];
+    pp[tid] = panel_pre(nx[a], ny[a], nx[b], ny[b], 1.0, 0.0);  // W = conj(e)
   }
-  double au[kPanTile], av[kPanTile];
+  __syncthreads();
+  // sum over particles of s_j L_j per panel; conj(e) is applied once at the end
+  double ar[kPanTile], ai[kPanTile];
 #pragma unroll
-  for (int k = 0; k < kPanTile; ++k) au[k] = av[k] = 0.0;
+  for (int k = 0; k < kPanTile; ++k) ar[k] = ai[k] = 0.0;
   const int64_t jb = (int64_t)blockIdx.y * chunk;
   const int64_t je = jb + chunk < ns ? jb + chunk : ns;
   for (int64_t j = jb + tid; j < je; j += kBlock) {
     const double qx = sx[j], qy = sy[j], qs = ss[j];
 #pragma unroll
     for (int k = 0; k < kPanTile; ++k) {
-      double u, v;
-      panel_vel(x0[k], y0[k], x1[k], y1[k], qs, 0.0, qx, qy, u, v);
-      au[k] += u;
-      av[k] += v;
+      // read the panel from LDS at each use rather than keeping all eight in registers
+      asm volatile("" ::: "memory");
+      const PanelPre q = pp[k];
+      double lr, li;
+      panel_log(q, qx, qy, lr, li);
+      ar[k] = __builtin_fma(qs, lr, ar[k]);
+      ai[k] = __builtin_fma(qs, li, ai[k]);
     }
   }
   const int wv = tid / 64, ln = tid % 64;
 #pragma unroll
   for (int k = 0; k < kPanTile; ++k) {
-    const double su = wave_sum(au[k]);
-    const double sv = wave_sum(av[k]);
+    // v + i u = conj(e) sum_j s_j L_j
+    const double su = wave_sum(__builtin_fma(pp[k].wr, ai[k], pp[k].wi * ar[k]));
+    const double sv = wave_sum(__builtin_fma(pp[k].wr, ar[k], -pp[k].wi * ai[k]));
     if (ln == 0) {
       red[wv][2 * k] = su;
       red[wv][2 * k + 1] = sv;
