@@ -29,13 +29,15 @@ struct alignas(16) Probe {
 };
 
 template <int CORE, int TPL>
-__global__ __launch_bounds__(kBlock) void adjoint_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void adjoint_kernel(
     int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ ss, const double* __restrict__ sr, int64_t nt,
     const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ gu,
     const double* __restrict__ gv, int64_t chunk, double* __restrict__ ox, double* __restrict__ oy,
     double* __restrict__ os, double* __restrict__ orr, int direct) {
   __shared__ Probe tile[kTile];
+  __shared__ double4 box[kBlock / 64];
   const int tid = threadIdx.x;
   const int64_t jbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
   double px[TPL], py[TPL], pr[TPL], ax[TPL], ay[TPL], as[TPL], ar[TPL];
@@ -66,7 +68,58 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(
     }
     __syncthreads();
     tile[tid] = p;
+    if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+      // bounding box of this tile's probes (padding probes at the origin only enlarge it)
+      double b0 = p.x, b1 = p.x, b2 = p.y, b3 = p.y;
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        b0 = fmin(b0, __shfl_xor(b0, m));
+        b1 = fmax(b1, __shfl_xor(b1, m));
+        b2 = fmin(b2, __shfl_xor(b2, m));
+        b3 = fmax(b3, __shfl_xor(b3, m));
+      }
+      if ((tid & 63) == 0) box[tid / 64] = make_double4(b0, b1, b2, b3);
+    }
     __syncthreads();
+    if constexpr (CORE == O2D_CORE_GAUSSIAN) {
+      double bx0 = box[0].x, bx1 = box[0].y, by0 = box[0].z, by1 = box[0].w;
+#pragma unroll
+      for (int w = 1; w < kBlock / 64; ++w) {
+        bx0 = fmin(bx0, box[w].x);
+        bx1 = fmax(bx1, box[w].y);
+        by0 = fmin(by0, box[w].z);
+        by1 = fmax(by1, box[w].w);
+      }
+      // every probe of the tile beyond the compact cutoff of this lane's sources: then
+      // K = 1/d^2, bbb = -2/d^4 and dK/dsigma = 0 for every pair, with no branch per pair.
+      // With z = dx + i dy, t = 1/z and G = gv - i gu the sums above reduce to
+      //   dJ/ds += Re(G t),   (dJ/dx, dJ/dy) / s += (Re(G t^2), -Im(G t^2))
+      bool far = true;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        const double ex = fmax(fmax(bx0 - px[t], px[t] - bx1), 0.0);
+        const double ey = fmax(fmax(by0 - py[t], py[t] - by1), 0.0);
+        far = far && (ex * ex + ey * ey > 16.0 * pr[t] * pr[t] * (1.0 + 1e-12));
+      }
+      if (__all(far)) {
+#pragma unroll 4
+        for (int q = 0; q < kTile; ++q) {
+          const Probe pr_ = tile[q];
+#pragma unroll
+          for (int t = 0; t < TPL; ++t) {
+            const double dx = pr_.x - px[t];
+            const double dy = pr_.y - py[t];
+            const double kk = rcp64(__builtin_fma(dx, dx, dy * dy));
+            const double tr = dx * kk, ti = -dy * kk;  // t = 1/z
+            as[t] = __builtin_fma(pr_.gv, tr, __builtin_fma(pr_.gu, ti, as[t]));
+            const double sr2 = __builtin_fma(tr, tr, -ti * ti), si2 = 2.0 * tr * ti;  // t^2
+            ax[t] = __builtin_fma(pr_.gv, sr2, __builtin_fma(pr_.gu, si2, ax[t]));
+            ay[t] = __builtin_fma(pr_.gu, sr2, __builtin_fma(-pr_.gv, si2, ay[t]));
+          }
+        }
+        continue;
+      }
+    }
 #pragma unroll 2
     for (int q = 0; q < kTile; ++q) {
       const Probe pr_ = tile[q];

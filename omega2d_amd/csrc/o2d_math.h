@@ -107,6 +107,23 @@ __device__ __forceinline__ void core_kg64(double d2, double w, double& k, double
   }
 }
 
+struct KAdj {
+  double k, bbb, dks;
+};
+
+// Gaussian inside its compact cutoff, for the adjoint; out of line so that the pair loop keeps
+// only the far-field registers (beyond the cutoff K = 1/d^2 and dK/dsigma = 0)
+__device__ __noinline__ KAdj gauss_adj_core64(double d2, double sr) {
+  const double s2 = sr * sr;
+  const double cf = rcp64(s2);
+  const double x = d2 * cf;
+  if (x < 0.001) return KAdj{cf, 0.0, -2.0 * cf * rcp64(sr)};
+  const double ood2 = rcp64(d2);
+  const double e = exp(-x);
+  const double r = ood2 * (1.0 - e);
+  return KAdj{r, -2.0 * r * ood2 + 2.0 * cf * ood2 * e, -2.0 * e * cf * rcp64(sr)};
+}
+
 // K, bbb consistent with the branch of K actually taken, and dK/dsigma_source, for the adjoint
 // of point-target sums (t2 = 0).
 template <int CORE>
@@ -125,20 +142,10 @@ __device__ __forceinline__ void core_adj64(double d2, double sr, double& k, doub
       bbb = -2.0 * ood2 * ood2;
       dks = 0.0;
     } else {
-      const double cf = rcp64(s2);
-      const double x = d2 * cf;
-      if (x < 0.001) {
-        k = cf;
-        bbb = 0.0;
-        dks = -2.0 * cf * rcp64(sr);
-      } else {
-        const double ood2 = rcp64(d2);
-        const double e = exp(-x);
-        const double r = ood2 * (1.0 - e);
-        k = r;
-        bbb = -2.0 * r * ood2 + 2.0 * cf * ood2 * e;
-        dks = -2.0 * e * cf * rcp64(sr);
-      }
+      const KAdj a = gauss_adj_core64(d2, sr);
+      k = a.k;
+      bbb = a.bbb;
+      dks = a.dks;
     }
   } else if constexpr (CORE == O2D_CORE_WL) {
     const double dd = d2 + s2;
