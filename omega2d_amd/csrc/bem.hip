@@ -173,6 +173,9 @@ __global__ __launch_bounds__(kBlock) void augment_kernel(int64_t np, const Geom*
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
+}  // namespace
+
+// the device's rocBLAS handle (created on first use), bound to stream s
 int blas_of(Device* d, hipStream_t s, rocblas_handle* out) {
   if (!d->blas) {
     rocblas_handle h;
@@ -186,8 +189,6 @@ int blas_of(Device* d, hipStream_t s, rocblas_handle* out) {
   *out = h;
   return 0;
 }
-
-}  // namespace
 
 void release_blas(Device* d) {
   if (d->blas) {
@@ -269,10 +270,14 @@ int bem_factor_dev(Device* d, hipStream_t s, int64_t n, const double* A, int64_t
                            hipMemcpyDeviceToDevice, s));
   rocblas_handle h;
   O2D_TRY(blas_of(d, s, &h));
-  if (rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, static_cast<double*>(lu), (rocblas_int)n,
-                       static_cast<rocblas_int*>(piv), static_cast<rocblas_int*>(info)) !=
-      rocblas_status_success)
+  if (n <= lu_max_order()) {
+    O2D_TRY(lu_factor_dev(d, s, (int)n, static_cast<double*>(lu), (int)n,
+                          static_cast<int*>(piv), static_cast<int*>(info)));
+  } else if (rocsolver_dgetrf(h, (rocblas_int)n, (rocblas_int)n, static_cast<double*>(lu),
+                              (rocblas_int)n, static_cast<rocblas_int*>(piv),
+                              static_cast<rocblas_int*>(info)) != rocblas_status_success) {
     return fail(O2D_ERR_HIP, "rocsolver_dgetrf failed");
+  }
   rocblas_int hinfo = 0;
   O2D_HIP(hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, s));
   O2D_HIP(hipStreamSynchronize(s));

@@ -34,6 +34,10 @@ int bem_assemble_dev(Device* d, hipStream_t s, int64_t nnodes, const double* nx,
                      int64_t lda);
 
 int bem_factor_dev(Device* d, hipStream_t s, int64_t n, const double* A, int64_t lda);
+
+// blocked LU with partial pivoting (lu.hip), n <= lu_max_order(); ipiv 1-based, info device int
+int64_t lu_max_order();
+int lu_factor_dev(Device* d, hipStream_t s, int n, double* A, int lda, int* ipiv, int* info);
 int bem_solve_dev(Device* d, hipStream_t s, int64_t n, const double* b, double* x);
 
 int rk_move_dev(Device* d, hipStream_t s, int64_t n, double dt, int nstage, const double* w,

@@ -118,6 +118,31 @@ def test_bem_repeated_device_solves_in_place():
         assert rel_l2(db.cpu().numpy(), np.linalg.solve(A, b)) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 63, 64, 65, 130, 1000, 4100, 6145])
+def test_lu_of_general_matrices_matches_lapack(n):
+    """Random matrices need row interchanges at almost every column: the blocked factorisation
+    (orders up to 6144) and the library fallback above it solve like LAPACK."""
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n))
+    o2d.bem_factor(A)
+    for k in range(2):  # the fresh factors, then the inverse built from them
+        b = rng.normal(size=n)
+        x = o2d.bem_solve(b)
+        assert rel_l2(x, np.linalg.solve(A, b)) <= 1e-9, (n, k)
+        assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("n", [70, 300])
+def test_lu_reports_a_singular_matrix(n):
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(n, n))
+    A[:, 5] = 0.0
+    with pytest.raises(o2d.O2DError) as e:
+        o2d.bem_factor(A)
+    assert e.value.code == -7
+    assert "singular" in str(e.value)
+
+
 def test_bem_solve_without_factor_of_that_size_fails():
     o2d.bem_factor(np.eye(3) * 2.0)
     with pytest.raises(o2d.O2DError):
