@@ -25,11 +25,19 @@ class BEM {
     A_is_current = false;
     solver_initialized = false;
   }
+  // the time of the previous solve_bem call (a function-static in src/BEMHelper.h:54; kept per
+  // system here so that two systems do not share it)
+  double last_time() const { return last_t; }
+  void set_last_time(double t) { last_t = t; }
+  // blocks assembled since construction (src/BEM.h has no counter; for tests of the rebuild
+  // policy)
+  size_t blocks_built() const { return nblocks; }
   void set_device(int32_t d) { device = d; }
 
   // column-major block copy into A, growing A as needed (src/BEM.h:116-136)
   void set_block(size_t rstart, size_t nrows, size_t cstart, size_t ncols,
                  const std::vector<double>& in) {
+    ++nblocks;
     resize(std::max(nr, rstart + nrows), std::max(nc, cstart + ncols));
     size_t k = 0;
     for (size_t j = 0; j < ncols; ++j)
@@ -73,6 +81,8 @@ class BEM {
   std::vector<double> A, b, x;
   size_t nr = 0, nc = 0;
   bool A_is_current = false, solver_initialized = false;
+  double last_t = -99.9;
+  size_t nblocks = 0;
   int32_t device = 0;
 };
 
@@ -138,7 +148,6 @@ template <class S, class A, class I>
 void solve_bem(double time, const std::array<double, Dimensions>& fs,
                std::vector<Collection<S>>& vort, std::vector<Collection<S>>& bdry,
                BEM<S, I>& bem, const ExecEnv& env) {
-  (void)time;
   if (bdry.empty()) return;
   if (!bem.is_A_current()) {
     I row = 0;
@@ -156,9 +165,11 @@ void solve_bem(double time, const std::array<double, Dimensions>& fs,
   for (auto& t : bdry) {
     auto* surf = std::get_if<Surfaces<S>>(&t);
     if (!surf) continue;
+    surf->transform(time);  // prescribed motion of the parent body
     surf->zero_vels();
     for (auto& s : vort) std::visit(iv, std::as_const(s), t);
     surf->finalize_vels(fs);
+    surf->add_body_motion(S(-1.0), time);
     std::vector<S> rhs = vels_to_rhs_panels<S>(*surf);
     if (surf->is_augmented()) {
       const S tot = surf->get_last_body_circ() + surf->get_reabsorbed() +
@@ -168,15 +179,24 @@ void solve_bem(double time, const std::array<double, Dimensions>& fs,
     bem.set_rhs(surf->get_first_row(), surf->get_num_rows(), rhs);
   }
 
-  if (!bem.is_A_current()) {
-    bem.panels_changed();
-    // every (target, source) pair of bodies, src/BEMHelper.h:176-219
+  // src/BEMHelper.h:147-219: after a reset every block is built; at a new time only the blocks
+  // between bodies in relative motion are rebuilt
+  const bool rebuild_all = !bem.is_A_current();
+  const bool check_motion = !rebuild_all && bem.last_time() != time;
+  if (rebuild_all || check_motion) {
+    const double last = bem.last_time();
+    bool rebuilt = false;
     for (auto& t : bdry) {
       auto* targ = std::get_if<Surfaces<S>>(&t);
       if (!targ) continue;
       for (auto& sc : bdry) {
         auto* src = std::get_if<Surfaces<S>>(&sc);
         if (!src) continue;
+        bool rebuild = rebuild_all;
+        if (check_motion && targ->get_body_ptr() && src->get_body_ptr())
+          rebuild = targ->get_body_ptr()->relative_motion_vs(*src->get_body_ptr(), last, time);
+        if (!rebuild) continue;
+        rebuilt = true;
         if (src->is_augmented()) {
           // velocity on the target's panels from the source's unit-rotation strengths
           targ->zero_vels();
@@ -191,6 +211,9 @@ void solve_bem(double time, const std::array<double, Dimensions>& fs,
                       src->get_num_rows(), coeffs);
       }
     }
+    // the reference re-initialises its solver whenever the time changes; the factors here are
+    // redone only when a block changed, which gives the same solution
+    if (rebuilt) bem.panels_changed();
     bem.just_made_A();
   }
 
@@ -200,6 +223,7 @@ void solve_bem(double time, const std::array<double, Dimensions>& fs,
     if (!surf) continue;
     surf->set_str(bem.get_str(surf->get_first_row(), surf->get_num_rows()));
   }
+  bem.set_last_time(time);
 }
 
 }  // namespace omega2d

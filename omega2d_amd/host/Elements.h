@@ -1,7 +1,7 @@
 // Elements.h - SoA element collections for the HIP path: particles/tracers (Points) and
 // straight two-node panels (Surfaces). Host mirror of the members of src/Points.h,
 // src/Surfaces.h and src/ElementBase.h that the direct Biot-Savart path reads and writes;
-// drawing, VTK output, diffusion and feature bookkeeping are not part of it.
+// drawing, diffusion and feature bookkeeping are not part of it.
 #pragma once
 
 #include <algorithm>
@@ -9,6 +9,8 @@
 #include <cassert>
 #include <cmath>
 #include <cstdint>
+#include <functional>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <optional>
@@ -28,13 +30,64 @@ enum move_t { lagrangian = 1, bodybound = 2, fixed = 3 };
 template <class S>
 using Vector = std::vector<S>;
 
-// A rigid body reduced to what the BEM path asks of it: its name (the "ground" body is never
-// augmented when it bounds internal flow) and its rotation rate.
+// A rigid body (src/Body.cpp): its name (the "ground" body is never augmented when it bounds
+// internal flow and never moves the boundary condition) and its prescribed motion. The
+// reference parses position and orientation as expressions of t; here they are callables, and
+// an empty one keeps that coordinate at its constant value. Velocities are the same two-point
+// differences with step 1e-6 (src/Body.cpp:165-224); transform(t) stores position, velocity,
+// orientation (wrapped by remainder(., 2 pi)) and rotation rate for the no-argument getters.
 struct Body {
+  using Vec = std::array<double, Dimensions>;
+  static constexpr double kDT = 1.e-6;
+
   std::string name = "ground";
-  double rotvel = 0.0;
+  std::array<std::function<double(double)>, Dimensions> pos_func;
+  std::function<double(double)> apos_func;
+  Vec pos = {0.0, 0.0}, vel = {0.0, 0.0};
+  double apos = 0.0, avel = 0.0;
+
   const std::string& get_name() const { return name; }
-  double get_rotvel() const { return rotvel; }
+  Vec get_pos() const { return pos; }
+  Vec get_pos(double t) const {
+    Vec p;
+    for (size_t i = 0; i < Dimensions; ++i) p[i] = pos_func[i] ? pos_func[i](t) : pos[i];
+    return p;
+  }
+  Vec get_vel() const { return vel; }
+  Vec get_vel(double t) const {
+    Vec v;
+    for (size_t i = 0; i < Dimensions; ++i)
+      v[i] = pos_func[i] ? (pos_func[i](t + kDT) - pos_func[i](t - kDT)) / (2.0 * kDT) : vel[i];
+    return v;
+  }
+  double get_orient() const { return apos; }
+  double get_orient(double t) const {
+    return apos_func ? std::remainder(apos_func(t), 2.0 * M_PI) : apos;
+  }
+  double get_rotvel() const { return avel; }
+  double get_rotvel(double t) const {
+    return apos_func ? (apos_func(t + kDT) - apos_func(t - kDT)) / (2.0 * kDT) : avel;
+  }
+  void transform(double t) {
+    pos = get_pos(t);
+    vel = get_vel(t);
+    apos = get_orient(t);
+    avel = get_rotvel(t);
+  }
+  // src/Body.cpp:237-265: translation or rotation of one body relative to the other between two
+  // times, beyond 4 machine epsilons
+  bool relative_motion_vs(const Body& o, double last, double current) const {
+    const double eps4 = 4.0 * std::numeric_limits<double>::epsilon();
+    const Vec a0 = get_pos(last), b0 = o.get_pos(last), a1 = get_pos(current),
+              b1 = o.get_pos(current);
+    bool motion = false;
+    for (size_t i = 0; i < Dimensions; ++i)
+      if (std::abs(a1[i] - a0[i] - b1[i] + b0[i]) > eps4) motion = true;
+    const double rel = get_orient(current) - get_orient(last) - o.get_orient(current) +
+                       o.get_orient(last);
+    if (std::abs(rel) > eps4) motion = true;
+    return motion;
+  }
 };
 
 // ---------------------------------------------------------------------------------- Points
@@ -188,6 +241,7 @@ class Surfaces {
     }
     for (size_t d = 0; d < Dimensions; ++d) pu[d].assign(np, S(0));
     if (B && M == bodybound) set_geom_center();
+    tc = utc;
   }
 
   size_t get_n() const { return n; }
@@ -307,6 +361,52 @@ class Surfaces {
   void zero_vels() {
     for (size_t d = 0; d < Dimensions; ++d) std::fill(pu[d].begin(), pu[d].end(), S(0));
   }
+
+  // src/ElementBase.h:250-273 and src/Surfaces.h:728-752: a body-bound surface follows its body
+  // (nodes rotated about the body origin and translated, from the untransformed nodes); the
+  // bases are recomputed and the geometric centre is carried along
+  void transform(double time) {
+    const bool moves = B && M == bodybound;
+    S st = S(0), ct = S(1);
+    std::array<double, Dimensions> p = {0.0, 0.0};
+    if (moves) {
+      B->transform(time);
+      p = B->get_pos();
+      const double th = B->get_orient();
+      st = S(std::sin(th));
+      ct = S(std::cos(th));
+      for (size_t i = 0; i < n; ++i) {
+        x[0][i] = S(p[0]) + ux[0][i] * ct - ux[1][i] * st;
+        x[1][i] = S(p[1]) + ux[0][i] * st + ux[1][i] * ct;
+      }
+    }
+    compute_bases();
+    if (moves) {
+      tc[0] = S(p[0]) + utc[0] * ct - utc[1] * st;
+      tc[1] = S(p[1]) + utc[0] * st + utc[1] * ct;
+    } else {
+      tc = utc;
+    }
+  }
+
+  // src/Surfaces.h:492-523: factor times the body's velocity at each panel centre (translation
+  // plus rotation about the transformed geometric centre) added to the panel velocities; the
+  // reference rounds the body's velocities to float here
+  void add_body_motion(S factor, double time) {
+    if (!B || B->get_name() == "ground") return;
+    assert(vol > S(0) && "a moving body needs a positive area");
+    const std::array<double, Dimensions> v = B->get_vel(time);
+    const double w = B->get_rotvel(time);
+    for (size_t i = 0; i < np; ++i) {
+      for (size_t d = 0; d < Dimensions; ++d) pu[d][i] += factor * S(float(v[d]));
+      const Int id0 = idx[2 * i], id1 = idx[2 * i + 1];
+      const S xc = S(0.5) * (x[0][id1] + x[0][id0]);
+      const S yc = S(0.5) * (x[1][id1] + x[1][id0]);
+      pu[0][i] -= factor * S(float(w)) * (yc - tc[1]);
+      pu[1][i] += factor * S(float(w)) * (xc - tc[0]);
+    }
+  }
+
   // src/Surfaces.h:764-774
   void finalize_vels(const std::array<double, Dimensions>& fs) {
     const double factor = 0.5 / M_PI;
@@ -407,7 +507,7 @@ class Surfaces {
   bool source_str_is_unknown = true;
   Int istart = 0;
   S vol = S(-1.0);
-  std::array<S, Dimensions> utc = {S(0), S(0)};
+  std::array<S, Dimensions> utc = {S(0), S(0)}, tc = {S(0), S(0)};
   S solved_omega = S(0), omega_error = S(0), this_omega = S(0), reabsorbed_gamma = S(0);
 };
 

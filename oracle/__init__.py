@@ -120,6 +120,9 @@ def ref():
                                              _vp, _int, _i64, _i64, _dbl, ctypes.c_char_p, _i64]
         R.ref_write_panels_vtk_f.argtypes = [ctypes.c_char_p, _i64, _vp, _vp, _i64, _vp, _vp, _vp,
                                              _i64, _i64, _dbl, ctypes.c_char_p, _i64]
+        R.ref_body_motion_d.argtypes = [_i64, _vp, _vp, _i64, _vp, ctypes.c_char_p,
+                                        ctypes.c_char_p, ctypes.c_char_p, _dbl, _dbl, _vp, _vp,
+                                        _vp, _vp, _vp, _vp, _vp]
         _ref = R
     return _ref
 
@@ -583,3 +586,21 @@ def ref_bem_block(snx, sny, sidx, skind, tnx, tny, tidx, tkind, nunk):
     r, c = nr.value, nc.value
     return dict(A=A[: r * c].reshape(c, r).T.copy(), urot_u=uu, urot_v=uv)
 
+
+def ref_body_motion(nx, ny, idx, xexpr, yexpr, rexpr, t, factor=-1.0):
+    """Reference body moving by tinyexpr expressions of t: its body-bound Surfaces transformed to
+    time t and the body's motion added (times factor) to zeroed panel velocities. Returns
+    dict(x, y, tx, ty, pu, pv, body=[x, y, u, v, theta, omega])."""
+    nx, ny = _f32nodes(nx, ny)
+    idx = _u32(idx)
+    nn, np_ = len(nx), len(idx) // 2
+    out = {k: np.zeros(nn if k in ("x", "y") else np_) for k in ("x", "y", "tx", "ty", "pu", "pv")}
+    body = np.zeros(6)
+    with quiet():
+        rc = ref().ref_body_motion_d(nn, _p(nx), _p(ny), np_, _p(idx), xexpr.encode(),
+                                     yexpr.encode(), rexpr.encode(), float(t), float(factor),
+                                     *[_p(out[k]) for k in ("x", "y", "tx", "ty", "pu", "pv")],
+                                     _p(body))
+    assert rc == 0, rc
+    out["body"] = body
+    return out

@@ -491,5 +491,38 @@ int ref_bem_block_d(int64_t snn, const float* snx, const float* sny, int64_t snp
                                    tkind, nunk, coeffs, nrows, ncols, urot_u, urot_v);
 }
 
-}  // extern "C"
+// A body moving by the tinyexpr expressions xexpr(t), yexpr(t) and rexpr(t) (src/Body.cpp; an
+// empty string keeps that coordinate still), its body-bound Surfaces transformed to time t
+// (src/ElementBase.h:250-273, src/Surfaces.h:728-752) and the body's motion added with factor f to
+// zeroed panel velocities (src/Surfaces.h:492-523). Out: nodes x, y (nnodes); tangents tx, ty
+// and panel velocities pu, pv (np); body = pos(t) x, y, vel(t) x, y, orient(t), rotvel(t).
+int ref_body_motion_d(int64_t nnodes, const float* nx, const float* ny, int64_t np,
+                      const uint32_t* idx, const char* xexpr, const char* yexpr,
+                      const char* rexpr, double t, double f, double* ox, double* oy,
+                      double* otx, double* oty, double* opu, double* opv, double* body) {
+  auto b = std::make_shared<Body>();
+  b->set_name("mover");
+  if (xexpr && *xexpr) b->set_pos(0, std::string(xexpr));
+  if (yexpr && *yexpr) b->set_pos(1, std::string(yexpr));
+  if (rexpr && *rexpr) b->set_rot(std::string(rexpr));
+  HarnessSurfaces<double> s = make_surfaces<double>(nnodes, nx, ny, np, idx, reactive, bodybound, b);
+  s.transform(t);
+  s.zero_vels();
+  s.add_body_motion(f, t);
+  copy_out(s.get_pos()[0], ox);
+  copy_out(s.get_pos()[1], oy);
+  copy_out(s.get_tang()[0], otx);
+  copy_out(s.get_tang()[1], oty);
+  copy_out(s.get_vel()[0], opu);
+  copy_out(s.get_vel()[1], opv);
+  const auto p = b->get_pos(t), v = b->get_vel(t);
+  body[0] = p[0];
+  body[1] = p[1];
+  body[2] = v[0];
+  body[3] = v[1];
+  body[4] = b->get_orient(t);
+  body[5] = b->get_rotvel(t);
+  return 0;
+}
 
+}  // extern "C"

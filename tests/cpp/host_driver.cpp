@@ -16,6 +16,7 @@
 //                  in place of <out.bin> (no GPU work)
 //
 // Input: int64 header [mode-specific counts], then float64 arrays (node coordinates as float32).
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -182,6 +183,114 @@ void run_bem2(Reader& in, Writer& out) {
   out.vec(p.get_vel()[1]);
 }
 
+// the prescribed motions the tests give bodies (the reference parses the same expressions)
+void set_motion(Body& b, int64_t kind) {
+  if (kind == 1) {  // "0.3*sin(2*t)", "0.1*t*t", "0.5*t"
+    b.pos_func[0] = [](double t) { return 0.3 * std::sin(2 * t); };
+    b.pos_func[1] = [](double t) { return 0.1 * t * t; };
+    b.apos_func = [](double t) { return 0.5 * t; };
+  } else if (kind == 2) {  // "1.5+0.1*sin(t)", "", "0.3*t"
+    b.pos_func[0] = [](double t) { return 1.5 + 0.1 * std::sin(t); };
+    b.apos_func = [](double t) { return 0.3 * t; };
+  } else if (kind == 3) {  // "", "", "7*t" (rotation only, wraps past 2 pi)
+    b.apos_func = [](double t) { return 7 * t; };
+  }
+}
+
+// header: nnodes, npan, kind; then t, factor, nodes x/y (float32), idx. Body motion only (no
+// device calls): transformed nodes, tangents, panel velocities and the body state.
+void run_motion(Reader& in, Writer& out) {
+  const int64_t nn = in.i64(), np = in.i64(), kind = in.i64();
+  const double t = in.f64(), factor = in.f64();
+  const auto nx = in.vec<float>(nn), ny = in.vec<float>(nn);
+  const auto idx = in.vec<uint32_t>(2 * np);
+  auto b = std::make_shared<Body>();
+  b->name = "mover";
+  set_motion(*b, kind);
+  Surfaces<double> s(nn, nx.data(), ny.data(), np, idx.data(), reactive, bodybound, b);
+  s.transform(t);
+  s.zero_vels();
+  s.add_body_motion(factor, t);
+  out.vec(s.get_pos()[0]);
+  out.vec(s.get_pos()[1]);
+  out.vec(s.get_tang()[0]);
+  out.vec(s.get_tang()[1]);
+  out.vec(s.get_vel()[0]);
+  out.vec(s.get_vel()[1]);
+  const auto p = b->get_pos(t), v = b->get_vel(t);
+  out.vec(std::vector<double>{p[0], p[1], v[0], v[1], b->get_orient(t), b->get_rotvel(t)});
+}
+
+// header as bem2 (npanA, nnodesA, npanB, nnodesB, nwake, nunk, core) plus t0, t1. Body A is the
+// ground, body B moves by motion 2. One system is solved at t0, t1 and t1 again (only the blocks
+// in relative motion rebuilt at t1); a second one at t0, then reset, t1 (every block rebuilt)
+// and t1 again, so both carry the same circulation history. Out: blocks built after each solve
+// of the first system, the relative difference of the two matrices, then the strengths and
+// solved rotation rates of both bodies for each system.
+void run_bem_move(Reader& in, Writer& out) {
+  const int64_t npa = in.i64(), nna = in.i64(), npb = in.i64(), nnb = in.i64(), nw = in.i64(),
+                nunk = in.i64(), core = in.i64();
+  const double t0 = in.f64(), t1 = in.f64();
+  const auto ax = in.vec<float>(nna), ay = in.vec<float>(nna);
+  const auto ai = in.vec<uint32_t>(2 * npa);
+  const auto bx = in.vec<float>(nnb), by = in.vec<float>(nnb);
+  const auto bi = in.vec<uint32_t>(2 * npb);
+  auto wx = in.vec<double>(nw), wy = in.vec<double>(nw), ws = in.vec<double>(nw),
+       wr = in.vec<double>(nw);
+  ExecEnv env(true, direct, gpu_hip);
+  env.set_core((int32_t)core);
+  auto make = [&](std::vector<Collection<double>>& bdry) {
+    auto ground = std::make_shared<Body>();
+    auto mover = std::make_shared<Body>();
+    mover->name = "cylinder";
+    set_motion(*mover, 2);
+    Surfaces<double> sa(nna, ax.data(), ay.data(), npa, ai.data(), reactive, bodybound, ground);
+    Surfaces<double> sb(nnb, bx.data(), by.data(), npb, bi.data(), reactive, bodybound, mover);
+    if (nunk == 1) {
+      sa.set_vortex_only();
+      sb.set_vortex_only();
+    }
+    bdry.emplace_back(sa);
+    bdry.emplace_back(sb);
+  };
+  std::vector<Collection<double>> vort;
+  vort.emplace_back(Points<double>(nw, wx.data(), wy.data(), ws.data(), wr.data()));
+  auto dump = [&](std::vector<Collection<double>>& bdry) {
+    for (auto& c : bdry) {
+      const auto& surf = std::get<Surfaces<double>>(c);
+      out.vec(surf.get_str());
+      out.vec(surf.get_src_str());
+      out.f64(surf.get_solved_omega());
+    }
+  };
+  const std::array<double, Dimensions> fs = {1.0, 0.0};
+  std::vector<Collection<double>> bdry, fresh;
+  make(bdry);
+  BEM<double, Int> bem;
+  solve_bem<double, double, Int>(t0, fs, vort, bdry, bem, env);
+  const double b0 = (double)bem.blocks_built();
+  solve_bem<double, double, Int>(t1, fs, vort, bdry, bem, env);
+  const double b1 = (double)bem.blocks_built();
+  solve_bem<double, double, Int>(t1, fs, vort, bdry, bem, env);  // same time: nothing rebuilt
+  const double b2 = (double)bem.blocks_built();
+  make(fresh);
+  BEM<double, Int> bem2;
+  solve_bem<double, double, Int>(t0, fs, vort, fresh, bem2, env);
+  bem2.panels_changed();
+  solve_bem<double, double, Int>(t1, fs, vort, fresh, bem2, env);
+  solve_bem<double, double, Int>(t1, fs, vort, fresh, bem2, env);
+  const auto& m1 = bem.matrix();
+  const auto& m2 = bem2.matrix();
+  double dd = 0.0, nn = 0.0;
+  for (size_t i = 0; i < m1.size() && i < m2.size(); ++i) {
+    dd += (m1[i] - m2[i]) * (m1[i] - m2[i]);
+    nn += m2[i] * m2[i];
+  }
+  out.vec(std::vector<double>{b0, b1, b2, std::sqrt(dd / nn)});
+  dump(bdry);
+  dump(fresh);
+}
+
 // header: n; then x, y, s, r
 void run_vort(Reader& in, Writer& out, int64_t n) {
   const auto x = in.vec<double>(n), y = in.vec<double>(n), s = in.vec<double>(n),
@@ -225,7 +334,7 @@ void run_vtk(Reader& in, const std::string& dir) {
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bem2|vort|vtk> <in.bin> <out.bin|dir>\n";
+    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bem2|bemmove|vort|motion|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
@@ -234,6 +343,11 @@ int main(int argc, char** argv) {
     if (mode == "vtk") {
       run_vtk(in, argv[3]);
       return 0;
+    }
+    if (mode == "motion") {
+      Writer mo(argv[3]);
+      run_motion(in, mo);
+      return mo.f ? 0 : 1;
     }
     Writer out(argv[3]);
     if (mode == "rk" || mode == "rktree") {
@@ -248,6 +362,8 @@ int main(int argc, char** argv) {
       run_vort(in, out, in.i64());
     } else if (mode == "bem2") {
       run_bem2(in, out);
+    } else if (mode == "bemmove") {
+      run_bem_move(in, out);
 
     } else {
       std::cerr << "unknown mode " << mode << "\n";

@@ -169,6 +169,19 @@ def main():
                 key = f"bp_{tag}_{nunk}_{sk}{tk}"
                 d[key + "_A"], d[key + "_uu"], d[key + "_uv"] = R["A"], R["urot_u"], R["urot_v"]
 
+    # prescribed body motion (src/Body.cpp, src/ElementBase.h:250-273, src/Surfaces.h:492-523,
+    # 728-752): a body-bound circle moved by tinyexpr expressions of t, transformed and its
+    # motion added (factor -1) to zeroed panel velocities, at three times
+    mx_, my_, mi_ = circle_panels(20, diam=0.5, cx=0.4, cy=-0.2)
+    d.update(bm_nx=mx_, bm_ny=my_, bm_ni=mi_, bm_times=np.array([0.0, 0.37, 1.9]))
+    motions = {1: ("0.3*sin(2*t)", "0.1*t*t", "0.5*t"), 2: ("1.5+0.1*sin(t)", "", "0.3*t"),
+               3: ("", "", "7*t")}
+    for kind, ex in motions.items():
+        for ti, t in enumerate(d["bm_times"]):
+            R = O.ref_body_motion(mx_, my_, mi_, *ex, t, -1.0)
+            for k in ("x", "y", "tx", "ty", "pu", "pv", "body"):
+                d[f"bm_{kind}_{ti}_{k}"] = R[k]
+
     # VTU files written by the reference's own writers (src/Points.h:556-740,
     # src/Surfaces.h:1238-1379, src/VtkXmlWriter.h), kept byte for byte under tests/golden/vtk/
     r4 = np.random.default_rng(20241204)
