@@ -194,6 +194,38 @@ void set_motion(Body& b, int64_t kind) {
     b.apos_func = [](double t) { return 0.3 * t; };
   } else if (kind == 3) {  // "", "", "7*t" (rotation only, wraps past 2 pi)
     b.apos_func = [](double t) { return 7 * t; };
+  } else if (kind == 4) {  // "t", "", "" (unit speed along x)
+    b.pos_func[0] = [](double t) { return t; };
+  }
+}
+
+// header: npan, nnodes, nunk; then t, nodes x/y (float32), idx. A cylinder moving at unit speed
+// along x through still fluid, solved at time t, and the same cylinder held still where the
+// first one is at t in a freestream (-1, 0): the two boundary conditions are the same, so are
+// the solutions. Out: strengths, source strengths and solved rotation rate of each.
+void run_bem_galilean(Reader& in, Writer& out) {
+  const int64_t np = in.i64(), nn = in.i64(), nunk = in.i64();
+  const double t = in.f64();
+  const auto nx = in.vec<float>(nn), ny = in.vec<float>(nn);
+  const auto idx = in.vec<uint32_t>(2 * np);
+  ExecEnv env(true, direct, gpu_hip);
+  env.set_core(O2D_CORE_V2);
+  for (int still = 0; still < 2; ++still) {
+    auto b = std::make_shared<Body>();
+    b->name = "cylinder";
+    if (still) b->pos = {t, 0.0};
+    else set_motion(*b, 4);
+    Surfaces<double> surf(nn, nx.data(), ny.data(), np, idx.data(), reactive, bodybound, b);
+    if (nunk == 1) surf.set_vortex_only();
+    std::vector<Collection<double>> vort, bdry;
+    bdry.emplace_back(surf);
+    BEM<double, Int> bem;
+    const std::array<double, Dimensions> fs = {still ? -1.0 : 0.0, 0.0};
+    solve_bem<double, double, Int>(t, fs, vort, bdry, bem, env);
+    const auto& s2 = std::get<Surfaces<double>>(bdry[0]);
+    out.vec(s2.get_str());
+    out.vec(s2.get_src_str());
+    out.f64(s2.get_solved_omega());
   }
 }
 
@@ -334,7 +366,7 @@ void run_vtk(Reader& in, const std::string& dir) {
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bem2|bemmove|vort|motion|vtk> <in.bin> <out.bin|dir>\n";
+    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bem2|bemmove|bemgalilean|vort|motion|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
@@ -364,6 +396,8 @@ int main(int argc, char** argv) {
       run_bem2(in, out);
     } else if (mode == "bemmove") {
       run_bem_move(in, out);
+    } else if (mode == "bemgalilean") {
+      run_bem_galilean(in, out);
 
     } else {
       std::cerr << "unknown mode " << mode << "\n";

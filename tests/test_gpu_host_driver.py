@@ -368,3 +368,23 @@ def test_moving_body_rebuilds_only_the_blocks_in_relative_motion(tmp_path, nunk)
     assert got[3] <= 1e-7
     moved, fresh = got[4:4 + per], got[4 + per:]
     assert rel_l2(moved, fresh) <= 1e-6
+
+
+@pytest.mark.parametrize("nunk", [1, 2])
+def test_moving_body_boundary_condition_is_galilean(tmp_path, nunk):
+    """A cylinder moving at unit speed through still fluid (its motion enters the right-hand side
+    through add_body_motion, src/Surfaces.h:492-523) and the same cylinder held still in a
+    freestream (-1, 0) have the same boundary condition: the strengths agree, and the vortex
+    sheet follows the potential-flow slip 2|sin theta| of a circle."""
+    nx, ny, idx = circle_panels(128)
+    npan = len(idx) // 2
+    t = 0.3
+    got = run_driver(tmp_path, "bemgalilean", [npan, len(nx), nunk], [np.array([t]), nx, ny, idx],
+                     2 * (2 * npan + 1))
+    moving, still = got[:2 * npan + 1], got[2 * npan + 1:]
+    assert rel_l2(moving, still) <= 1e-8
+    if nunk == 1:
+        mx = 0.5 * (nx[idx[0::2]].astype(np.float64) + nx[idx[1::2]])
+        my = 0.5 * (ny[idx[0::2]].astype(np.float64) + ny[idx[1::2]])
+        th = np.arctan2(my, mx)
+        assert np.max(np.abs(np.abs(moving[:npan]) - 2 * np.abs(np.sin(th)))) < 0.1
