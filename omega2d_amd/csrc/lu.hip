@@ -71,6 +71,7 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
   __shared__ int bmap[kMaxOrder];  // row map of the block's interchanges, rows P..n-1
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int m = n - c;
+  int szero = -1;  // first zero pivot of the leaf (thread 0)
   double a[Q][kLeaf];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -116,10 +117,8 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
         for (int k = 0; k < kLeaf; ++k) jrow[b][k] = a[q][k];
       }
     }
-    if (t == 0) {
-      spiv[j] = p;
-      ipiv[c + j] = c + p + 1;
-    }
+    if (t == 0) spiv[j] = p;  // to global memory after the loop: a pending global store
+                              // would hold the next barrier
     __syncthreads();
     if (p != j) {
 #pragma unroll
@@ -148,10 +147,11 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
             if (k < l) a[q][k] = __builtin_fma(-lr, prow[b][k], a[q][k]);
         }
       }
-    } else if (t == 0 && *info == 0) {
-      *info = c + j + 1;  // singular: the column is left as it is, like dgetf2
+    } else if (t == 0 && szero < 0) {
+      szero = j;  // singular: the column is left as it is, like dgetf2
     }
   }
+  if (t == 0 && szero >= 0 && *info == 0) *info = c + szero + 1;
   // the factored leaf back to memory; its first l rows (L11 below the diagonal) to LDS; the
   // leaf's interchanges as one permutation (destination row <- source row, leaf-relative)
 #pragma unroll
@@ -167,6 +167,7 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
       for (int k = 0; k < kLeaf; ++k) sl[t][k] = a[0][k];
     }
   }
+  if (t < l) ipiv[c + t] = c + spiv[t] + 1;
   if (t == 0) {
     int np = 0;
     for (int j = 0; j < l; ++j) {
