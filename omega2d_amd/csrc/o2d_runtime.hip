@@ -128,14 +128,24 @@ KernelTimer::KernelTimer(int c, hipStream_t st) : cls(c), s(st) {
   std::lock_guard<std::mutex> lk(g_timing_mu);
   a = take_event();
   b = take_event();
-  if (a && b) hipEventRecord(a, s);
+  // a launch whose start event cannot be recorded is simply not timed
+  if (!a || !b || hipEventRecord(a, s) != hipSuccess) {
+    if (a) g_event_pool.push_back(a);
+    if (b) g_event_pool.push_back(b);
+    a = b = nullptr;
+  }
 }
 
 void KernelTimer::stop() {
   if (!a || !b) return;
-  hipEventRecord(b, s);
+  const bool ok = hipEventRecord(b, s) == hipSuccess;
   std::lock_guard<std::mutex> lk(g_timing_mu);
-  g_pending.push_back({cls, a, b});
+  if (ok) {
+    g_pending.push_back({cls, a, b});
+  } else {
+    g_event_pool.push_back(a);
+    g_event_pool.push_back(b);
+  }
   a = b = nullptr;
 }
 

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
   for (int64_t base = j0; base < j1; base += kTile) {
     const int n = (int)((j1 - base) < kTile ? (j1 - base) : kTile);
     __syncthreads();
-    if (threadIdx.x < n) {
+    if ((int)threadIdx.x < n) {
       const int64_t j = base + threadIdx.x;
       const double r2 = sr[j] * sr[j];
       const double rr = 1.0 / r2;
