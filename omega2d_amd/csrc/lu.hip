@@ -5,7 +5,7 @@
 // A library dgetrf on a matrix of a few thousand rows spends its time in three kernel launches
 // per column (pivot search, singularity check, scale and update), microseconds each. Here the
 // factorisation is right-looking in blocks of kNB columns, each factored in leaves of kLeaf:
-//   lu_leaf_kernel    one workgroup of 1024 threads holds the leaf's rows in registers and
+//   lu_leaf_kernel    one workgroup (kLeafThreads) holds the leaf's rows in registers and
 //                     does the pivot search, interchange, scaling and rank-1 updates of its
 //                     columns with two workgroup barriers per column; then it applies the
 //                     leaf's interchanges to the block's other columns (as one permutation of at
@@ -36,10 +36,13 @@ namespace {
 
 constexpr int kNB = 64;          // columns per block (trailing update by dgemm)
 constexpr int kLeaf = 8;         // columns per leaf (one workgroup)
-constexpr int kLeafThreads = 1024;
+#ifndef O2D_LU_THREADS
+#define O2D_LU_THREADS 512
+#endif
+constexpr int kLeafThreads = O2D_LU_THREADS;  // one workgroup per leaf
 constexpr int kWaves = kLeafThreads / 64;
-constexpr int kMaxRowsPerThread = 6;  // leaves of up to 6144 rows
-constexpr int kMaxOrder = kLeafThreads * kMaxRowsPerThread;
+constexpr int kMaxOrder = 6144;               // leaves of up to 6144 rows
+constexpr int kMaxRowsPerThread = kMaxOrder / kLeafThreads;
 constexpr int kMaxPairs = 2 * kNB;
 constexpr int kBlock = 256;
 
@@ -52,7 +55,7 @@ __device__ __forceinline__ void pick(double& v, int& i, double ov, int oi) {
 }
 
 // Leaf of columns [c, c + l) of block [P, pend): rows c..n-1 (m = n - c), thread t holding rows
-// t, t + 1024, ... (Q of them). bpairs: after the block's last leaf, [count, (dst, src)...].
+// t, t + kLeafThreads, ... (Q of them). bpairs: after the block's last leaf, [count, (dst, src)...].
 template <int Q>
 __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int l, int P,
                                                                 int pend, double* __restrict__ A,
@@ -323,9 +326,16 @@ __global__ __launch_bounds__(64) void lu_trsm_kernel(int n, int P, double* __res
 
 __global__ void lu_zero_info_kernel(int* info) { *info = 0; }
 
+// the leaf kernel for q rows per thread, q <= Q
 template <int Q>
-void launch_leaf(hipStream_t s, int n, int c, int l, int P, int pend, double* A, int lda,
+void launch_leaf(int q, hipStream_t s, int n, int c, int l, int P, int pend, double* A, int lda,
                  int* ipiv, int* info, int* bpairs) {
+  if constexpr (Q > 1) {
+    if (q < Q) {
+      launch_leaf<Q - 1>(q, s, n, c, l, P, pend, A, lda, ipiv, info, bpairs);
+      return;
+    }
+  }
   lu_leaf_kernel<Q><<<1, kLeafThreads, 0, s>>>(n, c, l, P, pend, A, lda, ipiv, info, bpairs);
 }
 
@@ -354,14 +364,7 @@ int lu_factor_dev(Device* d, hipStream_t s, int n, double* A, int lda, int* ipiv
     for (int c = P; c < pend; c += kLeaf) {
       const int l = std::min(kLeaf, pend - c);
       const int q = (n - c + kLeafThreads - 1) / kLeafThreads;
-      switch (q) {
-        case 1: launch_leaf<1>(s, n, c, l, P, pend, A, lda, ipiv, info, bpairs); break;
-        case 2: launch_leaf<2>(s, n, c, l, P, pend, A, lda, ipiv, info, bpairs); break;
-        case 3: launch_leaf<3>(s, n, c, l, P, pend, A, lda, ipiv, info, bpairs); break;
-        case 4: launch_leaf<4>(s, n, c, l, P, pend, A, lda, ipiv, info, bpairs); break;
-        case 5: launch_leaf<5>(s, n, c, l, P, pend, A, lda, ipiv, info, bpairs); break;
-        default: launch_leaf<6>(s, n, c, l, P, pend, A, lda, ipiv, info, bpairs); break;
-      }
+      launch_leaf<kMaxRowsPerThread>(q, s, n, c, l, P, pend, A, lda, ipiv, info, bpairs);
       O2D_LAUNCH_CHECK();
       const int rest = pend - (c + l), rows = n - (c + l);
       if (rest > 0 && rows > 0) {
