@@ -11,7 +11,9 @@
 //                     leaf's interchanges to the block's other columns (as one permutation of at
 //                     most 2 kLeaf rows), solves the block's top rows right of the leaf
 //                     (U12 = L11^-1 A12), and after the block's last leaf lists the block's
-//                     interchanges as (destination, source) row pairs;
+//                     interchanges as (destination, source) row pairs; a pair's source is
+//                     found by undoing the interchanges in reverse order, so no lane waits
+//                     on another;
 //   lu_update_kernel  the rank-kLeaf update of the block's columns right of the leaf;
 //   lu_laswp_kernel   the block's row pairs applied to every column outside the block, all
 //                     loads of a column before its stores;
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
   __shared__ int spiv[kLeaf];
   __shared__ int pdst[2 * kLeaf], psrc[2 * kLeaf];
   __shared__ int s_np;
-  __shared__ int bmap[kMaxOrder];  // row map of the block's interchanges, rows P..n-1
+  __shared__ int sbpiv[kNB];  // the block's pivot rows (0-based)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int m = n - c;
   int szero = -1;  // first zero pivot of the leaf (thread 0)
@@ -171,30 +173,20 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
     }
   }
   if (t < l) ipiv[c + t] = c + spiv[t] + 1;
-  if (t == 0) {
-    int np = 0;
-    for (int j = 0; j < l; ++j) {
+  // The leaf's interchanges as one permutation: only rows 0..l-1 and the pivot rows move. For
+  // each of those 2l candidate rows, the row whose content ends there is found by undoing the
+  // interchanges in reverse order (a repeated candidate repeats its pair, which is harmless).
+  if (t < 2 * l) {
+    const int r = t < l ? t : spiv[t - l];
+    int src = r;
+    for (int j = l - 1; j >= 0; --j) {
       const int p = spiv[j];
-      if (p == j) continue;
-      int ij = -1, ip = -1;
-      for (int i = 0; i < np; ++i) {
-        if (pdst[i] == j) ij = i;
-        if (pdst[i] == p) ip = i;
-      }
-      const int vj = ij >= 0 ? psrc[ij] : j, vp = ip >= 0 ? psrc[ip] : p;
-      if (ij < 0) {
-        ij = np;
-        pdst[np++] = j;
-      }
-      if (ip < 0) {
-        ip = np;
-        pdst[np++] = p;
-      }
-      psrc[ij] = vp;
-      psrc[ip] = vj;
+      src = src == j ? p : (src == p ? j : src);
     }
-    s_np = np;
+    pdst[t] = r;
+    psrc[t] = src;
   }
+  if (t == 0) s_np = 2 * l;
   __syncthreads();
   // the same rows permuted in the block's other columns: every load before any store
   const int np = s_np;
@@ -231,36 +223,23 @@ __global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int
     }
   }
   // after the block's last leaf: the block's interchanges (rows P..pend-1, in order) as row
-  // pairs for the columns outside the block. Only rows P..pend-1 and the pivot rows move.
+  // pairs for the columns outside the block, found in the same way over the 2 nb candidates
   if (c + l == pend && pend - P < n) {
-    const int mm = n - P;
-    for (int i = t; i < mm; i += kLeafThreads) bmap[i] = i;
+    const int nb = pend - P;
+    __syncthreads();  // the leaf's own pivots are in global memory
+    if (t < nb) sbpiv[t] = ipiv[P + t] - 1;
     __syncthreads();
-    if (t == 0) {
-      for (int k = P; k < pend; ++k) {
-        const int p = ipiv[k] - 1;
-        if (p != k) {
-          const int x = bmap[k - P];
-          bmap[k - P] = bmap[p - P];
-          bmap[p - P] = x;
-        }
+    if (t < 2 * nb) {
+      const int r = t < nb ? P + t : sbpiv[t - nb];
+      int src = r;
+      for (int k = nb - 1; k >= 0; --k) {
+        const int p = sbpiv[k];
+        src = src == P + k ? p : (src == p ? P + k : src);
       }
-      int cnt = 0;
-      for (int k = P; k < pend; ++k) {
-        if (bmap[k - P] != k - P) {
-          bpairs[1 + 2 * cnt] = k;
-          bpairs[2 + 2 * cnt] = P + bmap[k - P];
-          ++cnt;
-        }
-        const int p = ipiv[k] - 1;
-        if (p >= pend && bmap[p - P] != p - P) {  // repeated pivot rows give repeated pairs
-          bpairs[1 + 2 * cnt] = p;
-          bpairs[2 + 2 * cnt] = P + bmap[p - P];
-          ++cnt;
-        }
-      }
-      bpairs[0] = cnt;
+      bpairs[1 + 2 * t] = r;
+      bpairs[2 + 2 * t] = src;
     }
+    if (t == 0) bpairs[0] = 2 * nb;
   }
 }
 
