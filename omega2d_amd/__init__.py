@@ -180,8 +180,8 @@ def points_on_points_adjoint(sx, sy, ss, sr, tx, ty, gu, gv, *, core="gauss", de
     return tuple(out)
 
 
-def points_on_points_tree(sx, sy, ss, sr, tx, ty, tr=None, *, core="gauss", theta=0.5,
-                          order=20, core_tol=1e-12, device=0, tu=None, tv=None):
+def points_on_points_tree(sx, sy, ss, sr, tx, ty, tr=None, *, core="gauss", theta=0.6,
+                          order=28, core_tol=1e-12, device=0, tu=None, tv=None):
     """Treecode form of points_on_points (summation_t barneshut, src/ExecEnv.h:26-31): same
     unscaled, accumulated velocity sum, fp64, with far cells replaced by their moments."""
     sx, sy, ss, sr, tx, ty = (_arr(a, np.float64) for a in (sx, sy, ss, sr, tx, ty))

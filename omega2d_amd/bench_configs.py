@@ -275,7 +275,7 @@ def c2tree():
                                  r[idx].contiguous(), core="gauss", precision="fp64")
     res = {"config": "C2tree", "name": w["name"], "particles": n, "tracers": nt,
            "equivalent_pairs": pairs}
-    for theta, order in ((0.5, 20), (0.4, 30)):
+    for theta, order in ((0.6, 28), (0.5, 20), (0.4, 30)):
         def step():
             u, v = dv.points_on_points_tree(x, y, s, r, x, y, r, theta=theta, order=order)
             dv.points_on_points_tree(x, y, s, r, tx, ty, theta=theta, order=order)

@@ -16,6 +16,7 @@
 // Otherwise the cell is opened, and a leaf is summed directly with the exact core in fp64.
 // With S = sum_k M_k / (z - c)^(k+1), the unscaled velocity is u - i v = -i S, i.e.
 // u = Im S, v = Re S.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -197,6 +198,11 @@ struct TreeArgs {
   int p;
   double theta;
   double tol;
+  // the walk may be split over ksplit waves per target group: wave k starts from the cells of
+  // level slevel whose index is k mod ksplit and writes its partial sums to part
+  int ksplit;
+  int slevel;
+  double* part;  // [2][ksplit][nt]: u then v partials (ksplit > 1)
 };
 
 // makes an in-wave LDS write visible to the other lanes of the wave before they read it, and
@@ -243,9 +249,18 @@ __global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t 
   // a wave with no valid lane (the tail of the grid) has nothing to do
   if (__any(valid)) {
     int sp = 0;
-    const int root = a.lv.n - 1;
-    if (lane == 0) st[0] = (unsigned)root << 27;  // (level << 27) | index; the root is index 0
-    sp = 1;
+    if (a.ksplit == 1) {
+      const int root = a.lv.n - 1;
+      if (lane == 0) st[0] = (unsigned)root << 27;  // (level << 27) | index; the root is index 0
+      sp = 1;
+    } else {
+      // cells k, k + ksplit, ... of level slevel, the smallest index on top
+      const int k = blockIdx.y;
+      const int ck = (int)((a.lv.cnt[a.slevel] - k + a.ksplit - 1) / a.ksplit);
+      if (lane < ck)
+        st[lane] = ((unsigned)a.slevel << 27) | (unsigned)(k + (ck - 1 - lane) * a.ksplit);
+      sp = ck;
+    }
     wave_lds_sync();
     while (sp > 0) {
       const unsigned e = __builtin_amdgcn_readfirstlane(st[--sp]);
@@ -323,9 +338,30 @@ __global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t 
     }
   }
   if (valid) {
-    tu[ti] += u + si;
-    tv[ti] += v + sr;
+    if (a.ksplit == 1) {
+      tu[ti] += u + si;
+      tv[ti] += v + sr;
+    } else {
+      a.part[(int64_t)blockIdx.y * nt + ti] = u + si;
+      a.part[(int64_t)(a.ksplit + blockIdx.y) * nt + ti] = v + sr;
+    }
   }
+}
+
+// tu[i] += sum over k (in order) of the split walk's partials
+__global__ __launch_bounds__(kBlock) void tree_reduce_kernel(int64_t nt, int ksplit,
+                                                             const double* __restrict__ part,
+                                                             double* __restrict__ tu,
+                                                             double* __restrict__ tv) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nt) return;
+  double su = 0.0, sv = 0.0;
+  for (int k = 0; k < ksplit; ++k) {
+    su += part[(int64_t)k * nt + i];
+    sv += part[(int64_t)(ksplit + k) * nt + i];
+  }
+  tu[i] += su;
+  tv[i] += sv;
 }
 
 }  // namespace
@@ -346,7 +382,8 @@ int tree_dev(Device* d, hipStream_t s, int core, double theta, int p, double tol
     if (cnt == 1) break;
     cnt = (cnt + kFan - 1) / kFan;
   }
-  if ((kFan - 1) * lv.n + 1 > kStack) return fail(O2D_ERR_SIZE, "tree too deep");
+  // a split walk starts from up to kFan cells of one level
+  if ((kFan - 1) * lv.n + kFan > kStack) return fail(O2D_ERR_SIZE, "tree too deep");
   // buffers
   void *ko, *to, *tk, *sbuf, *cb, *mb;
   O2D_TRY(slot(d, kTreeKeys, sizeof(uint32_t) * (ns > nt ? ns : nt), s, &tk));
@@ -382,8 +419,27 @@ int tree_dev(Device* d, hipStream_t s, int core, double theta, int p, double tol
                          torder));
     kt.stop();
   }
-  TreeArgs a{lv, cells, mom, xs, ys, ssorted, ws, ns, p, theta, tol};
-  const dim3 grid((unsigned)((nt + kTBlock - 1) / kTBlock));
+  // few target groups leave the GPU short of waves (each wave's walk is latency-bound): split
+  // the walk of each group over up to 16 waves, each from its share of one level's cells
+  const int64_t waves = (nt + 63) / 64, want = 16 * (int64_t)d->num_cu;
+  int ksplit = 1, slevel = lv.n - 1;
+  if (waves < want && lv.n > 1) {
+    ksplit = (int)std::min<int64_t>(16, (want + waves - 1) / waves);
+    for (int l = lv.n - 1; l >= 0; --l)
+      if (lv.cnt[l] >= ksplit) {
+        slevel = l;
+        break;
+      }
+    if (lv.cnt[slevel] < ksplit) ksplit = (int)lv.cnt[slevel];
+  }
+  double* part = nullptr;
+  if (ksplit > 1) {
+    void* pp;
+    O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * ksplit * nt, s, &pp));
+    part = static_cast<double*>(pp);
+  }
+  TreeArgs a{lv, cells, mom, xs, ys, ssorted, ws, ns, p, theta, tol, ksplit, slevel, part};
+  const dim3 grid((unsigned)((nt + kTBlock - 1) / kTBlock), (unsigned)ksplit);
   KernelTimer kt(O2D_KT_TREE_EVAL, s);
   switch (core) {
     case O2D_CORE_RM:
@@ -405,6 +461,11 @@ int tree_dev(Device* d, hipStream_t s, int core, double theta, int p, double tol
   }
   kt.stop();
   O2D_LAUNCH_CHECK();
+  if (ksplit > 1) {
+    tree_reduce_kernel<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(nt, ksplit, part,
+                                                                               tu, tv);
+    O2D_LAUNCH_CHECK();
+  }
   return 0;
 }
 

@@ -147,7 +147,7 @@ def morton_order(x, y, x0, y0, inv_extent):
     return keys, order
 
 
-def points_on_points_tree(sx, sy, ss, sr, tx, ty, tr=None, *, core="gauss", theta=0.5, order=20,
+def points_on_points_tree(sx, sy, ss, sr, tx, ty, tr=None, *, core="gauss", theta=0.6, order=28,
                           core_tol=1e-12, tu=None, tv=None):
     """Treecode velocity sum on device tensors (accumulates into tu, tv; zeros when absent)."""
     dev = _check(sx, sy, ss, sr, tx, ty, tr, tu, tv)

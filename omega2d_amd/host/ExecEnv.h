@@ -74,7 +74,7 @@ class ExecEnv {
   int32_t m_core = O2D_CORE_V2;
   int32_t m_prec = O2D_PLAIN_FP64;
   int32_t m_device = 0;
-  o2d_tree_opts m_tree = {0.5, 20, 0, 1e-12};
+  o2d_tree_opts m_tree = {0.6, 28, 0, 1e-12};
 };
 
 enum results_t { velonly = 1, velandgrad = 2, psionly = 3, velandvort = 4, velandshear = 5 };

@@ -122,3 +122,27 @@ def test_tree_on_the_1m_blob_workload():
                                  r[idx].contiguous(), core="gauss", precision="fp64")
     got = (tu[idx].cpu().numpy(), tv[idx].cpu().numpy())
     assert rel2(got, (ru.cpu().numpy(), rv.cpu().numpy())) <= 1e-7
+
+
+def test_tree_split_walk_for_few_targets():
+    """65,536 tracers are 1,024 target groups, too few waves for the chip, so each group's walk is
+    split over several waves (each from a share of one level's cells) and the partial sums are
+    added in a fixed order: the result is reproducible bit for bit, accumulates, and keeps the
+    error bar against the fp64 direct sum."""
+    import torch
+    import omega2d_amd.device as dv
+    w = W.c2_blobs()
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    x, y, s, r = (cu(w[k]) for k in ("x", "y", "s", "r"))
+    tx, ty = cu(w["tx"]), cu(w["ty"])
+    tu, tv = dv.points_on_points_tree(x, y, s, r, tx, ty, None, theta=0.6, order=28)
+    u2, v2 = dv.points_on_points_tree(x, y, s, r, tx, ty, None, theta=0.6, order=28)
+    assert torch.equal(tu, u2) and torch.equal(tv, v2)
+    au, av = dv.points_on_points_tree(x, y, s, r, tx, ty, None, theta=0.6, order=28,
+                                      tu=torch.ones_like(tu), tv=-torch.ones_like(tv))
+    assert torch.allclose(au - 1.0, tu, rtol=1e-15, atol=1e-15)
+    assert torch.allclose(av + 1.0, tv, rtol=1e-15, atol=1e-15)
+    ru, rv = dv.points_on_points(x, y, s, r, tx, ty, None, core="gauss", precision="fp64")
+    err = rel2((tu.cpu().numpy(), tv.cpu().numpy()), (ru.cpu().numpy(), rv.cpu().numpy()))
+    print("split tree walk, 65,536 tracers:", err)
+    assert err <= 1e-9
