@@ -39,6 +39,16 @@ constexpr int kGroup = 64;   // sources sharing one centroid in the split form
 #ifndef O2D_WAVES32
 #define O2D_WAVES32 8
 #endif
+// source chunks: enough workgroups for this many per CU (each chunk at least kMinChunkTiles
+// tiles). Many short-lived workgroups keep every SIMD issuing to the end of the launch; with
+// one resident round of long-lived ones, the 1M-particle launch ran 10% slower.
+#ifndef O2D_WANT32
+#define O2D_WANT32 128
+#endif
+#ifndef O2D_WANT64
+#define O2D_WANT64 8
+#endif
+constexpr int kMinChunkTiles = 16;
 constexpr int kTpl64 = O2D_TPL64;  // targets per lane, plain form
 constexpr int kTpl32 = O2D_TPL32;  // targets per lane, split form
 
@@ -211,11 +221,25 @@ void nbody32_kernel(
     pw[t] = TR ? (float)rterm<CORE>(tr[ii]) : 0.0f;
     au[t] = av[t] = aw[t] = 0.0;
   }
+  // when every target of the block has the same radius term w0, it is added to each source's
+  // term once as the tile is staged, and the pair loop runs without the per-pair add (the same
+  // fp32 sum, so the same result)
+  float w0 = 0.0f;
+  bool uni = false;
+  if constexpr (TR) {
+    const int64_t i0 = (int64_t)blockIdx.x * (kBlock * TPL);
+    w0 = (float)rterm<CORE>(tr[i0 < nt ? i0 : nt - 1]);
+    bool same = true;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) same = same && pw[t] == w0;
+    uni = __syncthreads_and(same) != 0;
+  }
 
   const int64_t tb = (int64_t)blockIdx.y * tiles_per_chunk;
   const int64_t te = tb + tiles_per_chunk < ntiles ? tb + tiles_per_chunk : ntiles;
   for (int64_t it = tb; it < te; ++it) {
-    const float4 s = src[it * kTile + tid];
+    float4 s = src[it * kTile + tid];
+    if (uni) s.w += w0;
     __syncthreads();
     tile[tid] = s;
     if (tid < kGroups) tgrp[tid] = grp[it * kGroups + tid];
@@ -245,8 +269,13 @@ void nbody32_kernel(
         }
         if (__all(far))
           group_pairs<CORE, RES, TR, TPL, 16, false>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
+        else if (uni)
+          group_pairs<CORE, RES, false, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
         else
           group_pairs<CORE, RES, TR, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
+      } else if (uni) {
+        group_pairs<CORE, RES, false, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv,
+                                                         fw);
       } else {
         group_pairs<CORE, RES, TR, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
       }
@@ -494,10 +523,17 @@ struct ChunkPlan {
   int64_t per_chunk;  // sources (plain) or tiles (split) per chunk
 };
 
-// Enough workgroups for ~8 per CU; never more chunks than tiles.
-ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles) {
-  const int64_t want = 8 * (int64_t)d->num_cu;
-  int64_t nchunk = target_wgs >= want ? 1 : (want + target_wgs - 1) / target_wgs;
+// Splits the source tiles into chunks (grid.y) so that the launch has per_cu workgroups per CU.
+// Beyond 8 per CU a chunk keeps at least kMinChunkTiles tiles, so that few targets do not turn
+// into thousands of tiny chunks and partial sums.
+ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles, int per_cu) {
+  const auto chunks_for = [&](int64_t wgs_per_cu) {
+    const int64_t want = wgs_per_cu * (int64_t)d->num_cu;
+    return target_wgs >= want ? (int64_t)1 : (want + target_wgs - 1) / target_wgs;
+  };
+  int64_t nchunk = chunks_for(per_cu);
+  const int64_t most = std::max(chunks_for(8), (ntiles + kMinChunkTiles - 1) / kMinChunkTiles);
+  if (nchunk > most) nchunk = most;
   if (nchunk > ntiles) nchunk = ntiles;
   if (nchunk < 1) nchunk = 1;
   const int64_t per = (ntiles + nchunk - 1) / nchunk;
@@ -617,7 +653,7 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   if (prec == O2D_PLAIN_FP64) {
     const int64_t twgs = (nt + kBlock * kTpl64 - 1) / (kBlock * kTpl64);
     const int64_t ntiles = (ns + kTile - 1) / kTile;
-    ChunkPlan cp = plan_chunks(d, twgs, ntiles);
+    ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT64);
     const int64_t chunk = cp.per_chunk * kTile;
     const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
     double *ou = tu, *ov = tv, *ow = tw;
@@ -674,7 +710,7 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   O2D_LAUNCH_CHECK();
 
   const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);
-  ChunkPlan cp = plan_chunks(d, twgs, ntiles);
+  ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT32);
   const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
   double *ou = tu, *ov = tv, *ow = tw;
   const int direct = cp.nchunk == 1;

@@ -419,12 +419,15 @@ int tree_dev(Device* d, hipStream_t s, int core, double theta, int p, double tol
                          torder));
     kt.stop();
   }
-  // few target groups leave the GPU short of waves (each wave's walk is latency-bound): split
-  // the walk of each group over up to 16 waves, each from its share of one level's cells
-  const int64_t waves = (nt + 63) / 64, want = 16 * (int64_t)d->num_cu;
+  // each wave's walk is a chain of dependent loads, so the walk wants many waves per SIMD, and
+  // the groups' costs vary widely: below 256 waves per CU, split the walk of each group over up
+  // to 64 waves, each from its share of one level's cells (measured on the 1M blob workload: the
+  // tracers' 1,024 groups over 64 waves each and the particles' 16,384 groups over 4 each took
+  // 6.1 ms, against 7.5 ms for 4 and 1)
+  const int64_t waves = (nt + 63) / 64, want = 256 * (int64_t)d->num_cu;
   int ksplit = 1, slevel = lv.n - 1;
   if (waves < want && lv.n > 1) {
-    ksplit = (int)std::min<int64_t>(16, (want + waves - 1) / waves);
+    ksplit = (int)std::min<int64_t>(64, (want + waves - 1) / waves);
     for (int l = lv.n - 1; l >= 0; --l)
       if (lv.cnt[l] >= ksplit) {
         slevel = l;
