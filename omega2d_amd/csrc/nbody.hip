@@ -41,12 +41,12 @@ constexpr int kGroup = 64;   // sources sharing one centroid in the split form
 #endif
 // source chunks: enough workgroups for this many per CU (each chunk at least kMinChunkTiles
 // tiles). Many short-lived workgroups keep every SIMD issuing to the end of the launch; with
-// one resident round of long-lived ones, the 1M-particle launch ran 10% slower.
+// one resident round of long-lived ones, the 1M-particle launches ran 8-10% slower.
 #ifndef O2D_WANT32
 #define O2D_WANT32 128
 #endif
 #ifndef O2D_WANT64
-#define O2D_WANT64 8
+#define O2D_WANT64 128
 #endif
 constexpr int kMinChunkTiles = 16;
 constexpr int kTpl64 = O2D_TPL64;  // targets per lane, plain form
