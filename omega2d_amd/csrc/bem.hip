@@ -288,16 +288,20 @@ int bem_factor_dev(Device* d, hipStream_t s, int64_t n, const double* A, int64_t
 }
 
 // Solves A x = b with the kept factors. A matrix that is solved against again and again (a
-// body that does not move keeps its matrix for the whole run) is worth inverting: the second
-// solve with the same factors builds A^-1 from them (rocsolver getri, about the cost of one more
-// factorisation) and every solve from then on is one dgemv, an HBM-bound read of n^2 doubles
-// instead of two latency-bound triangular solves.
+// body that does not move keeps its matrix for the whole run) is worth inverting: the fourth
+// solve with the same factors builds A^-1 from them (rocsolver getri, about half the cost of the
+// factorisation at order 2049) and every solve from then on is one dgemv, an HBM-bound read of
+// n^2 doubles instead of two latency-bound triangular solves (1.1 ms at order 2049). A body in
+// relative motion is refactored every step and solved once per Runge-Kutta stage, at most three
+// times, so its factors never pay for an inverse.
+constexpr int kSolvesBeforeInverse = 3;
+
 int bem_solve_dev(Device* d, hipStream_t s, int64_t n, const double* b, double* x) {
   if (!d->lu_ready || d->lu_n != n)
     return fail(O2D_ERR_STATE, "no LU factors of size %lld (factor first)", (long long)n);
   rocblas_handle h;
   O2D_TRY(blas_of(d, s, &h));
-  if (!d->inv_ready && d->lu_solves >= 1) {
+  if (!d->inv_ready && d->lu_solves >= kSolvesBeforeInverse) {
     void* inv;
     O2D_TRY(slot(d, kInv, sizeof(double) * n * n, s, &inv));
     O2D_HIP(hipMemcpyAsync(inv, d->buf[kLU], sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
