@@ -83,6 +83,53 @@ def test_chunked_sources_few_targets(core):
 
 
 @pytest.mark.parametrize("core", CORES)
+@pytest.mark.parametrize("precision", ["split", "fp64"])
+@pytest.mark.parametrize("nt", [1, 3, 300])
+def test_many_source_chunks_for_few_targets(core, precision, nt):
+    """300,000 sources on 1, 3 or 300 targets: the launch splits the sources into hundreds of
+    chunks whose partial sums are added in order.
+
+    The split form's bar is 1e-6 in relative L2 over many targets. With one or three targets the
+    bar is 1e-5: the reference's Gaussian core jumps by x/2 = 5e-4 (relative) where
+    x = d^2/sigma^2 crosses 0.001 (src/CoreFunc.h:117-131), and a pair whose fp32 x rounds across
+    that point moves a lone target's sum by a few 1e-6."""
+    rng = np.random.default_rng(nt)
+    sx, sy, ss, sr = particles(rng, 300000, rad=(0.005, 0.05))
+    tx, ty, _, tr = particles(rng, nt, rad=(0.005, 0.05))
+    got = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr, core=core, precision=precision)
+    ref = O.points_on_points(core, sx, sy, ss, sr, tx, ty, tr)
+    bar = 1e-12 if precision == "fp64" else (1e-6 if nt >= 300 else 1e-5)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= bar, (core, rel_l2(g, r))
+
+
+@pytest.mark.parametrize("core", CORES)
+@pytest.mark.parametrize("results", ["vel", "vort"])
+def test_uniform_target_radius_fold_is_bit_identical(core, results):
+    """When every target of a 1,024-target workgroup has the same radius, the split kernel adds
+    that radius term to each staged source once instead of per pair: the same fp32 sum, so the
+    same bits. Changing one target's radius in the first workgroup takes that workgroup off the
+    fold and leaves every other target's result unchanged, bit for bit."""
+    rng = np.random.default_rng(17)
+    sx, sy, ss, _ = particles(rng, 5000)
+    sr = np.full(5000, 0.02)
+    tx, ty, _, _ = particles(rng, 2048)
+    tr = np.full(2048, 0.02)
+    a = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr, core=core, results=results,
+                             precision="split")
+    tr2 = tr.copy()
+    tr2[1023] = 0.03
+    b = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr2, core=core, results=results,
+                             precision="split")
+    keep = np.r_[0:1023, 1024:2048]
+    for ga, gb in zip(a, b):
+        assert np.array_equal(ga[keep], gb[keep]), core
+    ref = O.points_on_points(core, sx, sy, ss, sr, tx, ty, tr2, results)
+    for g, r in zip(b, ref):
+        assert rel_l2(g, r) <= 1e-6, (core, rel_l2(g, r))
+
+
+@pytest.mark.parametrize("core", CORES)
 @pytest.mark.parametrize("particle_targets", [False, True])
 def test_split_fp32_error_bound(core, particle_targets):
     # clustered sources (a Lamb-Oseen-like blob plus a uniform background), mixed-sign strengths
