@@ -349,6 +349,14 @@ __device__ __forceinline__ double warp_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
+__device__ __forceinline__ double warp_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float warp_maxf(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
 
 // per-block min/max of x and y -> part[4*block + {0,1,2,3}] = xmin, ymin, xmax, ymax
 __global__ __launch_bounds__(kBlock) void bbox_partial_kernel(int64_t n, const double* __restrict__ x,
@@ -457,63 +465,54 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(int64_t n, const double*
 // one thread per group of the sorted sources: fp64 centroid of the group's real sources, then
 // the largest length of their fp32 offsets (as the rebase kernel rounds them) and the largest
 // radius term; padding sources (zero strength, offset 0) need no entry
+// One wave per group of 64 sources (in Morton order): the fp64 centroid (a wave sum in a fixed
+// order), the fp32 offsets from it, the largest offset length and radius term, and the staged
+// source record (x - cx, y - cy, strength, radius term). Padding past ns is a zero-strength
+// record at the centroid with radius term 1; a group with no sources has centroid (0, 0) and
+// zero radius and radius term.
 template <int CORE>
 __global__ __launch_bounds__(kBlock) void group_kernel(int64_t ns, int64_t ngroups,
                                                        const uint32_t* __restrict__ order,
                                                        const double* __restrict__ sx,
                                                        const double* __restrict__ sy,
+                                                       const double* __restrict__ ss,
                                                        const double* __restrict__ sr,
-                                                       GroupInfo* __restrict__ grp) {
-  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (g >= ngroups) return;
-  const int64_t b = g * kGroup;
-  const int64_t e = b + kGroup < ns ? b + kGroup : ns;
+                                                       GroupInfo* __restrict__ grp,
+                                                       float4* __restrict__ out) {
+  static_assert(kGroup == 64, "one wave per group");
+  const int64_t g = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroups) return;  // the whole wave
+  const int64_t j = g * kGroup + lane;
+  const bool valid = j < ns;
+  const int64_t left = ns - g * kGroup;
+  const int cnt = left <= 0 ? 0 : (left < kGroup ? (int)left : kGroup);
+  uint32_t o = 0;
+  double x = 0.0, y = 0.0;
+  if (valid) {
+    o = order[j];
+    x = sx[o];
+    y = sy[o];
+  }
   double cx = 0.0, cy = 0.0;
   float rad = 0.0f, wmax = 0.0f;
-  if (e > b) {
-    for (int64_t j = b; j < e; ++j) {
-      const uint32_t o = order[j];
-      cx += sx[o];
-      cy += sy[o];
+  float4 rec = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  if (cnt > 0) {
+    const double inv = 1.0 / (double)cnt;
+    cx = warp_sum(x) * inv;
+    cy = warp_sum(y) * inv;
+    if (valid) {
+      const float ox = (float)(x - cx), oy = (float)(y - cy);
+      const float w = (float)rterm<CORE>(sr[o]);
+      rec = make_float4(ox, oy, (float)ss[o], w);
+      rad = __builtin_sqrtf(__builtin_fmaf(ox, ox, oy * oy));
+      wmax = w;
     }
-    const double inv = 1.0 / (double)(e - b);
-    cx *= inv;
-    cy *= inv;
-    for (int64_t j = b; j < e; ++j) {
-      const uint32_t o = order[j];
-      const float ox = (float)(sx[o] - cx), oy = (float)(sy[o] - cy);
-      rad = fmaxf(rad, __builtin_sqrtf(__builtin_fmaf(ox, ox, oy * oy)));
-      wmax = fmaxf(wmax, (float)rterm<CORE>(sr[o]));
-    }
+    rad = warp_maxf(rad);
+    wmax = warp_maxf(wmax);
   }
-  GroupInfo gi;
-  gi.cx = cx;
-  gi.cy = cy;
-  gi.rad = rad;
-  gi.wmax = wmax;
-  gi.pad0 = gi.pad1 = 0.0f;
-  grp[g] = gi;
-}
-
-template <int CORE>
-__global__ __launch_bounds__(kBlock) void rebase_kernel(int64_t ns, int64_t npad,
-                                                        const uint32_t* __restrict__ order,
-                                                        const double* __restrict__ sx,
-                                                        const double* __restrict__ sy,
-                                                        const double* __restrict__ ss,
-                                                        const double* __restrict__ sr,
-                                                        const GroupInfo* __restrict__ grp,
-                                                        float4* __restrict__ out) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= npad) return;
-  if (j < ns) {
-    const uint32_t o = order[j];
-    const GroupInfo& c = grp[j / kGroup];
-    out[j] = make_float4((float)(sx[o] - c.cx), (float)(sy[o] - c.cy), (float)ss[o],
-                         (float)rterm<CORE>(sr[o]));
-  } else {
-    out[j] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-  }
+  out[j] = rec;
+  if (lane == 0) grp[g] = GroupInfo{cx, cy, rad, wmax, 0.0f, 0.0f};
 }
 
 // ---------------------------------------------------------------------------- launch helpers
@@ -602,10 +601,10 @@ template <int CORE>
 void launch_prep(int64_t ngroups, int64_t npad, hipStream_t s, int64_t ns, const uint32_t* order,
                  const double* sx, const double* sy, const double* ss, const double* sr,
                  GroupInfo* grp, float4* out) {
-  group_kernel<CORE><<<(unsigned)((ngroups + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-      ns, ngroups, order, sx, sy, sr, grp);
-  rebase_kernel<CORE><<<(unsigned)((npad + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-      ns, npad, order, sx, sy, ss, sr, grp, out);
+  (void)npad;  // ngroups * kGroup
+  constexpr int kGroupsPerBlock = kBlock / 64;
+  group_kernel<CORE><<<(unsigned)((ngroups + kGroupsPerBlock - 1) / kGroupsPerBlock), kBlock, 0,
+                       s>>>(ns, ngroups, order, sx, sy, ss, sr, grp, out);
 }
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
