@@ -1,6 +1,6 @@
 """Per-configuration measurements beside the headline bench line (one JSON object per config).
 
-    python -m omega2d_amd.bench_configs [--configs c1,c2tree,c3,c4,c5] [--out FILE]
+    python -m omega2d_amd.bench_configs [--configs c1,c2host,c2tree,c3,c4,c5] [--out FILE]
 
 C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update;
     the Gaussian vorticity of the disc on itself
@@ -263,6 +263,37 @@ def c5():
             "split_err_vs_fp64_16k": err}
 
 
+def c2host():
+    """The C2 evaluation (1M particles on themselves plus 65,536 tracers, split form) through the
+    host entry point, as the reference calls it: host arrays in, copies over PCIe both ways,
+    results in host arrays on return. Reported beside the same evaluation on device-resident
+    arrays."""
+    import omega2d_amd as o2d
+    w = wl.c2_blobs()
+    n, nt = len(w["x"]), len(w["tx"])
+    pairs = float(n) * n + float(nt) * n
+
+    def host_step():
+        o2d.points_on_points(w["x"], w["y"], w["s"], w["r"], w["x"], w["y"], w["r"], core="gauss",
+                             precision="split")
+        o2d.points_on_points(w["x"], w["y"], w["s"], w["r"], w["tx"], w["ty"], core="gauss",
+                             precision="split")
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    tx, ty = T(w["tx"]), T(w["ty"])
+
+    def dev_step():
+        dv.points_on_points(x, y, s, r, x, y, r, core="gauss", precision="split")
+        dv.points_on_points(x, y, s, r, tx, ty, None, core="gauss", precision="split")
+    host_step()
+    th = timed(host_step, reps=3)
+    dev_step()
+    td = timed(dev_step, reps=3)
+    return {"config": "C2host", "name": w["name"], "particles": n, "tracers": nt,
+            "host_entry_ms": round(th * 1e3, 2), "host_entry_Gint_s": round(pairs / th / 1e9, 1),
+            "device_entry_ms": round(td * 1e3, 2),
+            "device_entry_Gint_s": round(pairs / td / 1e9, 1)}
+
+
 def c2tree():
     """The 1M blob workload (1,048,576 particles on themselves + 65,536 tracers, Gaussian) with
     the treecode: build and walk kernel times, the equivalent direct-sum pair rate, and the error
@@ -307,7 +338,7 @@ def main():
     ap.add_argument("--configs", default="c1,c3,c4,c5")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
-    fns = {"c1": c1, "c2tree": c2tree, "c3": c3, "c4": c4, "c5": c5}
+    fns = {"c1": c1, "c2host": c2host, "c2tree": c2tree, "c3": c3, "c4": c4, "c5": c5}
     out = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()
