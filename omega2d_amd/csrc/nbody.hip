@@ -462,9 +462,6 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(int64_t n, const double*
   order[i] = (uint32_t)i;
 }
 
-// one thread per group of the sorted sources: fp64 centroid of the group's real sources, then
-// the largest length of their fp32 offsets (as the rebase kernel rounds them) and the largest
-// radius term; padding sources (zero strength, offset 0) need no entry
 // One wave per group of 64 sources (in Morton order): the fp64 centroid (a wave sum in a fixed
 // order), the fp32 offsets from it, the largest offset length and radius term, and the staged
 // source record (x - cx, y - cy, strength, radius term). Padding past ns is a zero-strength
