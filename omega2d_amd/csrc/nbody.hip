@@ -65,7 +65,11 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
     const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tr,
     int64_t chunk, double* __restrict__ ou, double* __restrict__ ov, double* __restrict__ ow,
     int direct) {
+  // the Gaussian's velocity sum takes a loop without the per-pair cutoff test for a tile that
+  // lies beyond the compact cutoff of every target of the wave (d^2 > 16 w for all its pairs)
+  constexpr bool kFar = CORE == O2D_CORE_GAUSSIAN && RES == O2D_VELONLY;
   __shared__ Src64 tile[kTile];
+  __shared__ double tbox[kBlock / 64][5];  // per wave: x min/max, y min/max, largest radius term
   const int tid = threadIdx.x;
   const int64_t tbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
 
@@ -90,15 +94,70 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
       s.y = sy[j];
       s.s = ss[j];
       s.w = rterm<CORE>(sr[j]);
-    } else {  // zero-strength padding with a positive radius term: contributes exactly 0
-      s.x = 0.0;
-      s.y = 0.0;
+    } else {  // zero strength at the chunk's last source: contributes exactly 0
+      s.x = sx[je - 1];
+      s.y = sy[je - 1];
       s.s = 0.0;
-      s.w = 1.0;
+      s.w = rterm<CORE>(sr[je - 1]);
     }
-    __syncthreads();
+    if constexpr (kFar) {
+      double x0 = s.x, x1 = s.x, y0 = s.y, y1 = s.y, wm = s.w;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        x0 = fmin(x0, __shfl_xor(x0, o, 64));
+        x1 = fmax(x1, __shfl_xor(x1, o, 64));
+        y0 = fmin(y0, __shfl_xor(y0, o, 64));
+        y1 = fmax(y1, __shfl_xor(y1, o, 64));
+        wm = fmax(wm, __shfl_xor(wm, o, 64));
+      }
+      __syncthreads();
+      if ((tid & 63) == 0) {
+        double* b = tbox[tid >> 6];
+        b[0] = x0;
+        b[1] = x1;
+        b[2] = y0;
+        b[3] = y1;
+        b[4] = wm;
+      }
+    } else {
+      __syncthreads();
+    }
     tile[tid] = s;
     __syncthreads();
+    if constexpr (kFar) {
+      double x0 = tbox[0][0], x1 = tbox[0][1], y0 = tbox[0][2], y1 = tbox[0][3], wm = tbox[0][4];
+#pragma unroll
+      for (int k = 1; k < kBlock / 64; ++k) {
+        x0 = fmin(x0, tbox[k][0]);
+        x1 = fmax(x1, tbox[k][1]);
+        y0 = fmin(y0, tbox[k][2]);
+        y1 = fmax(y1, tbox[k][3]);
+        wm = fmax(wm, tbox[k][4]);
+      }
+      bool far = true;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        const double ex = fmax(fmax(x0 - px[t], px[t] - x1), 0.0);
+        const double ey = fmax(fmax(y0 - py[t], py[t] - y1), 0.0);
+        // the nearest pair's d^2 clears 16 w by far more than its rounding
+        far = far && ex * ex + ey * ey > 16.0 * (wm + pw[t]) * (1.0 + 1e-12);
+      }
+      if (__all(far)) {
+#pragma unroll 4
+        for (int k = 0; k < kTile; ++k) {
+          const Src64 q = tile[k];
+#pragma unroll
+          for (int t = 0; t < TPL; ++t) {
+            const double dx = px[t] - q.x;
+            const double dy = py[t] - q.y;
+            const double g = q.s * rcp64(__builtin_fma(dx, dx, dy * dy));
+            au[t] = __builtin_fma(-g, dy, au[t]);
+            av[t] = __builtin_fma(g, dx, av[t]);
+          }
+        }
+        continue;
+      }
+    }
 #pragma unroll 4
     for (int k = 0; k < kTile; ++k) {
       const Src64 q = tile[k];
