@@ -58,6 +58,45 @@ struct alignas(16) Src64 {
 
 // ---------------------------------------------------------------------------- plain fp64
 
+// the pair loop over one staged tile; TR adds each target's radius term per pair
+template <int CORE, int RES, bool TR, int TPL>
+__device__ __forceinline__ void tile_pairs64(const Src64* __restrict__ tile, const double* px,
+                                             const double* py, const double* pw, double* au,
+                                             double* av, double* aw) {
+#pragma unroll 4
+  for (int k = 0; k < kTile; ++k) {
+    const Src64 q = tile[k];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      const double dx = px[t] - q.x;
+      const double dy = py[t] - q.y;
+      const double d2 = __builtin_fma(dx, dx, dy * dy);
+      const double w = TR ? q.w + pw[t] : q.w;
+      if constexpr (RES == O2D_VELONLY) {
+        const double g = q.s * core_k64<CORE>(d2, w);
+        au[t] = __builtin_fma(-g, dy, au[t]);
+        av[t] = __builtin_fma(g, dx, av[t]);
+      } else {
+        double kk, bbb;
+        core_kg64<CORE>(d2, w, kk, bbb);
+        kk *= q.s;
+        bbb *= q.s;
+        au[t] = __builtin_fma(-kk, dy, au[t]);
+        av[t] = __builtin_fma(kk, dx, av[t]);
+        if constexpr (RES == O2D_VELANDVORT) {
+          // (bbb dx^2 + K) - (-bbb dy^2 - K)   src/Kernels.h:176-178
+          aw[t] += __builtin_fma(bbb, d2, 2.0 * kk);
+        } else {
+          // sqrt(dudx^2 + asym^2/4)           src/Kernels.h:225-230
+          const double dudx = -bbb * dx * dy;
+          const double asym = bbb * (dx * dx - dy * dy);
+          aw[t] += sqrt(__builtin_fma(dudx, dudx, 0.25 * asym * asym));
+        }
+      }
+    }
+  }
+}
+
 template <int CORE, int RES, bool TR, int TPL>
 __global__ __launch_bounds__(kBlock) void nbody64_kernel(
     int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
@@ -83,6 +122,19 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
     pw[t] = TR ? rterm<CORE>(tr[ii]) : 0.0;
     au[t] = av[t] = aw[t] = 0.0;
   }
+  // when every target of the block has the same radius term w0, it is added to each source's
+  // term once as the tile is staged, and the pair loop runs without the per-pair add (the same
+  // fp64 sum, so the same result)
+  double w0 = 0.0;
+  bool uni = false;
+  if constexpr (TR) {
+    const int64_t i0 = (int64_t)blockIdx.x * (kBlock * TPL);
+    w0 = rterm<CORE>(tr[i0 < nt ? i0 : nt - 1]);
+    bool same = true;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) same = same && pw[t] == w0;
+    uni = __syncthreads_and(same) != 0;
+  }
 
   const int64_t jb = (int64_t)blockIdx.y * chunk;
   const int64_t je = jb + chunk < ns ? jb + chunk : ns;
@@ -100,6 +152,7 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
       s.s = 0.0;
       s.w = rterm<CORE>(sr[je - 1]);
     }
+    if (uni) s.w += w0;
     if constexpr (kFar) {
       double x0 = s.x, x1 = s.x, y0 = s.y, y1 = s.y, wm = s.w;
 #pragma unroll
@@ -140,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
         const double ex = fmax(fmax(x0 - px[t], px[t] - x1), 0.0);
         const double ey = fmax(fmax(y0 - py[t], py[t] - y1), 0.0);
         // the nearest pair's d^2 clears 16 w by far more than its rounding
-        far = far && ex * ex + ey * ey > 16.0 * (wm + pw[t]) * (1.0 + 1e-12);
+        far = far && ex * ex + ey * ey > 16.0 * (uni ? wm : wm + pw[t]) * (1.0 + 1e-12);
       }
       if (__all(far)) {
 #pragma unroll 4
@@ -158,38 +211,10 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
         continue;
       }
     }
-#pragma unroll 4
-    for (int k = 0; k < kTile; ++k) {
-      const Src64 q = tile[k];
-#pragma unroll
-      for (int t = 0; t < TPL; ++t) {
-        const double dx = px[t] - q.x;
-        const double dy = py[t] - q.y;
-        const double d2 = __builtin_fma(dx, dx, dy * dy);
-        const double w = TR ? q.w + pw[t] : q.w;
-        if constexpr (RES == O2D_VELONLY) {
-          const double g = q.s * core_k64<CORE>(d2, w);
-          au[t] = __builtin_fma(-g, dy, au[t]);
-          av[t] = __builtin_fma(g, dx, av[t]);
-        } else {
-          double kk, bbb;
-          core_kg64<CORE>(d2, w, kk, bbb);
-          kk *= q.s;
-          bbb *= q.s;
-          au[t] = __builtin_fma(-kk, dy, au[t]);
-          av[t] = __builtin_fma(kk, dx, av[t]);
-          if constexpr (RES == O2D_VELANDVORT) {
-            // (bbb dx^2 + K) - (-bbb dy^2 - K)   src/Kernels.h:176-178
-            aw[t] += __builtin_fma(bbb, d2, 2.0 * kk);
-          } else {
-            // sqrt(dudx^2 + asym^2/4)           src/Kernels.h:225-230
-            const double dudx = -bbb * dx * dy;
-            const double asym = bbb * (dx * dx - dy * dy);
-            aw[t] += sqrt(__builtin_fma(dudx, dudx, 0.25 * asym * asym));
-          }
-        }
-      }
-    }
+    if (uni)
+      tile_pairs64<CORE, RES, false, TPL>(tile, px, py, pw, au, av, aw);
+    else
+      tile_pairs64<CORE, RES, TR, TPL>(tile, px, py, pw, au, av, aw);
   }
 
 #pragma unroll
