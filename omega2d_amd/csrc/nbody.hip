@@ -36,8 +36,10 @@ constexpr int kGroup = 64;   // sources sharing one centroid in the split form
 #ifndef O2D_TPL32
 #define O2D_TPL32 4
 #endif
+// 5 waves per SIMD give the split kernels 96 VGPRs and no scratch; at 8 they spilled about
+// 100 B per lane (written out to memory by every wave) and ran no faster
 #ifndef O2D_WAVES32
-#define O2D_WAVES32 8
+#define O2D_WAVES32 5
 #endif
 // source chunks: enough workgroups for this many per CU (each chunk at least kMinChunkTiles
 // tiles). Many short-lived workgroups keep every SIMD issuing to the end of the launch; with
