@@ -105,28 +105,50 @@ def test_many_source_chunks_for_few_targets(core, precision, nt):
 
 @pytest.mark.parametrize("core", CORES)
 @pytest.mark.parametrize("results", ["vel", "vort"])
-def test_uniform_target_radius_fold_is_bit_identical(core, results):
-    """When every target of a 1,024-target workgroup has the same radius, the split kernel adds
-    that radius term to each staged source once instead of per pair: the same fp32 sum, so the
-    same bits. Changing one target's radius in the first workgroup takes that workgroup off the
-    fold and leaves every other target's result unchanged, bit for bit."""
+@pytest.mark.parametrize("precision", ["split", "fp64"])
+def test_uniform_target_radius_fold_is_bit_identical(core, results, precision):
+    """When every target of a workgroup (1,024 targets split, 512 fp64) has the same radius, the
+    kernel adds that radius term to each staged source once instead of per pair: the same sum,
+    so the same bits. Changing one target's radius in the first workgroup takes that workgroup
+    off the fold and leaves every other target's result unchanged, bit for bit."""
     rng = np.random.default_rng(17)
     sx, sy, ss, _ = particles(rng, 5000)
     sr = np.full(5000, 0.02)
     tx, ty, _, _ = particles(rng, 2048)
     tr = np.full(2048, 0.02)
     a = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr, core=core, results=results,
-                             precision="split")
+                             precision=precision)
     tr2 = tr.copy()
-    tr2[1023] = 0.03
+    tr2[511] = 0.03
     b = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr2, core=core, results=results,
-                             precision="split")
-    keep = np.r_[0:1023, 1024:2048]
+                             precision=precision)
+    keep = np.r_[0:511, 1024:2048]
     for ga, gb in zip(a, b):
         assert np.array_equal(ga[keep], gb[keep]), core
     ref = O.points_on_points(core, sx, sy, ss, sr, tx, ty, tr2, results)
     for g, r in zip(b, ref):
-        assert rel_l2(g, r) <= 1e-6, (core, rel_l2(g, r))
+        assert rel_l2(g, r) <= (1e-6 if precision == "split" else 1e-12), (core, rel_l2(g, r))
+
+
+@pytest.mark.parametrize("particle_targets", [False, True])
+def test_fp64_gaussian_tiles_beyond_the_cutoff(particle_targets):
+    """Sources stored cluster by cluster, so that whole 256-source tiles lie beyond the Gaussian
+    cutoff of every target of a wave and take the loop without the per-pair cutoff test, next to
+    tiles that do not (targets inside one of the clusters); target radii vary."""
+    rng = np.random.default_rng(23)
+    centres = [(-3.0, -3.0), (3.0, -2.0), (-2.5, 3.0), (0.0, 0.0)]
+    per = 2048
+    sx = np.concatenate([c[0] + 0.2 * rng.normal(size=per) for c in centres])
+    sy = np.concatenate([c[1] + 0.2 * rng.normal(size=per) for c in centres])
+    ss = rng.normal(0, 1e-3, len(sx))
+    sr = rng.uniform(0.005, 0.02, len(sx))
+    tx = np.concatenate([3.0 + 0.3 * rng.normal(size=900), 0.2 * rng.normal(size=300)])
+    ty = np.concatenate([3.0 + 0.3 * rng.normal(size=900), 0.2 * rng.normal(size=300)])
+    tr = rng.uniform(0.005, 0.03, len(tx)) if particle_targets else None
+    got = o2d.points_on_points(sx, sy, ss, sr, tx, ty, tr, core="gauss", precision="fp64")
+    ref = O.points_on_points("gauss", sx, sy, ss, sr, tx, ty, tr)
+    for g, r in zip(got, ref):
+        assert rel_l2(g, r) <= 1e-12, rel_l2(g, r)
 
 
 @pytest.mark.parametrize("core", CORES)
