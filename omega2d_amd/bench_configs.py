@@ -1,6 +1,6 @@
 """Per-configuration measurements beside the headline bench line (one JSON object per config).
 
-    python -m omega2d_amd.bench_configs [--configs c1,c2host,c2tree,c3,c4,c5] [--out FILE]
+    python -m omega2d_amd.bench_configs [--configs c1,c2host,c2tree,c3,c3step,c4,c5] [--out FILE]
 
 C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update;
     the Gaussian vorticity of the disc on itself
@@ -263,6 +263,41 @@ def c5():
             "split_err_vs_fp64_16k": err}
 
 
+def c3step(nsteps=10):
+    """Whole Ralston RK2 advection steps of C3 through the C++ host mirror (omega2d_amd/host,
+    src/Convection.h:390-466): each of the two stages solves the BEM (RHS from the wake, the
+    2049^2 vortex-only system), sums panels and particles on the 200,000 wake particles, moves
+    them and clears the inner layer (method 1). The driver passes host arrays, so every call
+    copies its inputs over PCIe, as the reference's call would. Wall time per step after three
+    untimed steps, split and fp64 pair sums."""
+    import os
+    import subprocess
+    import tempfile
+    from . import build as B
+    w = wl.c3_circle_wake()
+    npan, nw = len(w["idx"]) // 2, len(w["x"])
+    res = {"config": "C3step", "name": w["name"], "panels": npan, "wake": nw, "core": "v2",
+           "unknowns_per_panel": 1, "steps_timed": nsteps}
+    with tempfile.TemporaryDirectory() as tmp:
+        for prec, pid in (("split", 2), ("fp64", 1)):
+            inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+            with open(inp, "wb") as f:
+                f.write(np.array([npan, len(w["nx"]), nw, 1, 4, 2, nsteps, pid],
+                                 np.int64).tobytes())
+                f.write(np.array([w["fs"][0], w["fs"][1], 0.01, 0.5]).tobytes())
+                for a in (w["nx"].astype(np.float32), w["ny"].astype(np.float32),
+                          w["idx"].astype(np.uint32), w["x"], w["y"], w["s"], w["r"]):
+                    f.write(np.ascontiguousarray(a).tobytes())
+            r = subprocess.run([B.HOST_TEST_BIN, "bemtime", inp, out], capture_output=True,
+                               text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(r.stderr[-2000:])
+            d = np.fromfile(out, dtype=np.float64)
+            assert d.size == 1 + 2 * nw and np.all(np.isfinite(d))
+            res[prec] = {"ms_per_step": round(float(d[0]), 3)}
+    return res
+
+
 def c2host():
     """The C2 evaluation (1M particles on themselves plus 65,536 tracers, split form) through the
     host entry point, as the reference calls it: host arrays in, copies over PCIe both ways,
@@ -338,7 +373,8 @@ def main():
     ap.add_argument("--configs", default="c1,c3,c4,c5")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
-    fns = {"c1": c1, "c2host": c2host, "c2tree": c2tree, "c3": c3, "c4": c4, "c5": c5}
+    fns = {"c1": c1, "c2host": c2host, "c2tree": c2tree, "c3": c3, "c3step": c3step,
+           "c4": c4, "c5": c5}
     out = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()

@@ -16,6 +16,7 @@
 //                  in place of <out.bin> (no GPU work)
 //
 // Input: int64 header [mode-specific counts], then float64 arrays (node coordinates as float32).
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -140,6 +141,43 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
     out.vec(p.get_pos()[0]);
     out.vec(p.get_pos()[1]);
   }
+}
+
+// Wall time of whole advection steps (src/Convection.h:249-598: every stage solves the BEM,
+// sums panels and particles on the wake, moves it and clears the inner layer), after three
+// untimed steps (the matrix is built and factored once; the fourth solve with the same factors
+// builds the inverse). header: npan, nnodes, nwake, nunk, core, order, nsteps, precision; then
+// as for run_bem. Output: milliseconds per step, then the wake positions.
+void run_bem_time(Reader& in, Writer& out) {
+  const int64_t npan = in.i64(), nnodes = in.i64(), nw = in.i64(), nunk = in.i64(),
+                core = in.i64(), order = in.i64(), nsteps = in.i64(), prec = in.i64();
+  const double fsx = in.f64(), fsy = in.f64(), dt = in.f64(), ips = in.f64();
+  const auto nx = in.vec<float>(nnodes), ny = in.vec<float>(nnodes);
+  const auto idx = in.vec<uint32_t>(2 * npan);
+  auto wx = in.vec<double>(nw), wy = in.vec<double>(nw), ws = in.vec<double>(nw),
+       wr = in.vec<double>(nw);
+  auto ground = std::make_shared<Body>();
+  Surfaces<double> circle(nnodes, nx.data(), ny.data(), npan, idx.data(), reactive, bodybound,
+                          ground);
+  if (nunk == 1) circle.set_vortex_only();
+  std::vector<Collection<double>> vort, bdry, fldpt;
+  vort.emplace_back(Points<double>(nw, wx.data(), wy.data(), ws.data(), wr.data()));
+  bdry.emplace_back(circle);
+  ExecEnv env(true, direct, gpu_hip);
+  env.set_core((int32_t)core);
+  env.set_precision((int32_t)prec);
+  Convection<double, double, Int> conv(env, (int)order);
+  BEM<double, Int> bem;
+  const std::array<double, Dimensions> fs = {fsx, fsy};
+  double t = 0.0;
+  for (int k = 0; k < 3; ++k, t += dt) conv.advect(t, dt, fs, ips, vort, bdry, fldpt, bem);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int64_t k = 0; k < nsteps; ++k, t += dt) conv.advect(t, dt, fs, ips, vort, bdry, fldpt, bem);
+  const auto t1 = std::chrono::steady_clock::now();
+  out.f64(std::chrono::duration<double, std::milli>(t1 - t0).count() / (double)nsteps);
+  const auto& p = std::get<Points<double>>(vort[0]);
+  out.vec(p.get_pos()[0]);
+  out.vec(p.get_pos()[1]);
 }
 
 // header: npanA, nnodesA, npanB, nnodesB, nwake, nunk, core; then fsx, fsy, nodes of A
@@ -366,7 +404,7 @@ void run_vtk(Reader& in, const std::string& dir) {
 
 int main(int argc, char** argv) {
   if (argc != 4) {
-    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bem2|bemmove|bemgalilean|vort|motion|vtk> <in.bin> <out.bin|dir>\n";
+    std::cerr << "usage: o2d_host_driver <rk|rktree|bem|bemstep|bemvort|bemtime|bem2|bemmove|bemgalilean|vort|motion|vtk> <in.bin> <out.bin|dir>\n";
     return 2;
   }
   try {
@@ -398,6 +436,8 @@ int main(int argc, char** argv) {
       run_bem_move(in, out);
     } else if (mode == "bemgalilean") {
       run_bem_galilean(in, out);
+    } else if (mode == "bemtime") {
+      run_bem_time(in, out);
 
     } else {
       std::cerr << "unknown mode " << mode << "\n";
