@@ -93,9 +93,15 @@ int o2d_hip_points_on_points_d(const o2d_env* env, int64_t ns, const double* sx,
   O2D_TRY(upload<double>(d, kSrcY, sy, ns, s, &dsy));
   O2D_TRY(upload<double>(d, kSrcS, sstr, ns, s, &dss));
   O2D_TRY(upload<double>(d, kSrcR, srad, ns, s, &dsr));
-  O2D_TRY(upload<double>(d, kTrgX, tx, nt, s, &dtx));
-  O2D_TRY(upload<double>(d, kTrgY, ty, nt, s, &dty));
-  if (trad) O2D_TRY(upload<double>(d, kTrgR, trad, nt, s, &dtr));
+  // a collection acting on itself (the usual particle case) passes the same host arrays as
+  // sources and targets: their device copies serve both
+  const bool self = nt == ns;
+  if (self && tx == sx) dtx = dsx;
+  else O2D_TRY(upload<double>(d, kTrgX, tx, nt, s, &dtx));
+  if (self && ty == sy) dty = dsy;
+  else O2D_TRY(upload<double>(d, kTrgY, ty, nt, s, &dty));
+  if (trad && self && trad == srad) dtr = dsr;
+  else if (trad) O2D_TRY(upload<double>(d, kTrgR, trad, nt, s, &dtr));
   O2D_TRY(upload<double>(d, kOutU, tu, nt, s, &du));
   O2D_TRY(upload<double>(d, kOutV, tv, nt, s, &dv));
   double* haux = env->results == O2D_VELANDVORT ? tw : (env->results == O2D_VELANDSHEAR ? te : nullptr);
@@ -124,9 +130,13 @@ int o2d_hip_points_on_points_f(const o2d_env* env, int64_t ns, const float* sx, 
   O2D_TRY(upload_f2d(d, kStageF1, kSrcY, sy, ns, s, &dsy));
   O2D_TRY(upload_f2d(d, kStageF2, kSrcS, sstr, ns, s, &dss));
   O2D_TRY(upload_f2d(d, kStageF3, kSrcR, srad, ns, s, &dsr));
-  O2D_TRY(upload_f2d(d, kStageF4, kTrgX, tx, nt, s, &dtx));
-  O2D_TRY(upload_f2d(d, kStageF5, kTrgY, ty, nt, s, &dty));
-  if (trad) O2D_TRY(upload_f2d(d, kStageF6, kTrgR, trad, nt, s, &dtr));
+  const bool self = nt == ns;
+  if (self && tx == sx) dtx = dsx;
+  else O2D_TRY(upload_f2d(d, kStageF4, kTrgX, tx, nt, s, &dtx));
+  if (self && ty == sy) dty = dsy;
+  else O2D_TRY(upload_f2d(d, kStageF5, kTrgY, ty, nt, s, &dty));
+  if (trad && self && trad == srad) dtr = dsr;
+  else if (trad) O2D_TRY(upload_f2d(d, kStageF6, kTrgR, trad, nt, s, &dtr));
   O2D_TRY(upload_f2d(d, kStageF7, kOutU, tu, nt, s, &du));
   O2D_TRY(upload_f2d(d, kStageF8, kOutV, tv, nt, s, &dv));
   float* haux = env->results == O2D_VELANDVORT ? tw : (env->results == O2D_VELANDSHEAR ? te : nullptr);
