@@ -250,8 +250,10 @@ def main():
 
     # ---- CPU baseline: the reference's own OpenMP path (S=float, A=double), bounded sample
     cpu = None
+    cpu_rows = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, args.cpu_seconds)
+        cpu_rows = cpu_reference_c3(args.cpu_seconds / 10.0)
 
     out = {
         "metric": METRIC,
@@ -291,6 +293,7 @@ def main():
                      "note": "same evaluation, O(N log N) with fp64 moments; not the headline"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_reference_c3": cpu_rows,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -357,6 +360,64 @@ def cpu_baseline(w, seconds):
     return {"value": round(rate / 1e9, 4), "unit": "G pair-interactions/s", "cores": threads,
             "kind": kind, "sample": f"{m2} of {n} particle targets x {n} sources, {dt:.1f} s; "
                                     f"{label}; OpenMP x86 env (the Vc env cannot be built offline)"}
+
+
+def _bounded_rate(fn, total_units, per_call_units, seconds, start=8):
+    """Units per second of fn(m) (m targets of per_call_units work each), timed on a sample of
+    about `seconds` of CPU work."""
+    m = start
+    t0 = time.perf_counter()
+    fn(m)
+    dt = time.perf_counter() - t0
+    rate = m * per_call_units / max(dt, 1e-9)
+    m = int(min(total_units, max(m, seconds * rate / per_call_units)))
+    t0 = time.perf_counter()
+    fn(m)
+    dt = time.perf_counter() - t0
+    return m, dt, m * per_call_units / dt
+
+
+def cpu_reference_c3(seconds):
+    """The reference's own CPU loops (compiled from its sources by oracle/Makefile, OpenMP) for the
+    per-stage pieces of the C3 workload (2,048 panels, 200,000 wake particles, Vatistas n=2, the
+    reference's compiled-in core), each on a bounded target sample, in (target, source) pairs
+    per second. The library's times for the same pieces come from
+    `python -m omega2d_amd.bench_configs --configs c3`."""
+    try:
+        import oracle as O
+        from omega2d_amd import workloads as wl
+    except Exception as e:
+        log(f"C3 cpu rows skipped: {e}")
+        return None
+    if not O.have_ref():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    w = wl.c3_circle_wake()
+    npan, nw = len(w["idx"]) // 2, len(w["x"])
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    x, y, s, r = f32(w["x"]), f32(w["y"]), f32(w["s"]), f32(w["r"])
+    vs = np.full(npan, 1e-3, np.float32)
+    rows = {}
+    m, dt, rate = _bounded_rate(
+        lambda k: O.ref_points_on_points(x, y, s, r, x[:k], y[:k], r[:k], "vel",
+                                         dtype=np.float32), nw, nw, seconds, start=64)
+    rows["particles_on_particles"] = {"pairs_s": round(rate, 1), "sample": f"{m} targets"}
+    m, dt, rate = _bounded_rate(
+        lambda k: O.ref_panels_on_points(w["nx"], w["ny"], w["idx"], vs, vs, x[:k], y[:k],
+                                         dtype=np.float32), nw, npan, seconds, start=256)
+    rows["panels_on_points"] = {"pairs_s": round(rate, 1), "sample": f"{m} targets"}
+    m, dt, rate = _bounded_rate(
+        lambda k: O.ref_points_on_panels(x, y, s, r, w["nx"], w["ny"], w["idx"][:2 * k],
+                                         dtype=np.float32), npan, nw, seconds, start=4)
+    rows["points_on_panels"] = {"pairs_s": round(rate, 1), "sample": f"{m} panels"}
+    m, dt, rate = _bounded_rate(
+        lambda k: O.ref_nearest_distances(w["nx"], w["ny"], w["idx"], w["x"][:k], w["y"][:k]),
+        nw, npan, seconds, start=256)
+    rows["nearest_distances"] = {"pairs_s": round(rate, 1), "sample": f"{m} targets"}
+    log("C3 reference CPU rows: " + ", ".join(f"{k} {v['pairs_s'] / 1e9:.3f} G/s"
+                                               for k, v in rows.items()))
+    return {"threads": threads, "unit": "(target, source) pairs/s", "workload":
+            "C3: 2,048-panel circle, 200,000 wake particles, Vatistas n=2, S=float", **rows}
 
 
 if __name__ == "__main__":
