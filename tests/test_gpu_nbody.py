@@ -103,6 +103,24 @@ def test_many_source_chunks_for_few_targets(core, precision, nt):
         assert rel_l2(g, r) <= bar, (core, rel_l2(g, r))
 
 
+@pytest.mark.parametrize("core", ["gauss", "rm", "v2"])
+def test_split_form_keeps_its_accuracy_far_from_the_origin(core):
+    """The split form rebases every pair on its source group's fp64 centroid, so moving the whole
+    problem to (1e4, -1e4) leaves its error against the fp64 sum where it was (an fp32 form on
+    absolute coordinates would lose about eight more digits there)."""
+    rng = np.random.default_rng(31)
+    n = 12000
+    sx, sy = 0.3 * rng.normal(size=n), 0.3 * rng.normal(size=n)
+    ss, sr = rng.normal(0, 1e-3, n), np.full(n, 0.01)
+    errs = []
+    for ox, oy in ((0.0, 0.0), (1.0e4, -1.0e4)):
+        x, y = sx + ox, sy + oy
+        got = o2d.points_on_points(x, y, ss, sr, x, y, sr, core=core, precision="split")
+        ref = o2d.points_on_points(x, y, ss, sr, x, y, sr, core=core, precision="fp64")
+        errs.append(rel_l2(np.concatenate(got), np.concatenate(ref)))
+    assert max(errs) <= 1e-6 and errs[1] <= 10.0 * errs[0] + 1e-9, errs
+
+
 @pytest.mark.parametrize("core", CORES)
 @pytest.mark.parametrize("results", ["vel", "vort"])
 @pytest.mark.parametrize("precision", ["split", "fp64"])
