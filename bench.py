@@ -62,8 +62,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "layout (targets sharded, sources all-gathered) is described in DESIGN.md and not "
-              "run here.", file=sys.stderr)
+              "layout (targets sharded, sources all-gathered) is a design that is not run here.",
+              file=sys.stderr)
         return 2
 
     import torch
