@@ -316,8 +316,10 @@ __global__ __launch_bounds__(kBlock) void near_kernel(int64_t np, const PanelRec
       __syncthreads();
       for (int e = threadIdx.x; e < n * kRecDoubles; e += kBlock) tile[e] = src[e];
       __syncthreads();
-      if (act)
+      if (act) {
+#pragma unroll 4
         for (int k = 0; k < n; ++k) scan_panel<MODE>(lp[k], x, y, st);
+      }
     }
     if (!act) continue;
     ++passes;
