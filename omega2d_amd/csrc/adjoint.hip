@@ -20,6 +20,12 @@ namespace o2d {
 
 namespace {
 
+// workgroups per CU the target chunks aim for (32: 4% faster than 8 on 250k sources x 4,096
+// probes, where the 16 probe tiles cap the split)
+#ifndef O2D_WANT_ADJ
+#define O2D_WANT_ADJ 32
+#endif
+
 constexpr int kBlock = 256;
 constexpr int kTile = 256;
 constexpr int kTpl = 2;
@@ -200,7 +206,7 @@ int adjoint_dev(Device* d, hipStream_t s, int core, int64_t ns, const double* sx
   if (ns == 0 || nt == 0) return 0;
   const int64_t swgs = (ns + kBlock * kTpl - 1) / (kBlock * kTpl);
   const int64_t ntiles = (nt + kTile - 1) / kTile;
-  const int64_t want = 8 * (int64_t)d->num_cu;
+  const int64_t want = O2D_WANT_ADJ * (int64_t)d->num_cu;
   int64_t nchunk = swgs >= want ? 1 : std::min<int64_t>((want + swgs - 1) / swgs, ntiles);
   const int64_t per = (ntiles + nchunk - 1) / nchunk;
   nchunk = (ntiles + per - 1) / per;
