@@ -21,6 +21,12 @@
 #include "o2d_runtime.h"
 #include "panel_math.h"
 
+// workgroups per CU the particle chunks of points_on_panels aim for (32: 0.990 -> 0.975 ms on
+// 200k particles x 2,048 panels; 128 took 1.073 ms)
+#ifndef O2D_WANT_RHS
+#define O2D_WANT_RHS 32
+#endif
+
 namespace o2d {
 
 namespace {
@@ -222,7 +228,7 @@ int points_on_panels_dev(Device* d, hipStream_t s, int64_t ns, const double* sx,
   (void)nnodes;
   if (np == 0 || ns == 0) return 0;
   const int64_t pwgs = (np + kPanTile - 1) / kPanTile;
-  const int64_t want = 8 * (int64_t)d->num_cu;
+  const int64_t want = O2D_WANT_RHS * (int64_t)d->num_cu;
   int64_t nchunk = pwgs >= want ? 1 : (want + pwgs - 1) / pwgs;
   const int64_t maxchunk = (ns + kBlock - 1) / kBlock;  // at least one particle per lane
   if (nchunk > maxchunk) nchunk = maxchunk;

@@ -8,6 +8,12 @@
 #include "nbody.h"
 #include "o2d_runtime.h"
 
+// workgroups per CU the source chunks aim for (128: 17.1 -> 13.8 ms for 200k particles on
+// themselves, against 8)
+#ifndef O2D_WANT_VORT
+#define O2D_WANT_VORT 128
+#endif
+
 namespace o2d {
 
 namespace {
@@ -90,7 +96,7 @@ int vorticity_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const 
   if (ns == 0 || nt == 0) return 0;
   const int64_t twgs = (nt + kBlock * kTpl - 1) / (kBlock * kTpl);
   const int64_t ntiles = (ns + kTile - 1) / kTile;
-  const int64_t want = 8 * (int64_t)d->num_cu;
+  const int64_t want = O2D_WANT_VORT * (int64_t)d->num_cu;
   int64_t nchunk = twgs >= want ? 1 : (want + twgs - 1) / twgs;
   if (nchunk > ntiles) nchunk = ntiles;
   if (nchunk > 65535) nchunk = 65535;
