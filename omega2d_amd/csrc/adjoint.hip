@@ -135,7 +135,7 @@ void adjoint_kernel(
         const double dy = pr_.y - py[t];
         const double d2 = __builtin_fma(dx, dx, dy * dy);
         double kk, bbb, dks;
-        core_adj64<CORE>(d2, pr[t], kk, bbb, dks);
+        core_adj64<CORE>(dx, dy, d2, pr[t], kk, bbb, dks);
         const double w = __builtin_fma(pr_.gv, dx, -pr_.gu * dy);
         as[t] = __builtin_fma(kk, w, as[t]);
         ar[t] = __builtin_fma(dks, w, ar[t]);

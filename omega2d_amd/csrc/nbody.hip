@@ -73,14 +73,14 @@ __device__ __forceinline__ void tile_pairs64(const Src64* __restrict__ tile, con
       const double dx = px[t] - q.x;
       const double dy = py[t] - q.y;
       const double d2 = __builtin_fma(dx, dx, dy * dy);
-      const double w = TR ? q.w + pw[t] : q.w;
+      const double w = TR ? wsum(q.w, pw[t]) : q.w;
       if constexpr (RES == O2D_VELONLY) {
-        const double g = q.s * core_k64<CORE>(d2, w);
+        const double g = q.s * core_k64<CORE>(dx, dy, d2, w);
         au[t] = __builtin_fma(-g, dy, au[t]);
         av[t] = __builtin_fma(g, dx, av[t]);
       } else {
         double kk, bbb;
-        core_kg64<CORE>(d2, w, kk, bbb);
+        core_kg64<CORE>(dx, dy, d2, w, kk, bbb);
         kk *= q.s;
         bbb *= q.s;
         au[t] = __builtin_fma(-kk, dy, au[t]);
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
       s.s = 0.0;
       s.w = rterm<CORE>(sr[je - 1]);
     }
-    if (uni) s.w += w0;
+    if (uni) s.w = wsum(s.w, w0);
     if constexpr (kFar) {
       double x0 = s.x, x1 = s.x, y0 = s.y, y1 = s.y, wm = s.w;
 #pragma unroll
@@ -325,7 +325,7 @@ void nbody32_kernel(
   const int64_t te = tb + tiles_per_chunk < ntiles ? tb + tiles_per_chunk : ntiles;
   for (int64_t it = tb; it < te; ++it) {
     float4 s = src[it * kTile + tid];
-    if (uni) s.w += w0;
+    if (uni) s.w = wsum(s.w, w0);
     __syncthreads();
     tile[tid] = s;
     if (tid < kGroups) tgrp[tid] = grp[it * kGroups + tid];

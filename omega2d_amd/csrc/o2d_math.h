@@ -35,32 +35,76 @@ __device__ __forceinline__ double rsq64(double x) {
 // ---- radius term ---------------------------------------------------------------------------
 // Every core depends on the two core radii only through w = f(sr) + f(tr): f(r) = r^2 for RM,
 // Gaussian and WL, r^4 for V2, r^6 for V3 (src/CoreFunc.h:48,56,136-147,190-200,239-249,283-294).
-// Sources carry f(sr), targets f(tr) (0 for point targets).
+// Sources carry f(sr), targets f(tr) (0 for point targets). Products are rounded one by one
+// (no fused multiply-add), as the reference writes them, so that the fp64 Gaussian's cutoff
+// predicate below sees the same w.
 template <int CORE, class T>
 __host__ __device__ __forceinline__ T rterm(T r) {
+#pragma clang fp contract(off)
   const T r2 = r * r;
   if constexpr (CORE == O2D_CORE_V2) return r2 * r2;
   else if constexpr (CORE == O2D_CORE_V3) return r2 * r2 * r2;
   else return r2;
 }
 
-// ---- fp64 velocity factor K(d2; w) ---------------------------------------------------------
-
-// Gaussian inside its compact cutoff (src/CoreFunc.h:117-131), out of line like gauss_core32
-__device__ __noinline__ double gauss_core64(double d2, double w) {
-  const double cf = rcp64(w);
-  const double x = d2 * cf;
-  if (x < 0.001) return cf;
-  return (1.0 - exp(-x)) * rcp64(d2);
+// the sum of two radius terms, rounded once (never fused with the products that formed them)
+__host__ __device__ __forceinline__ double wsum(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
 }
 
+// ---- the Gaussian's compact cutoffs --------------------------------------------------------
+// The reference without Vc compiles the generic exp_cond (src/CoreFunc.h:117-131): with
+// reld2 = (1/w) / (1/d^2) it returns 1/d^2 for reld2 > 16.0f, 1/w for reld2 < 0.001f and
+// (1 - e^-reld2)/d^2 between. The literals are floats; at S = double the second is promoted to
+// 0.001000000047497451..., not 0.001, and the core jumps by about 5e-4 relative across it.
+constexpr double kGaussFar = 16.0;
+constexpr double kGaussNear = (double)0.001f;
+// The kernels test x = d^2 * rcp64(w), within a few ulps of reld2. Where x lies within this
+// relative margin of a cutoff, gauss_branch_ref evaluates the reference's own expression.
+constexpr double kCutMargin = 0x1p-40;
+
+// The branch as the reference computes it: IEEE-rounded divisions (1/w, 1/d^2 and their
+// quotient) and d^2 = dx*dx + dy*dy with each product rounded. 1: beyond the far cutoff,
+// -1: below the near one, 0: between. Out of line: it runs only for pairs at a cutoff.
+__device__ __noinline__ int gauss_branch_ref(double dx, double dy, double w) {
+#pragma clang fp contract(off)
+  const double d2 = dx * dx + dy * dy;
+  const double corefac = 1.0 / w;
+  const double ood2 = 1.0 / d2;
+  const double reld2 = corefac / ood2;
+  return reld2 > kGaussFar ? 1 : (reld2 < kGaussNear ? -1 : 0);
+}
+
+// the branch from x = d^2 / w (a few ulps from reld2), exact near the cutoffs
+__device__ __forceinline__ int gauss_branch(double x, double dx, double dy, double w) {
+  if (x < kGaussNear * (1.0 - kCutMargin)) return -1;
+  if (x > kGaussNear * (1.0 + kCutMargin) && x < kGaussFar * (1.0 - kCutMargin)) return 0;
+  if (x > kGaussFar * (1.0 + kCutMargin)) return 1;
+  return gauss_branch_ref(dx, dy, w);
+}
+
+// ---- fp64 velocity factor K(d2; w) ---------------------------------------------------------
+
+// Gaussian inside (or at) its compact cutoff (src/CoreFunc.h:117-131), out of line like
+// gauss_core32
+__device__ __noinline__ double gauss_core64(double dx, double dy, double d2, double w) {
+  const double cf = rcp64(w);
+  const double x = d2 * cf;
+  const int br = gauss_branch(x, dx, dy, w);
+  if (br < 0) return cf;
+  const double ood2 = rcp64(d2);
+  return br > 0 ? ood2 : (1.0 - exp(-x)) * ood2;
+}
+
+// dx, dy: the pair's offsets; d2 their squared length (possibly fused); w the radius term
 template <int CORE>
-__device__ __forceinline__ double core_k64(double d2, double w) {
+__device__ __forceinline__ double core_k64(double dx, double dy, double d2, double w) {
   if constexpr (CORE == O2D_CORE_RM) {
     return rcp64(d2 + w);
   } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
-    if (d2 > 16.0 * w) return rcp64(d2);
-    return gauss_core64(d2, w);
+    if (d2 > kGaussFar * (1.0 + kCutMargin) * w) return rcp64(d2);
+    return gauss_core64(dx, dy, d2, w);
   } else if constexpr (CORE == O2D_CORE_WL) {
     const double dd = d2 + w;
     const double r = rcp64(dd);
@@ -75,7 +119,8 @@ __device__ __forceinline__ double core_k64(double d2, double w) {
 // K and bbb = 2 dK/d(d^2) in the reference's gradient forms (src/CoreFunc.h:64-81, 156-180,
 // 208-229, 256-274, 301-321). Gaussian at d2 == 0: bbb = 0 (the reference yields NaN there).
 template <int CORE>
-__device__ __forceinline__ void core_kg64(double d2, double w, double& k, double& bbb) {
+__device__ __forceinline__ void core_kg64(double dx, double dy, double d2, double w, double& k,
+                                          double& bbb) {
   if constexpr (CORE == O2D_CORE_RM) {
     const double r = rcp64(d2 + w);
     k = r;
@@ -85,10 +130,8 @@ __device__ __forceinline__ void core_kg64(double d2, double w, double& k, double
     const double x = d2 * cf;
     const double ood2 = rcp64(d2);
     const double e = exp(-x);
-    double r;
-    if (x > 16.0) r = ood2;
-    else if (x < 0.001) r = cf;
-    else r = ood2 * (1.0 - e);
+    const int br = gauss_branch(x, dx, dy, w);
+    const double r = br > 0 ? ood2 : (br < 0 ? cf : ood2 * (1.0 - e));
     k = r;
     bbb = (d2 == 0.0) ? 0.0 : (-2.0 * r * ood2 + 2.0 * cf * ood2 * e);
   } else if constexpr (CORE == O2D_CORE_WL) {
@@ -113,14 +156,16 @@ struct KAdj {
   double k, bbb, dks;
 };
 
-// Gaussian inside its compact cutoff, for the adjoint; out of line so that the pair loop keeps
-// only the far-field registers (beyond the cutoff K = 1/d^2 and dK/dsigma = 0)
-__device__ __noinline__ KAdj gauss_adj_core64(double d2, double sr) {
-  const double s2 = sr * sr;
+// Gaussian inside (or at) its compact cutoff, for the adjoint; out of line so that the pair
+// loop keeps only the far-field registers (beyond the cutoff K = 1/d^2 and dK/dsigma = 0)
+__device__ __noinline__ KAdj gauss_adj_core64(double dx, double dy, double d2, double sr) {
+  const double s2 = rterm<O2D_CORE_GAUSSIAN>(sr);
   const double cf = rcp64(s2);
   const double x = d2 * cf;
-  if (x < 0.001) return KAdj{cf, 0.0, -2.0 * cf * rcp64(sr)};
+  const int br = gauss_branch(x, dx, dy, s2);
+  if (br < 0) return KAdj{cf, 0.0, -2.0 * cf * rcp64(sr)};
   const double ood2 = rcp64(d2);
+  if (br > 0) return KAdj{ood2, -2.0 * ood2 * ood2, 0.0};
   const double e = exp(-x);
   const double r = ood2 * (1.0 - e);
   return KAdj{r, -2.0 * r * ood2 + 2.0 * cf * ood2 * e, -2.0 * e * cf * rcp64(sr)};
@@ -129,8 +174,8 @@ __device__ __noinline__ KAdj gauss_adj_core64(double d2, double sr) {
 // K, bbb consistent with the branch of K actually taken, and dK/dsigma_source, for the adjoint
 // of point-target sums (t2 = 0).
 template <int CORE>
-__device__ __forceinline__ void core_adj64(double d2, double sr, double& k, double& bbb,
-                                           double& dks) {
+__device__ __forceinline__ void core_adj64(double dx, double dy, double d2, double sr, double& k,
+                                           double& bbb, double& dks) {
   const double s2 = sr * sr;
   if constexpr (CORE == O2D_CORE_RM) {
     const double r = rcp64(d2 + s2);
@@ -138,13 +183,13 @@ __device__ __forceinline__ void core_adj64(double d2, double sr, double& k, doub
     bbb = -2.0 * r * r;
     dks = -2.0 * sr * r * r;
   } else if constexpr (CORE == O2D_CORE_GAUSSIAN) {
-    if (d2 > 16.0 * s2) {
+    if (d2 > kGaussFar * (1.0 + kCutMargin) * s2) {
       const double ood2 = rcp64(d2);
       k = ood2;
       bbb = -2.0 * ood2 * ood2;
       dks = 0.0;
     } else {
-      const KAdj a = gauss_adj_core64(d2, sr);
+      const KAdj a = gauss_adj_core64(dx, dy, d2, sr);
       k = a.k;
       bbb = a.bbb;
       dks = a.dks;

@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kTBlock) void tree_eval_kernel(TreeArgs a, int64_t 
             const LeafSrc q = ls[k];
             const double ex = x - q.x, ey = y - q.y;
             const double e2 = __builtin_fma(ex, ex, ey * ey);
-            const double g = q.s * core_k64<CORE>(e2, q.w + wt);
+            const double g = q.s * core_k64<CORE>(ex, ey, e2, wsum(q.w, wt));
             u = __builtin_fma(-g, ey, u);
             v = __builtin_fma(g, ex, v);
           }

@@ -4,8 +4,13 @@ Two libraries, both built by ``oracle/Makefile``:
 
 * ``build/libo2d_oracle.so`` -- the fp64 C restatement (``o2d_oracle.c``), the parity oracle.
 * ``_ref/libo2d_ref.so``     -- the reference's own templates compiled from its sources
-  (``ref_harness.cpp``); used to write golden fixtures, to validate the restatement, and as
-  the "reference" CPU baseline. Absent where the reference tree was never available.
+  (``ref_harness.cpp``) with its compiled-in core (Vatistas n=2); used to write golden
+  fixtures, to validate the restatement, and as the "reference" CPU baseline. Absent where the
+  reference tree was never available.
+* ``_ref/libo2d_ref_<core>.so`` -- the same harness built with the reference's core define
+  switched to ``rm``, ``gauss``, ``wl`` or ``v3`` (see ``oracle/Makefile``).
+* ``_ref/libo2d_ref_dropin.so`` -- the reference's own ``points_affect_points`` built with
+  ``EXTERNAL_VEL_SOLVE`` and linked to the HIP library's hook (``ref_dropin.cpp``).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this
 module, always as the checker or the timed CPU baseline, never as the thing measured.
@@ -18,6 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(_HERE, "build", "libo2d_oracle.so")
 REF_SO = os.path.join(_HERE, "_ref", "libo2d_ref.so")
+DROPIN_SO = os.path.join(_HERE, "_ref", "libo2d_ref_dropin.so")
 
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -41,7 +47,6 @@ def _u32(a):
 
 
 _lib = None
-_ref = None
 
 
 def lib():
@@ -79,16 +84,30 @@ def lib():
     return _lib
 
 
-def have_ref():
-    return os.path.exists(REF_SO)
+def ref_so(core=None):
+    """Path of the reference build carrying ``core`` (None or "v2": the headers as they are)."""
+    if core in (None, "v2"):
+        return REF_SO
+    if core not in CORES:
+        raise ValueError(f"unknown core {core!r}")
+    return os.path.join(_HERE, "_ref", f"libo2d_ref_{core}.so")
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        if not have_ref():
-            raise RuntimeError(f"reference build missing: {REF_SO}")
-        R = ctypes.CDLL(REF_SO)
+def have_ref(core=None):
+    return os.path.exists(ref_so(core))
+
+
+_refs = {}
+
+
+def ref(core=None):
+    """ctypes handle of the reference build carrying ``core``; its compiled-in core is checked."""
+    key = "v2" if core is None else core
+    if key not in _refs:
+        path = ref_so(core)
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference build missing: {path}")
+        R = ctypes.CDLL(path)
         pp = [_i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp]
         R.ref_points_on_points_d.argtypes = pp
         R.ref_points_on_points_f.argtypes = pp
@@ -123,8 +142,31 @@ def ref():
         R.ref_body_motion_d.argtypes = [_i64, _vp, _vp, _i64, _vp, ctypes.c_char_p,
                                         ctypes.c_char_p, ctypes.c_char_p, _dbl, _dbl, _vp, _vp,
                                         _vp, _vp, _vp, _vp, _vp]
-        _ref = R
-    return _ref
+        R.ref_core_id.argtypes = []
+        got = R.ref_core_id()
+        if got != CORES[key]:
+            raise RuntimeError(f"{path} carries core id {got}, expected {CORES[key]} ({key})")
+        _refs[key] = R
+    return _refs[key]
+
+
+_dropin = None
+
+
+def have_dropin():
+    return os.path.exists(DROPIN_SO)
+
+
+def dropin():
+    global _dropin
+    if _dropin is None:
+        if not have_dropin():
+            raise RuntimeError(f"drop-in build missing: {DROPIN_SO}")
+        D = ctypes.CDLL(DROPIN_SO)
+        D.dropin_find_vels_f.argtypes = [_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _int,
+                                         _dbl, _dbl, _vp, _vp]
+        _dropin = D
+    return _dropin
 
 
 class _Quiet:
@@ -358,8 +400,10 @@ def clear_inner(method, nx, ny, idx, pnx, pny, cutoff_mult, ips, tx, ty, ts=None
 
 # ----------------------------------------------------------------------------- reference build
 
-def ref_points_on_points(sx, sy, ss, sr, tx, ty, tr=None, restype="vel", dtype=np.float64):
-    """Reference points_affect_points at S=dtype (A=double), V2 core (the compiled default)."""
+def ref_points_on_points(sx, sy, ss, sr, tx, ty, tr=None, restype="vel", dtype=np.float64,
+                         core=None):
+    """Reference points_affect_points at S=dtype (A=double), with the reference build carrying
+    ``core`` (None: the compiled default, Vatistas n=2)."""
     c = lambda a: np.ascontiguousarray(a, dtype=dtype)
     sx, sy, ss, sr, tx, ty = map(c, (sx, sy, ss, sr, tx, ty))
     tr = None if tr is None else c(tr)
@@ -367,7 +411,8 @@ def ref_points_on_points(sx, sy, ss, sr, tx, ty, tr=None, restype="vel", dtype=n
     tu, tv = np.zeros(nt, dtype), np.zeros(nt, dtype)
     tw = np.zeros(nt, dtype) if restype == "vort" else None
     te = np.zeros(nt, dtype) if restype == "shear" else None
-    fn = ref().ref_points_on_points_d if dtype == np.float64 else ref().ref_points_on_points_f
+    R = ref(core)
+    fn = R.ref_points_on_points_d if dtype == np.float64 else R.ref_points_on_points_f
     with quiet():
         rc = fn(len(sx), _p(sx), _p(sy), _p(ss), _p(sr), nt, _p(tx), _p(ty), _p(tr),
                 RESULTS[restype], _p(tu), _p(tv), _p(tw), _p(te))
@@ -604,3 +649,25 @@ def ref_body_motion(nx, ny, idx, xexpr, yexpr, rexpr, t, factor=-1.0):
     assert rc == 0, rc
     out["body"] = body
     return out
+
+
+def dropin_find_vels(sets, tx, ty, tr=None, finalize=False, fs=(0.0, 0.0)):
+    """The reference's find_vels particle loop (src/Convection.h:185-213) with an external
+    environment: every (x, y, strength, radius) set in ``sets`` acts on the targets through
+    src/Influence.h:74-98, i.e. through the HIP library's external_vel_solver_f_/_tr_f_.
+    float32 storage, as the application stores. Returns (u, v) as float32."""
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    sets = [tuple(map(f, st)) for st in sets]
+    k = len(sets)
+    ns = np.array([len(st[0]) for st in sets], dtype=np.int64)
+    ptrs = [(ctypes.c_void_p * max(k, 1))(*[st[c].ctypes.data for st in sets]) for c in range(4)]
+    tx, ty = f(tx), f(ty)
+    tr = None if tr is None else f(tr)
+    nt = len(tx)
+    tu, tv = np.zeros(nt, np.float32), np.zeros(nt, np.float32)
+    with quiet():
+        rc = dropin().dropin_find_vels_f(k, _p(ns), *[ctypes.cast(p, _vp) for p in ptrs], nt,
+                                         _p(tx), _p(ty), _p(tr), int(bool(finalize)),
+                                         float(fs[0]), float(fs[1]), _p(tu), _p(tv))
+    assert rc == 0, rc
+    return tu, tv

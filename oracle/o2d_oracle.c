@@ -62,6 +62,14 @@
 #define O2D_PI 3.14159265358979323846
 
 enum { CORE_RM = 1, CORE_GAUSS = 2, CORE_WL = 3, CORE_V2 = 4, CORE_V3 = 5 };
+
+/* The Gaussian core's two cutoffs on reld2 = d^2/sigma^2. The build without Vc compiles the
+ * generic exp_cond, which compares against the float literals 16.0f and 0.001f
+ * (src/CoreFunc.h:119-131); at S=double they are promoted, and (double)0.001f is
+ * 0.001000000047497451305389404296875, not 0.001. Between the two the core jumps by about
+ * 5e-4 relative, so the literal matters. */
+#define GAUSS_FAR_CUT ((double)16.0f)
+#define GAUSS_NEAR_CUT ((double)0.001f)
 enum { RES_VEL = 1, RES_VORT = 4, RES_SHEAR = 5 };
 
 /* velocity factor K(d^2; sr, tr) (src/CoreFunc.h), tr2 = tr*tr (0 for a point target) */
@@ -72,8 +80,8 @@ static double core_k(int core, double d2, double sr, double tr) {
       const double ood2 = 1.0 / d2;
       const double corefac = 1.0 / (sr * sr + tr * tr);
       const double reld2 = corefac / ood2;
-      if (reld2 > 16.0) return ood2;
-      if (reld2 < 0.001) return corefac;
+      if (reld2 > GAUSS_FAR_CUT) return ood2;
+      if (reld2 < GAUSS_NEAR_CUT) return corefac;
       return ood2 * (1.0 - exp(-reld2));
     }
     case CORE_WL: {
@@ -108,8 +116,8 @@ static void core_kg(int core, double d2, double sr, double tr, double* k, double
       const double corefac = 1.0 / (sr * sr + tr * tr);
       const double reld2 = corefac / ood2;
       double r;
-      if (reld2 > 16.0) r = ood2;
-      else if (reld2 < 0.001) r = corefac;
+      if (reld2 > GAUSS_FAR_CUT) r = ood2;
+      else if (reld2 < GAUSS_NEAR_CUT) r = corefac;
       else r = ood2 * (1.0 - exp(-reld2));
       *k = r;
       *bbb = (d2 == 0.0) ? 0.0 : (-2.0 * r * ood2 + 2.0 * corefac * ood2 * exp(-reld2));
@@ -517,11 +525,11 @@ static void core_adj(int core, double d2, double sr, double* k, double* bbb, dou
       const double ood2 = 1.0 / d2;
       const double corefac = 1.0 / (sr * sr);
       const double reld2 = corefac / ood2;
-      if (reld2 > 16.0) {
+      if (reld2 > GAUSS_FAR_CUT) {
         *k = ood2;
         *bbb = -2.0 * ood2 * ood2;
         *dks = 0.0;
-      } else if (reld2 < 0.001) {
+      } else if (reld2 < GAUSS_NEAR_CUT) {
         *k = corefac;
         *bbb = 0.0;
         *dks = -2.0 * corefac / sr;

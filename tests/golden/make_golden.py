@@ -9,7 +9,13 @@ outputs. The fixture is data only: inputs and the reference's outputs.
     python tests/golden/make_golden.py
 
 Core function: the reference compiles in Vatistas n=2 (src/CoreFunc.h:38), so every
-particle-on-point vector uses that core.
+particle-on-point vector of reference_vectors.npz uses that core. tests/golden/core_vectors.npz
+holds the particle-on-point vectors of every core: oracle/Makefile also builds the harness once
+per other core (rm, gauss, wl, v3) with only the reference's core define switched, and this
+script calls each build (oracle.ref(core) checks the core each build reports).
+
+    python tests/golden/make_golden.py            # both files
+    python tests/golden/make_golden.py cores      # core_vectors.npz only
 """
 import os
 import shutil
@@ -27,6 +33,72 @@ import oracle as O  # noqa: E402
 from helpers import circle_panels  # noqa: E402
 
 OUT = os.path.join(HERE, "reference_vectors.npz")
+OUT_CORES = os.path.join(HERE, "core_vectors.npz")
+CORE_NAMES = ("rm", "gauss", "wl", "v2", "v3")
+
+
+def gauss_cutoff_targets(sx, sy, w, angle):
+    """Targets around one source at squared distances x*w, with x on both sides of the Gaussian
+    core's cutoffs (src/CoreFunc.h:119-131): 16, 0.001 and the float literal 0.001f the
+    reference compares with, and inside the window between the last two. Offsets start at
+    1e-13 relative, clear of the last-ulp differences that fused multiply-adds make."""
+    cuts = (16.0, 0.001, float(np.float32(0.001)))
+    rels = (1e-13, 1e-11, 1e-9, 1e-7, 1e-5, 1e-3)
+    xs = [c * (1 + sg * r) for c in cuts for r in rels for sg in (-1, 1)]
+    lo, hi = 0.001, float(np.float32(0.001))
+    xs += list(lo + (hi - lo) * np.linspace(0.02, 0.98, 25))
+    xs = np.array(xs)
+    d = np.sqrt(xs * w)
+    return sx + d * np.cos(angle), sy + d * np.sin(angle), xs
+
+
+def core_vectors():
+    """Every core x {point, particle targets} x {vel, vel+vort, vel+shear}, S=double, from the
+    reference build carrying that core; velocity at S=float too; the Gaussian cutoff sets."""
+    for c in CORE_NAMES:
+        if not O.have_ref(c):
+            raise SystemExit(f"{O.ref_so(c)} missing: run make -C oracle ref")
+    rng = np.random.default_rng(20261203)
+    d = {}
+    ns, nt = 300, 200
+    # radii large against the spacing, so that many pairs sit inside the cores
+    sx, sy = rng.uniform(-1, 1, ns), rng.uniform(-1, 1, ns)
+    ss, sr = rng.normal(0, 1, ns), rng.uniform(0.02, 0.25, ns)
+    tx, ty = rng.uniform(-1.1, 1.1, nt), rng.uniform(-1.1, 1.1, nt)
+    tr = rng.uniform(0.02, 0.25, nt)
+    d.update(cv_sx=sx, cv_sy=sy, cv_ss=ss, cv_sr=sr, cv_tx=tx, cv_ty=ty, cv_tr=tr)
+    for c in CORE_NAMES:
+        for res in ("vel", "vort", "shear"):
+            for tag, trr in (("0p", None), ("0b", tr)):
+                out = O.ref_points_on_points(sx, sy, ss, sr, tx, ty, trr, res, core=c)
+                for k, a in zip("uvw", out):
+                    d[f"cv_{c}_{tag}_{res}_{k}"] = a
+        # the self interaction with vorticity (targets are the sources, particle targets)
+        out = O.ref_points_on_points(sx, sy, ss, sr, sx, sy, sr, "vort", core=c)
+        for k, a in zip("uvw", out):
+            d[f"cv_{c}_self_{k}"] = a
+        # float storage (the application's STORE=float), velocity
+        f = [a.astype(np.float32) for a in (sx, sy, ss, sr, tx, ty, tr)]
+        for tag, trr in (("0p", None), ("0b", f[6])):
+            fu, fv = O.ref_points_on_points(*f[:6], trr, "vel", dtype=np.float32, core=c)
+            d[f"cvf_{c}_{tag}_u"], d[f"cvf_{c}_{tag}_v"] = fu, fv
+
+    # the Gaussian at its cutoffs: one source, point targets (w = sr^2) and particle targets of
+    # one radius (w = sr^2 + tr^2)
+    s0 = np.array([0.1]), np.array([-0.2]), np.array([0.7]), np.array([0.05])
+    trad = 0.03
+    for tag, w, trr in (("0p", 0.05 ** 2, None), ("0b", 0.05 ** 2 + trad ** 2, trad)):
+        gx, gy, xs = gauss_cutoff_targets(0.1, -0.2, w, 0.3)
+        tt = None if trr is None else np.full(len(gx), trr)
+        d[f"gc_{tag}_tx"], d[f"gc_{tag}_ty"], d[f"gc_{tag}_x"] = gx, gy, xs
+        for res in ("vel", "vort", "shear"):
+            out = O.ref_points_on_points(*s0, gx, gy, tt, res, core="gauss")
+            for k, a in zip("uvw", out):
+                d[f"gc_{tag}_{res}_{k}"] = a
+    d["gc_src"] = np.concatenate(s0)
+    d["gc_trad"] = np.array([trad])
+    np.savez_compressed(OUT_CORES, **d)
+    print(f"wrote {OUT_CORES} ({os.path.getsize(OUT_CORES)} bytes, {len(d)} arrays)")
 
 
 def main():
@@ -210,4 +282,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "cores":
+        core_vectors()
+    else:
+        main()
+        core_vectors()

@@ -155,7 +155,7 @@ def test_closed_form_profiles(r):
     assert _single_vortex_speed("rm", r, s) == pytest.approx(r / (r * r + s * s), rel=1e-15)
     if x > 16:
         gk = 1.0 / r
-    elif x < 0.001:
+    elif x < float(np.float32(0.001)):  # the reference compares with the float 0.001f
         gk = r / (s * s)
     else:
         gk = -np.expm1(-x) / r
@@ -227,7 +227,8 @@ def test_brute_force_long_double():
                 k = 1 / (d2 + s2)
             elif core == "gauss":
                 x = d2 / s2
-                k = np.where(x > 16, 1 / d2, np.where(x < 0.001, 1 / s2, -np.expm1(-x) / d2))
+                k = np.where(x > 16, 1 / d2,
+                             np.where(x < L(np.float32(0.001)), 1 / s2, -np.expm1(-x) / d2))
             elif core == "wl":
                 k = (d2 + 2 * s2) / (d2 + s2) ** 2
             elif core == "v2":
