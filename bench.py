@@ -336,11 +336,11 @@ def cpu_baseline(w, seconds):
     n = len(w["x"])
     f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
     sx, sy, ss, sr = f32(w["x"]), f32(w["y"]), f32(w["s"]), f32(w["r"])
-    if O.have_ref():
+    if O.have_ref("gauss"):
         kind, fn = "reference", lambda m: O.ref_points_on_points(
-            sx, sy, ss, sr, sx[:m], sy[:m], sr[:m], "vel", dtype=np.float32)
-        label = ("reference points_affect_points<float,double>, its compiled-in Vatistas n=2 "
-                 "core (the Gaussian core needs a source edit), particle targets")
+            sx, sy, ss, sr, sx[:m], sy[:m], sr[:m], "vel", dtype=np.float32, core="gauss")
+        label = ("reference points_affect_points<float,double> built with its exponential "
+                 "(Gaussian) core define, particle targets, the GPU line's workload")
     else:
         kind, fn = "port", lambda m: O.points_on_points(
             "gauss", w["x"], w["y"], w["s"], w["r"], w["x"][:m], w["y"][:m], w["r"][:m])
