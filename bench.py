@@ -29,6 +29,7 @@ TREE_THETA, TREE_ORDER = 0.6, 28
 METRIC = "G pair-interactions/s at N=1M (fp32-split vs fp64, err vs fp64); adjoint/fwd"
 PEAK_FP32 = 157.3e12   # MI355X dense FP32 vector, FLOP/s (includes packed FP32)
 PEAK_FP64 = 78.6e12    # MI355X FP64 vector, FLOP/s
+ORACLE_TARGETS = 4096  # particle targets checked against the CPU oracle (all 1M sources each)
 
 
 def log(*a):
@@ -129,6 +130,39 @@ def main():
               flush=True)
         return 0
 
+    # ---- the same pairs in a random order (particles and tracers each permuted): the kernels
+    # sort both internally, so the rate should not depend on the caller's order
+    rng_p = np.random.default_rng(7)
+    pp, pq = rng_p.permutation(n), rng_p.permutation(nt)
+    px_, py_, ps_, pr_ = (T(w[k][pp]) for k in ("x", "y", "s", "r"))
+    ptx, pty = T(w["tx"][pq]), T(w["ty"][pq])
+
+    def step_shuffled():
+        pu.zero_()
+        pv.zero_()
+        qu.zero_()
+        qv.zero_()
+        dv.points_on_points(px_, py_, ps_, pr_, px_, py_, pr_, core="gauss", precision="split",
+                            tu=pu, tv=pv)
+        dv.points_on_points(px_, py_, ps_, pr_, ptx, pty, None, core="gauss", precision="split",
+                            tu=qu, tv=qv)
+
+    step_shuffled()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step_shuffled()
+    torch.cuda.synchronize()
+    ms_shuf = (time.perf_counter() - t0) * 1e3 / args.steps
+    hu, hv, hqu, hqv = (a.cpu().numpy() for a in (pu, pv, qu, qv))
+    err_shuf = float(np.sqrt((np.sum((hu - su[pp]) ** 2) + np.sum((hv - sv_[pp]) ** 2)
+                              + np.sum((hqu - squ[pq]) ** 2) + np.sum((hqv - sqv[pq]) ** 2))
+                             / (np.sum(su ** 2) + np.sum(sv_ ** 2) + np.sum(squ ** 2)
+                                + np.sum(sqv ** 2))))
+    log(f"shuffled order: {ms_shuf:.2f} ms/step ({ms_shuf / ms_split:.3f} of the ordered time), "
+        f"results vs ordered {err_shuf:.3e}")
+    del px_, py_, ps_, pr_, ptx, pty
+
     # ---- plain fp64 form
     step("fp64")
     torch.cuda.synchronize()
@@ -156,14 +190,15 @@ def main():
     try:
         import oracle as O
         rng = np.random.default_rng(0)
-        sub = np.sort(rng.choice(n, 256, replace=False))
+        sub = np.sort(rng.choice(n, ORACLE_TARGETS, replace=False))
         ou, ov = O.points_on_points("gauss", w["x"], w["y"], w["s"], w["r"], w["x"][sub],
                                     w["y"][sub], w["r"][sub])
         err_oracle = float(np.sqrt((np.sum((su[sub] - ou) ** 2) + np.sum((sv_[sub] - ov) ** 2))
                                    / (np.sum(ou ** 2) + np.sum(ov ** 2))))
         err64_oracle = float(np.sqrt((np.sum((fu[sub] - ou) ** 2) + np.sum((fv[sub] - ov) ** 2))
                                      / (np.sum(ou ** 2) + np.sum(ov ** 2))))
-        log(f"oracle subsample (256 targets): split err {err_oracle:.3e}, fp64 err {err64_oracle:.3e}")
+        log(f"oracle subsample ({ORACLE_TARGETS} targets): split err {err_oracle:.3e}, "
+            f"fp64 err {err64_oracle:.3e}")
     except Exception as e:  # the checker is optional on a box without the oracle build
         log(f"oracle check skipped: {e}")
         err64_oracle = None
@@ -277,6 +312,11 @@ def main():
                        "kernel_ms": round(k64_ms / max(k64_n, 1), 4),
                        "frac_fp64_peak": round(flops_main / (k64_ms / max(k64_n, 1) * 1e-3)
                                                / PEAK_FP64, 4)},
+        "shuffled": {"value": round(pairs / (ms_shuf * 1e-3) / 1e9, 2),
+                     "ms_per_step": round(ms_shuf, 3),
+                     "time_over_ordered": round(ms_shuf / ms_split, 4),
+                     "err_vs_ordered": err_shuf,
+                     "note": "the same pairs with particles and tracers randomly permuted"},
         "split_over_fp64": round(ms_64 / ms_split, 3),
         "err_vs_fp64": err_fp64,
         "err_vs_oracle_subsample": err_oracle,

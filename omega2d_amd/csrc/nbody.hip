@@ -1,11 +1,16 @@
 // nbody.hip - particles on points, the direct Biot-Savart sum (src/Influence.h:56-416).
 //
-// Plain form (fp64): every lane owns TPL targets and runs an unrolled loop over a 256-source
-// tile staged in LDS from the SoA inputs with coalesced loads; all lanes of a wave read the
-// same source (an LDS broadcast). Sums are fp64 per target.
+// Both forms put the sources and the targets in Morton order once per call (the stable order
+// of 32-bit keys; a collection acting on itself sorts once), so that a wave's targets and a
+// tile's sources are each compact whatever the caller's order, and results are scattered back
+// to the targets' input order as they are written.
 //
-// Split form (fp32 pair arithmetic): sources are put in Morton order once per call, cut into
-// groups of kGroup sources, and stored as fp32 offsets from their group's fp64 centroid.
+// Plain form (fp64): every lane owns TPL targets and runs an unrolled loop over a 256-source
+// tile staged in LDS with coalesced loads; all lanes of a wave read the same source (an LDS
+// broadcast). Sums are fp64 per target.
+//
+// Split form (fp32 pair arithmetic): the sorted sources are cut into groups of kGroup
+// sources, and stored as fp32 offsets from their group's fp64 centroid.
 // Per (target, group) a lane forms its own fp64 offset to that centroid once and rounds it to
 // fp32, so every pair difference dx = (t - c) - (s - c) is an fp32 difference of two small
 // numbers. Each group's fp32 partial sums are folded into fp64 per-target accumulators.
@@ -99,13 +104,13 @@ __device__ __forceinline__ void tile_pairs64(const Src64* __restrict__ tile, con
   }
 }
 
+// src: the staged sources in Morton order; tq, tw: the targets' positions and radius terms in
+// Morton order (tw unused unless TR); torder[i]: the input index of sorted target i
 template <int CORE, int RES, bool TR, int TPL>
 __global__ __launch_bounds__(kBlock) void nbody64_kernel(
-    int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
-    const double* __restrict__ ss, const double* __restrict__ sr, int64_t nt,
-    const double* __restrict__ tx, const double* __restrict__ ty, const double* __restrict__ tr,
-    int64_t chunk, double* __restrict__ ou, double* __restrict__ ov, double* __restrict__ ow,
-    int direct) {
+    int64_t ns, const Src64* __restrict__ src, int64_t nt, const double2* __restrict__ tq,
+    const double* __restrict__ tw, const uint32_t* __restrict__ torder, int64_t chunk,
+    double* __restrict__ ou, double* __restrict__ ov, double* __restrict__ ow, int direct) {
   // the Gaussian's velocity sum takes a loop without the per-pair cutoff test for a tile that
   // lies beyond the compact cutoff of every target of the wave (d^2 > 16 w for all its pairs)
   constexpr bool kFar = CORE == O2D_CORE_GAUSSIAN && RES == O2D_VELONLY;
@@ -119,9 +124,10 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
   for (int t = 0; t < TPL; ++t) {
     const int64_t i = tbase + (int64_t)t * kBlock;
     const int64_t ii = i < nt ? i : nt - 1;
-    px[t] = tx[ii];
-    py[t] = ty[ii];
-    pw[t] = TR ? rterm<CORE>(tr[ii]) : 0.0;
+    const double2 p = tq[ii];
+    px[t] = p.x;
+    py[t] = p.y;
+    pw[t] = TR ? tw[ii] : 0.0;
     au[t] = av[t] = aw[t] = 0.0;
   }
   // when every target of the block has the same radius term w0, it is added to each source's
@@ -131,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
   bool uni = false;
   if constexpr (TR) {
     const int64_t i0 = (int64_t)blockIdx.x * (kBlock * TPL);
-    w0 = rterm<CORE>(tr[i0 < nt ? i0 : nt - 1]);
+    w0 = tw[i0 < nt ? i0 : nt - 1];
     bool same = true;
 #pragma unroll
     for (int t = 0; t < TPL; ++t) same = same && pw[t] == w0;
@@ -142,18 +148,8 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
   const int64_t je = jb + chunk < ns ? jb + chunk : ns;
   for (int64_t j0 = jb; j0 < je; j0 += kTile) {
     const int64_t j = j0 + tid;
-    Src64 s;
-    if (j < je) {
-      s.x = sx[j];
-      s.y = sy[j];
-      s.s = ss[j];
-      s.w = rterm<CORE>(sr[j]);
-    } else {  // zero strength at the chunk's last source: contributes exactly 0
-      s.x = sx[je - 1];
-      s.y = sy[je - 1];
-      s.s = 0.0;
-      s.w = rterm<CORE>(sr[je - 1]);
-    }
+    Src64 s = src[j < je ? j : je - 1];
+    if (j >= je) s.s = 0.0;  // zero strength at the chunk's last source: contributes exactly 0
     if (uni) s.w = wsum(s.w, w0);
     if constexpr (kFar) {
       double x0 = s.x, x1 = s.x, y0 = s.y, y1 = s.y, wm = s.w;
@@ -224,9 +220,10 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
     const int64_t i = tbase + (int64_t)t * kBlock;
     if (i >= nt) continue;
     if (direct) {
-      ou[i] += au[t];
-      ov[i] += av[t];
-      if (RES != O2D_VELONLY) ow[i] += aw[t];
+      const int64_t o = torder[i];
+      ou[o] += au[t];
+      ov[o] += av[t];
+      if (RES != O2D_VELONLY) ow[o] += aw[t];
     } else {
       const int64_t o = (int64_t)blockIdx.y * nt + i;
       ou[o] = au[t];
@@ -286,8 +283,8 @@ template <int CORE, int RES, bool TR, int TPL, int UNROLL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(O2D_WAVES32, 8)))
 void nbody32_kernel(
     const float4* __restrict__ src, const GroupInfo* __restrict__ grp, int64_t ntiles,
-    int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty,
-    const double* __restrict__ tr, int64_t tiles_per_chunk, double* __restrict__ ou,
+    int64_t nt, const double2* __restrict__ tq, const float* __restrict__ tw,
+    const uint32_t* __restrict__ torder, int64_t tiles_per_chunk, double* __restrict__ ou,
     double* __restrict__ ov, double* __restrict__ ow, int direct) {
   constexpr int kGroups = kTile / kGroup;
   constexpr bool kAux = RES != O2D_VELONLY;
@@ -302,9 +299,10 @@ void nbody32_kernel(
   for (int t = 0; t < TPL; ++t) {
     const int64_t i = tbase + (int64_t)t * kBlock;
     const int64_t ii = i < nt ? i : nt - 1;
-    px[t] = tx[ii];
-    py[t] = ty[ii];
-    pw[t] = TR ? (float)rterm<CORE>(tr[ii]) : 0.0f;
+    const double2 p = tq[ii];
+    px[t] = p.x;
+    py[t] = p.y;
+    pw[t] = TR ? tw[ii] : 0.0f;
     au[t] = av[t] = aw[t] = 0.0;
   }
   // when every target of the block has the same radius term w0, it is added to each source's
@@ -314,7 +312,7 @@ void nbody32_kernel(
   bool uni = false;
   if constexpr (TR) {
     const int64_t i0 = (int64_t)blockIdx.x * (kBlock * TPL);
-    w0 = (float)rterm<CORE>(tr[i0 < nt ? i0 : nt - 1]);
+    w0 = tw[i0 < nt ? i0 : nt - 1];
     bool same = true;
 #pragma unroll
     for (int t = 0; t < TPL; ++t) same = same && pw[t] == w0;
@@ -379,9 +377,10 @@ void nbody32_kernel(
     const int64_t i = tbase + (int64_t)t * kBlock;
     if (i >= nt) continue;
     if (direct) {
-      ou[i] += au[t];
-      ov[i] += av[t];
-      if (kAux) ow[i] += aw[t];
+      const int64_t o = torder[i];
+      ou[o] += au[t];
+      ov[o] += av[t];
+      if (kAux) ow[o] += aw[t];
     } else {
       const int64_t o = (int64_t)blockIdx.y * nt + i;
       ou[o] = au[t];
@@ -391,8 +390,9 @@ void nbody32_kernel(
   }
 }
 
-// out[i] += sum over chunks c (in order) of part[c*nt + i]
+// out[order[i]] += sum over chunks c (in order) of part[c*nt + i]
 __global__ __launch_bounds__(kBlock) void reduce_chunks_kernel(int64_t nchunk, int64_t nt,
+                                                               const uint32_t* __restrict__ order,
                                                                const double* __restrict__ pu,
                                                                const double* __restrict__ pv,
                                                                const double* __restrict__ pw,
@@ -407,9 +407,10 @@ __global__ __launch_bounds__(kBlock) void reduce_chunks_kernel(int64_t nchunk, i
     sv += pv[c * nt + i];
     if (pw) sw += pw[c * nt + i];
   }
-  ou[i] += su;
-  ov[i] += sv;
-  if (pw) ow[i] += sw;
+  const int64_t o = order[i];
+  ou[o] += su;
+  ov[o] += sv;
+  if (pw) ow[o] += sw;
 }
 
 // ---------------------------------------------------------------------------- Morton order
@@ -598,6 +599,42 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int64_t ns, int64_t ngrou
   if (lane == 0) grp[g] = GroupInfo{cx, cy, rad, wmax, 0.0f, 0.0f};
 }
 
+// The fp64 form's staged sources: record i is input source order[i] with its radius term.
+template <int CORE>
+__global__ __launch_bounds__(kBlock) void stage64_kernel(int64_t n, const uint32_t* __restrict__ order,
+                                                         const double* __restrict__ sx,
+                                                         const double* __restrict__ sy,
+                                                         const double* __restrict__ ss,
+                                                         const double* __restrict__ sr,
+                                                         Src64* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = order[i];
+  Src64 q;
+  q.x = sx[o];
+  q.y = sy[o];
+  q.s = ss[o];
+  q.w = rterm<CORE>(sr[o]);
+  out[i] = q;
+}
+
+// Targets in Morton order: position i is input target order[i]; w (when tr is given) its radius
+// term, in the type the form's pair arithmetic adds it in.
+template <int CORE, class W>
+__global__ __launch_bounds__(kBlock) void gather_targets_kernel(int64_t n,
+                                                                const uint32_t* __restrict__ order,
+                                                                const double* __restrict__ tx,
+                                                                const double* __restrict__ ty,
+                                                                const double* __restrict__ tr,
+                                                                double2* __restrict__ pos,
+                                                                W* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = order[i];
+  pos[i] = make_double2(tx[o], ty[o]);
+  if (tr) w[i] = (W)rterm<CORE>(tr[o]);
+}
+
 // ---------------------------------------------------------------------------- launch helpers
 
 struct ChunkPlan {
@@ -624,20 +661,19 @@ ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles, int p
 }
 
 template <int CORE, int RES, bool TR>
-void launch64(dim3 grid, hipStream_t s, int64_t ns, const double* sx, const double* sy,
-              const double* ss, const double* sr, int64_t nt, const double* tx, const double* ty,
-              const double* tr, int64_t chunk, double* ou, double* ov, double* ow, int direct) {
-  nbody64_kernel<CORE, RES, TR, kTpl64><<<grid, kBlock, 0, s>>>(ns, sx, sy, ss, sr, nt, tx, ty, tr,
-                                                               chunk, ou, ov, ow, direct);
+void launch64(dim3 grid, hipStream_t s, int64_t ns, const Src64* src, int64_t nt,
+              const double2* tq, const double* tw, const uint32_t* torder, int64_t chunk,
+              double* ou, double* ov, double* ow, int direct) {
+  nbody64_kernel<CORE, RES, TR, kTpl64><<<grid, kBlock, 0, s>>>(ns, src, nt, tq, tw, torder, chunk,
+                                                               ou, ov, ow, direct);
 }
 
 template <int CORE>
-void dispatch64(int res, bool tr_present, dim3 grid, hipStream_t s, int64_t ns, const double* sx,
-                const double* sy, const double* ss, const double* sr, int64_t nt,
-                const double* tx, const double* ty, const double* tr, int64_t chunk, double* ou,
-                double* ov, double* ow, int direct) {
+void dispatch64(int res, bool tr_present, dim3 grid, hipStream_t s, int64_t ns, const Src64* src,
+                int64_t nt, const double2* tq, const double* tw, const uint32_t* torder,
+                int64_t chunk, double* ou, double* ov, double* ow, int direct) {
 #define O2D_L64(R, T) \
-  launch64<CORE, R, T>(grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct)
+  launch64<CORE, R, T>(grid, s, ns, src, nt, tq, tw, torder, chunk, ou, ov, ow, direct)
   if (res == O2D_VELONLY) {
     if (tr_present) O2D_L64(O2D_VELONLY, true);
     else O2D_L64(O2D_VELONLY, false);
@@ -653,31 +689,32 @@ void dispatch64(int res, bool tr_present, dim3 grid, hipStream_t s, int64_t ns, 
 
 template <int CORE, int RES>
 void launch32(bool tr_present, dim3 grid, hipStream_t s, const float4* src, const GroupInfo* grp,
-              int64_t ntiles, int64_t nt, const double* tx, const double* ty, const double* tr,
-              int64_t per, double* ou, double* ov, double* ow, int direct) {
+              int64_t ntiles, int64_t nt, const double2* tq, const float* tw,
+              const uint32_t* torder, int64_t per, double* ou, double* ov, double* ow,
+              int direct) {
   constexpr int kUnroll = 16;
   if (tr_present)
     nbody32_kernel<CORE, RES, true, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
-        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, ow, direct);
+        src, grp, ntiles, nt, tq, tw, torder, per, ou, ov, ow, direct);
   else
     nbody32_kernel<CORE, RES, false, kTpl32, kUnroll><<<grid, kBlock, 0, s>>>(
-        src, grp, ntiles, nt, tx, ty, tr, per, ou, ov, ow, direct);
+        src, grp, ntiles, nt, tq, tw, torder, per, ou, ov, ow, direct);
 }
 
 template <int CORE>
 void dispatch32(int res, bool tr_present, dim3 grid, hipStream_t s, const float4* src,
-                const GroupInfo* grp, int64_t ntiles, int64_t nt, const double* tx,
-                const double* ty, const double* tr, int64_t per, double* ou, double* ov,
+                const GroupInfo* grp, int64_t ntiles, int64_t nt, const double2* tq,
+                const float* tw, const uint32_t* torder, int64_t per, double* ou, double* ov,
                 double* ow, int direct) {
   if (res == O2D_VELONLY)
-    launch32<CORE, O2D_VELONLY>(tr_present, grid, s, src, grp, ntiles, nt, tx, ty, tr, per, ou,
-                                ov, ow, direct);
+    launch32<CORE, O2D_VELONLY>(tr_present, grid, s, src, grp, ntiles, nt, tq, tw, torder, per,
+                                ou, ov, ow, direct);
   else if (res == O2D_VELANDVORT)
-    launch32<CORE, O2D_VELANDVORT>(tr_present, grid, s, src, grp, ntiles, nt, tx, ty, tr, per,
-                                   ou, ov, ow, direct);
+    launch32<CORE, O2D_VELANDVORT>(tr_present, grid, s, src, grp, ntiles, nt, tq, tw, torder,
+                                   per, ou, ov, ow, direct);
   else
-    launch32<CORE, O2D_VELANDSHEAR>(tr_present, grid, s, src, grp, ntiles, nt, tx, ty, tr, per,
-                                    ou, ov, ow, direct);
+    launch32<CORE, O2D_VELANDSHEAR>(tr_present, grid, s, src, grp, ntiles, nt, tq, tw, torder,
+                                    per, ou, ov, ow, direct);
 }
 
 template <int CORE>
@@ -691,6 +728,33 @@ void launch_prep(int64_t ngroups, int64_t npad, hipStream_t s, int64_t ns, const
 }
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+template <int CORE>
+void launch_stage64(hipStream_t s, int64_t n, const uint32_t* order, const double* sx,
+                    const double* sy, const double* ss, const double* sr, Src64* out) {
+  stage64_kernel<CORE><<<grid1(n), kBlock, 0, s>>>(n, order, sx, sy, ss, sr, out);
+}
+
+template <int CORE, class W>
+void launch_gather(hipStream_t s, int64_t n, const uint32_t* order, const double* tx,
+                   const double* ty, const double* tr, double2* pos, W* w) {
+  gather_targets_kernel<CORE, W><<<grid1(n), kBlock, 0, s>>>(n, order, tx, ty, tr, pos, w);
+}
+
+// targets in the given order, radius terms of type W (when tr is given)
+template <class W>
+int gather_targets(hipStream_t s, int core, int64_t n, const uint32_t* order, const double* tx,
+                   const double* ty, const double* tr, double2* pos, W* w) {
+  switch (core) {
+    case O2D_CORE_RM: launch_gather<O2D_CORE_RM, W>(s, n, order, tx, ty, tr, pos, w); break;
+    case O2D_CORE_GAUSSIAN: launch_gather<O2D_CORE_GAUSSIAN, W>(s, n, order, tx, ty, tr, pos, w); break;
+    case O2D_CORE_WL: launch_gather<O2D_CORE_WL, W>(s, n, order, tx, ty, tr, pos, w); break;
+    case O2D_CORE_V2: launch_gather<O2D_CORE_V2, W>(s, n, order, tx, ty, tr, pos, w); break;
+    default: launch_gather<O2D_CORE_V3, W>(s, n, order, tx, ty, tr, pos, w); break;
+  }
+  O2D_LAUNCH_CHECK();
+  return 0;
+}
 
 }  // namespace
 
@@ -727,12 +791,69 @@ int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const dou
   return 0;
 }
 
+// Targets in Morton order for the kernels: their positions and radius terms gathered into *pos
+// and *w, and *order mapping each sorted target to its input index. Targets that are the
+// sources themselves (the same position arrays) reuse the sources' order src_order.
+template <class W>
+int prep_targets(Device* d, hipStream_t s, int core, int64_t nt, const double* tx,
+                 const double* ty, const double* tr, const uint32_t* src_order,
+                 const uint32_t** order, const double2** pos, const W** w) {
+  const uint32_t* o = src_order;
+  if (!o) {
+    void *kT, *oT;
+    O2D_TRY(slot(d, kTrgKeys, nt * sizeof(uint32_t), s, &kT));
+    O2D_TRY(slot(d, kTrgOrder, nt * sizeof(uint32_t), s, &oT));
+    O2D_TRY(morton_order(d, s, nt, tx, ty, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kT),
+                         static_cast<uint32_t*>(oT)));
+    o = static_cast<const uint32_t*>(oT);
+  }
+  void *pp, *wp = nullptr;
+  O2D_TRY(slot(d, kTrgPos, nt * sizeof(double2), s, &pp));
+  if (tr) O2D_TRY(slot(d, kTrgW, nt * sizeof(W), s, &wp));
+  O2D_TRY(gather_targets<W>(s, core, nt, o, tx, ty, tr, static_cast<double2*>(pp),
+                            static_cast<W*>(wp)));
+  *order = o;
+  *pos = static_cast<const double2*>(pp);
+  *w = static_cast<const W*>(wp);
+  return 0;
+}
+
 int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns, const double* sx,
               const double* sy, const double* ss, const double* sr, int64_t nt, const double* tx,
               const double* ty, const double* tr, double* tu, double* tv, double* tw) {
   if (nt == 0 || ns == 0) return 0;
+  // the radix sort of the Morton keys takes an int count
+  if (ns > 0x7fffffffLL || nt > 0x7fffffffLL)
+    return fail(O2D_ERR_SIZE, "more than 2^31 - 1 sources or targets");
   const bool trp = tr != nullptr;
+  // targets that are the sources themselves (a collection acting on itself) share their order
+  const bool self = tx == sx && ty == sy && nt == ns;
+  // sources in Morton order (both forms): the stable order of 32-bit Morton keys
+  void *kB, *oB;
+  O2D_TRY(slot(d, kKeysB, ns * sizeof(uint32_t), s, &kB));
+  O2D_TRY(slot(d, kOrderB, ns * sizeof(uint32_t), s, &oB));
+
   if (prec == O2D_PLAIN_FP64) {
+    void* sp;
+    O2D_TRY(slot(d, kSrc64, ns * sizeof(Src64), s, &sp));
+    O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
+                         static_cast<uint32_t*>(oB)));
+    const uint32_t* order = static_cast<const uint32_t*>(oB);
+    Src64* src = static_cast<Src64*>(sp);
+    switch (core) {
+      case O2D_CORE_RM: launch_stage64<O2D_CORE_RM>(s, ns, order, sx, sy, ss, sr, src); break;
+      case O2D_CORE_GAUSSIAN: launch_stage64<O2D_CORE_GAUSSIAN>(s, ns, order, sx, sy, ss, sr, src); break;
+      case O2D_CORE_WL: launch_stage64<O2D_CORE_WL>(s, ns, order, sx, sy, ss, sr, src); break;
+      case O2D_CORE_V2: launch_stage64<O2D_CORE_V2>(s, ns, order, sx, sy, ss, sr, src); break;
+      default: launch_stage64<O2D_CORE_V3>(s, ns, order, sx, sy, ss, sr, src); break;
+    }
+    O2D_LAUNCH_CHECK();
+    const uint32_t* torder;
+    const double2* tq;
+    const double* twv;
+    O2D_TRY(prep_targets<double>(d, s, core, nt, tx, ty, tr, self ? order : nullptr, &torder, &tq,
+                                 &twv));
+
     const int64_t twgs = (nt + kBlock * kTpl64 - 1) / (kBlock * kTpl64);
     const int64_t ntiles = (ns + kTile - 1) / kTile;
     ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT64);
@@ -750,29 +871,27 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
     }
     KernelTimer kt(trp ? O2D_KT_NBODY_PLAIN : O2D_KT_NBODY_PLAIN_PT, s);
     switch (core) {
-      case O2D_CORE_RM: dispatch64<O2D_CORE_RM>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
-      case O2D_CORE_GAUSSIAN: dispatch64<O2D_CORE_GAUSSIAN>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
-      case O2D_CORE_WL: dispatch64<O2D_CORE_WL>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
-      case O2D_CORE_V2: dispatch64<O2D_CORE_V2>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
-      default: dispatch64<O2D_CORE_V3>(res, trp, grid, s, ns, sx, sy, ss, sr, nt, tx, ty, tr, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_RM: dispatch64<O2D_CORE_RM>(res, trp, grid, s, ns, src, nt, tq, twv, torder, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_GAUSSIAN: dispatch64<O2D_CORE_GAUSSIAN>(res, trp, grid, s, ns, src, nt, tq, twv, torder, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_WL: dispatch64<O2D_CORE_WL>(res, trp, grid, s, ns, src, nt, tq, twv, torder, chunk, ou, ov, ow, direct); break;
+      case O2D_CORE_V2: dispatch64<O2D_CORE_V2>(res, trp, grid, s, ns, src, nt, tq, twv, torder, chunk, ou, ov, ow, direct); break;
+      default: dispatch64<O2D_CORE_V3>(res, trp, grid, s, ns, src, nt, tq, twv, torder, chunk, ou, ov, ow, direct); break;
     }
     kt.stop();
     O2D_LAUNCH_CHECK();
     if (!direct) {
-      reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, ou, ov, ow, tu, tv, tw);
+      reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, torder, ou, ov, ow, tu, tv,
+                                                        tw);
       O2D_LAUNCH_CHECK();
     }
     return 0;
   }
 
   // split form
-  if (ns > 0x7fffffffLL) return fail(O2D_ERR_SIZE, "too many sources for the split form");
   const int64_t ntiles = (ns + kTile - 1) / kTile;
   const int64_t npad = ntiles * kTile;
   const int64_t ngroups = npad / kGroup;
-  void *kB, *oB, *tsrc, *tcent;
-  O2D_TRY(slot(d, kKeysB, ns * sizeof(uint32_t), s, &kB));
-  O2D_TRY(slot(d, kOrderB, ns * sizeof(uint32_t), s, &oB));
+  void *tsrc, *tcent;
   O2D_TRY(slot(d, kTileSrc, npad * sizeof(float4), s, &tsrc));
   O2D_TRY(slot(d, kTileCent, ngroups * sizeof(GroupInfo), s, &tcent));
   KernelTimer kprep(O2D_KT_SPLIT_PREP, s);
@@ -788,8 +907,13 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
     case O2D_CORE_V2: launch_prep<O2D_CORE_V2>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
     default: launch_prep<O2D_CORE_V3>(ngroups, npad, s, ns, order, sx, sy, ss, sr, cent, src); break;
   }
-  kprep.stop();
   O2D_LAUNCH_CHECK();
+  const uint32_t* torder;
+  const double2* tq;
+  const float* twv;
+  O2D_TRY(prep_targets<float>(d, s, core, nt, tx, ty, tr, self ? order : nullptr, &torder, &tq,
+                              &twv));
+  kprep.stop();
 
   const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);
   ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT32);
@@ -806,16 +930,17 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   }
   KernelTimer kt(trp ? O2D_KT_NBODY_SPLIT : O2D_KT_NBODY_SPLIT_PT, s);
   switch (core) {
-    case O2D_CORE_RM: dispatch32<O2D_CORE_RM>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
-    case O2D_CORE_GAUSSIAN: dispatch32<O2D_CORE_GAUSSIAN>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
-    case O2D_CORE_WL: dispatch32<O2D_CORE_WL>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
-    case O2D_CORE_V2: dispatch32<O2D_CORE_V2>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
-    default: dispatch32<O2D_CORE_V3>(res, trp, grid, s, src, cent, ntiles, nt, tx, ty, tr, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_RM: dispatch32<O2D_CORE_RM>(res, trp, grid, s, src, cent, ntiles, nt, tq, twv, torder, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_GAUSSIAN: dispatch32<O2D_CORE_GAUSSIAN>(res, trp, grid, s, src, cent, ntiles, nt, tq, twv, torder, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_WL: dispatch32<O2D_CORE_WL>(res, trp, grid, s, src, cent, ntiles, nt, tq, twv, torder, cp.per_chunk, ou, ov, ow, direct); break;
+    case O2D_CORE_V2: dispatch32<O2D_CORE_V2>(res, trp, grid, s, src, cent, ntiles, nt, tq, twv, torder, cp.per_chunk, ou, ov, ow, direct); break;
+    default: dispatch32<O2D_CORE_V3>(res, trp, grid, s, src, cent, ntiles, nt, tq, twv, torder, cp.per_chunk, ou, ov, ow, direct); break;
   }
   kt.stop();
   O2D_LAUNCH_CHECK();
   if (!direct) {
-    reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, ou, ov, ow, tu, tv, tw);
+    reduce_chunks_kernel<<<grid1(nt), kBlock, 0, s>>>(cp.nchunk, nt, torder, ou, ov, ow, tu, tv,
+                                                      tw);
     O2D_LAUNCH_CHECK();
   }
   return 0;

@@ -22,6 +22,7 @@ enum Slot : int {
   kPartials,
   kKeysA, kKeysB, kOrderA, kOrderB, kSortTemp, kBBox,
   kTileSrc, kTileCent,
+  kTrgKeys, kTrgOrder, kTrgPos, kTrgW, kSrc64,
   kPanX, kPanY, kPanIdx, kPanVs, kPanSs,
   kPanGeom,
   kMatA, kRhs,
