@@ -270,6 +270,14 @@ def main():
     log(f"treecode: {ms_tree:.2f} ms/step, {pairs / (ms_tree * 1e-3) / 1e9:.1f} equivalent "
         f"G-int/s, err vs fp64 {err_tree:.3e}")
 
+    # ---- the other measured configurations: one Ralston RK3 step of 4M particles (C5) and
+    # Ralston RK2 steps of the 200k wake around the 2,048-panel circle (C3), device-resident
+    del pu, pv, qu, qv, sx, sy, ss, sr, tx, ty
+    torch.cuda.empty_cache()
+    c5 = c5_rk3_step(T)
+    torch.cuda.empty_cache()
+    c3 = c3_rk2_steps(T, args.steps)
+
     # ---- roofline of the dominant kernel: the split launch with particle targets (N x N pairs,
     # 19 flops each in the reference's convention plus 2 per target), timed by the library's
     # HIP events around each such launch on its stream
@@ -331,12 +339,105 @@ def main():
                      "equivalent_value": round(pairs / (ms_tree * 1e-3) / 1e9, 1),
                      "err_vs_fp64": err_tree,
                      "note": "same evaluation, O(N log N) with fp64 moments; not the headline"},
+        "c5_rk3": c5,
+        "c3_rk2": c3,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_reference_c3": cpu_rows,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def c5_rk3_step(T):
+    """One Ralston RK3 step (src/Convection.h:472-598) of the 4,194,304-particle shear layer:
+    three split-form velocity evaluations (Rosenhead-Moore, particle targets) and the fp64
+    position updates; the error of the first stage's velocity against the fp64 form on a
+    16,384-target subsample."""
+    import torch
+    from omega2d_amd import device_step as ds
+    from omega2d_amd import flops_per_pair, workloads as wl
+    from omega2d_amd._cabi import KT_NBODY_SPLIT, timing_enable, timing_read
+    w = wl.c5_shear_layer()
+    n = len(w["x"])
+    x0, y0, s, r = T(w["x"]), T(w["y"]), T(w["s"]), T(w["r"])
+    st = ds.Stepper(x0.clone(), y0.clone(), s, r, core="rm", precision="split", fs=w["fs"])
+    st.vels(x0, y0)  # untimed: the library's buffers for this size
+    torch.cuda.synchronize()
+    timing_read(KT_NBODY_SPLIT)
+    timing_enable(True)
+    t0 = time.perf_counter()
+    st.step(w["dt"], 3)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    timing_enable(False)
+    kms, kn = timing_read(KT_NBODY_SPLIT)
+    rng = np.random.default_rng(4)
+    sub = torch.from_numpy(np.sort(rng.choice(n, 16384, replace=False))).to(x0.device)
+    from omega2d_amd import device as dv
+    u6, v6 = dv.points_on_points(x0, y0, s, r, x0[sub].contiguous(), y0[sub].contiguous(),
+                                 r[sub].contiguous(), core="rm", precision="fp64")
+    c = 0.5 / np.pi
+    us, vs = st.stages[0]
+    du, dvv = us[sub] - c * u6, vs[sub] - c * v6
+    err = float(torch.sqrt((du * du + dvv * dvv).sum() / (c * c * (u6 * u6 + v6 * v6).sum())))
+    pairs = float(n) * n
+    k_ms = kms / max(kn, 1)
+    fl = flops_per_pair("rm", False) * pairs + 2.0 * n
+    log(f"C5 RK3 step: {t * 1e3:.1f} ms, {3 * pairs / t / 1e9:.1f} G-int/s; evaluation kernel "
+        f"{k_ms:.1f} ms ({fl / (k_ms * 1e-3) / PEAK_FP32:.3f} of FP32 peak); split error vs "
+        f"fp64 on 16,384 targets {err:.3e}")
+    return {"workload": "4,194,304 particles in a perturbed shear layer, Rosenhead-Moore core "
+                        "radius 0.002, one Ralston RK3 step dt=0.01: three split-form evaluations"
+                        " + fp64 position updates",
+            "ms_per_step": round(t * 1e3, 2), "G_int_s": round(3 * pairs / t / 1e9, 1),
+            "eval_kernel_ms": round(k_ms, 3), "evals_timed": kn,
+            "frac_fp32_peak": round(fl / (k_ms * 1e-3) / PEAK_FP32, 4),
+            "err_vs_fp64_16k": err}
+
+
+def c3_rk2_steps(T, steps):
+    """Ralston RK2 steps (src/Convection.h:390-466) of the 200,000-particle wake around the
+    2,048-panel circle (vortex panels, augmented 2049^2 system), all on the device: each stage
+    solves the BEM (points on panels, right-hand side, LU solve), sums the wake on itself in the
+    split form (Vatistas n=2) and the panels on the wake, moves the wake in fp64 and clears the
+    inner layer. The matrix is assembled and factored once (timed as setup)."""
+    import torch
+    from omega2d_amd import device_step as ds
+    from omega2d_amd import workloads as wl
+    from omega2d_amd._cabi import (KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS,
+                                   timing_enable, timing_read)
+    w = wl.c3_circle_wake()
+    ips = 0.005
+    x, y, s, r = T(w["x"]), T(w["y"]), T(w["s"]), T(w["r"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    body = ds.Body(w["nx"], w["ny"], w["idx"], nunk=1)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    st = ds.Stepper(x, y, s, r, core="v2", precision="split", fs=w["fs"], body=body, ips=ips)
+    for _ in range(2):
+        st.step(0.01, 2)
+    torch.cuda.synchronize()
+    cls = {"particles_split": KT_NBODY_SPLIT, "panels_on_particles": KT_PANELS,
+           "points_on_panels": KT_PANEL_RHS, "clear_inner": KT_NEAR}
+    for c in cls.values():
+        timing_read(c)
+    timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.step(0.01, 2)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    timing_enable(False)
+    parts = {k: round(timing_read(c)[0] / steps, 3) for k, c in cls.items()}
+    log(f"C3 RK2 step (device path): {t * 1e3:.2f} ms/step, setup {t_setup * 1e3:.1f} ms; "
+        f"kernel ms per step {parts}")
+    return {"workload": "2,048-panel circle (vortex panels, augmented 2049^2 LU) + 200,000 wake "
+                        "particles, Vatistas n=2 split form, Ralston RK2 dt=0.01, inner layer "
+                        "cleared at spacing 0.005; device-resident",
+            "ms_per_step": round(t * 1e3, 3), "steps": steps,
+            "setup_ms": round(t_setup * 1e3, 2), "kernel_ms_per_step": parts}
 
 
 def measured_traffic():

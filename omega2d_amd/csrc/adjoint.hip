@@ -60,22 +60,18 @@ void adjoint_kernel(
   const int64_t ke = kb + chunk < nt ? kb + chunk : nt;
   for (int64_t k0 = kb; k0 < ke; k0 += kTile) {
     const int64_t k = k0 + tid;
+    // past the chunk's end: a copy of its last probe with zero upstream gradient, which
+    // contributes exactly 0 (a probe anywhere else could coincide with a source and make 0 * NaN)
+    const int64_t kk = k < ke ? k : ke - 1;
     Probe p;
-    if (k < ke) {
-      p.x = tx[k];
-      p.y = ty[k];
-      p.gu = gu[k];
-      p.gv = gv[k];
-    } else {  // zero upstream gradient: contributes exactly 0
-      p.x = 0.0;
-      p.y = 0.0;
-      p.gu = 0.0;
-      p.gv = 0.0;
-    }
+    p.x = tx[kk];
+    p.y = ty[kk];
+    p.gu = k < ke ? gu[kk] : 0.0;
+    p.gv = k < ke ? gv[kk] : 0.0;
     __syncthreads();
     tile[tid] = p;
     if constexpr (CORE == O2D_CORE_GAUSSIAN) {
-      // bounding box of this tile's probes (padding probes at the origin only enlarge it)
+      // bounding box of this tile's probes (padding repeats a probe of the tile)
       double b0 = p.x, b1 = p.x, b2 = p.y, b3 = p.y;
 #pragma unroll
       for (int m = 32; m >= 1; m >>= 1) {

@@ -1,9 +1,13 @@
 // nbody.hip - particles on points, the direct Biot-Savart sum (src/Influence.h:56-416).
 //
-// Both forms put the sources and the targets in Morton order once per call (the stable order
-// of 32-bit keys; a collection acting on itself sorts once), so that a wave's targets and a
-// tile's sources are each compact whatever the caller's order, and results are scattered back
-// to the targets' input order as they are written.
+// Sources and targets are put in Morton order once per call (the stable order of 32-bit keys;
+// a collection acting on itself sorts once), so that a wave's targets and a tile's sources are
+// each compact whatever the caller's order, and results are scattered back to the targets'
+// input order as they are written. The split form always sorts its sources (its groups need
+// compact centroids) and sorts other targets for the Gaussian core; the fp64 form sorts both
+// for the Gaussian's velocity sum. Those are the sums whose speed depends on locality: the
+// Gaussian's wave-uniform far-field loops skip its in-core test. Every other sum keeps the
+// input order and skips the sort.
 //
 // Plain form (fp64): every lane owns TPL targets and runs an unrolled loop over a 256-source
 // tile staged in LDS with coalesced loads; all lanes of a wave read the same source (an LDS
@@ -105,7 +109,8 @@ __device__ __forceinline__ void tile_pairs64(const Src64* __restrict__ tile, con
 }
 
 // src: the staged sources in Morton order; tq, tw: the targets' positions and radius terms in
-// Morton order (tw unused unless TR); torder[i]: the input index of sorted target i
+// Morton order (tw unused unless TR); torder[i]: the input index of sorted target i (torder
+// null: the targets are in input order)
 template <int CORE, int RES, bool TR, int TPL>
 __global__ __launch_bounds__(kBlock) void nbody64_kernel(
     int64_t ns, const Src64* __restrict__ src, int64_t nt, const double2* __restrict__ tq,
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void nbody64_kernel(
     const int64_t i = tbase + (int64_t)t * kBlock;
     if (i >= nt) continue;
     if (direct) {
-      const int64_t o = torder[i];
+      const int64_t o = torder ? (int64_t)torder[i] : i;
       ou[o] += au[t];
       ov[o] += av[t];
       if (RES != O2D_VELONLY) ow[o] += aw[t];
@@ -377,7 +382,7 @@ void nbody32_kernel(
     const int64_t i = tbase + (int64_t)t * kBlock;
     if (i >= nt) continue;
     if (direct) {
-      const int64_t o = torder[i];
+      const int64_t o = torder ? (int64_t)torder[i] : i;
       ou[o] += au[t];
       ov[o] += av[t];
       if (kAux) ow[o] += aw[t];
@@ -390,7 +395,7 @@ void nbody32_kernel(
   }
 }
 
-// out[order[i]] += sum over chunks c (in order) of part[c*nt + i]
+// out[order[i]] += sum over chunks c (in order) of part[c*nt + i] (order null: the identity)
 __global__ __launch_bounds__(kBlock) void reduce_chunks_kernel(int64_t nchunk, int64_t nt,
                                                                const uint32_t* __restrict__ order,
                                                                const double* __restrict__ pu,
@@ -407,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void reduce_chunks_kernel(int64_t nchunk, i
     sv += pv[c * nt + i];
     if (pw) sw += pw[c * nt + i];
   }
-  const int64_t o = order[i];
+  const int64_t o = order ? (int64_t)order[i] : i;
   ou[o] += su;
   ov[o] += sv;
   if (pw) ow[o] += sw;
@@ -599,7 +604,8 @@ __global__ __launch_bounds__(kBlock) void group_kernel(int64_t ns, int64_t ngrou
   if (lane == 0) grp[g] = GroupInfo{cx, cy, rad, wmax, 0.0f, 0.0f};
 }
 
-// The fp64 form's staged sources: record i is input source order[i] with its radius term.
+// The fp64 form's staged sources: record i is input source order[i] (i when order is null)
+// with its radius term.
 template <int CORE>
 __global__ __launch_bounds__(kBlock) void stage64_kernel(int64_t n, const uint32_t* __restrict__ order,
                                                          const double* __restrict__ sx,
@@ -609,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void stage64_kernel(int64_t n, const uint32
                                                          Src64* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const uint32_t o = order[i];
+  const int64_t o = order ? (int64_t)order[i] : i;
   Src64 q;
   q.x = sx[o];
   q.y = sy[o];
@@ -618,8 +624,8 @@ __global__ __launch_bounds__(kBlock) void stage64_kernel(int64_t n, const uint32
   out[i] = q;
 }
 
-// Targets in Morton order: position i is input target order[i]; w (when tr is given) its radius
-// term, in the type the form's pair arithmetic adds it in.
+// Targets in Morton order: position i is input target order[i] (i when order is null); w (when
+// tr is given) its radius term, in the type the form's pair arithmetic adds it in.
 template <int CORE, class W>
 __global__ __launch_bounds__(kBlock) void gather_targets_kernel(int64_t n,
                                                                 const uint32_t* __restrict__ order,
@@ -630,7 +636,7 @@ __global__ __launch_bounds__(kBlock) void gather_targets_kernel(int64_t n,
                                                                 W* __restrict__ w) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const uint32_t o = order[i];
+  const int64_t o = order ? (int64_t)order[i] : i;
   pos[i] = make_double2(tx[o], ty[o]);
   if (tr) w[i] = (W)rterm<CORE>(tr[o]);
 }
@@ -791,15 +797,16 @@ int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const dou
   return 0;
 }
 
-// Targets in Morton order for the kernels: their positions and radius terms gathered into *pos
-// and *w, and *order mapping each sorted target to its input index. Targets that are the
-// sources themselves (the same position arrays) reuse the sources' order src_order.
+// Targets for the kernels: their positions and radius terms gathered into *pos and *w, and
+// *order mapping each gathered target to its input index. Targets that are the sources
+// themselves (the same position arrays) reuse the sources' order src_order; other targets are
+// put in Morton order when `sort`, else kept in input order (*order null).
 template <class W>
 int prep_targets(Device* d, hipStream_t s, int core, int64_t nt, const double* tx,
-                 const double* ty, const double* tr, const uint32_t* src_order,
+                 const double* ty, const double* tr, bool sort, const uint32_t* src_order,
                  const uint32_t** order, const double2** pos, const W** w) {
   const uint32_t* o = src_order;
-  if (!o) {
+  if (!o && sort) {
     void *kT, *oT;
     O2D_TRY(slot(d, kTrgKeys, nt * sizeof(uint32_t), s, &kT));
     O2D_TRY(slot(d, kTrgOrder, nt * sizeof(uint32_t), s, &oT));
@@ -834,11 +841,17 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   O2D_TRY(slot(d, kOrderB, ns * sizeof(uint32_t), s, &oB));
 
   if (prec == O2D_PLAIN_FP64) {
+    // only the Gaussian's velocity sum uses locality (its far-field tiles); every other fp64
+    // sum keeps the input order and skips the sort
+    const bool sort = core == O2D_CORE_GAUSSIAN && res == O2D_VELONLY;
     void* sp;
     O2D_TRY(slot(d, kSrc64, ns * sizeof(Src64), s, &sp));
-    O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
-                         static_cast<uint32_t*>(oB)));
-    const uint32_t* order = static_cast<const uint32_t*>(oB);
+    const uint32_t* order = nullptr;
+    if (sort) {
+      O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
+                           static_cast<uint32_t*>(oB)));
+      order = static_cast<const uint32_t*>(oB);
+    }
     Src64* src = static_cast<Src64*>(sp);
     switch (core) {
       case O2D_CORE_RM: launch_stage64<O2D_CORE_RM>(s, ns, order, sx, sy, ss, sr, src); break;
@@ -851,8 +864,8 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
     const uint32_t* torder;
     const double2* tq;
     const double* twv;
-    O2D_TRY(prep_targets<double>(d, s, core, nt, tx, ty, tr, self ? order : nullptr, &torder, &tq,
-                                 &twv));
+    O2D_TRY(prep_targets<double>(d, s, core, nt, tx, ty, tr, sort, self ? order : nullptr,
+                                 &torder, &tq, &twv));
 
     const int64_t twgs = (nt + kBlock * kTpl64 - 1) / (kBlock * kTpl64);
     const int64_t ntiles = (ns + kTile - 1) / kTile;
@@ -911,8 +924,9 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   const uint32_t* torder;
   const double2* tq;
   const float* twv;
-  O2D_TRY(prep_targets<float>(d, s, core, nt, tx, ty, tr, self ? order : nullptr, &torder, &tq,
-                              &twv));
+  // the target order matters to the Gaussian's wave-uniform far-field test only
+  O2D_TRY(prep_targets<float>(d, s, core, nt, tx, ty, tr, core == O2D_CORE_GAUSSIAN,
+                              self ? order : nullptr, &torder, &tq, &twv));
   kprep.stop();
 
   const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);

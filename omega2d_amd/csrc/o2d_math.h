@@ -17,6 +17,8 @@ namespace o2d {
 // step: with e = 1 - x r, 1/x = r / (1 - e) = r (1 + e + e^2) + O(e^3). The e^3 term is below
 // 1.3e-22, so the result is within rounding of two Newton steps (measured on 2^20 log-uniform
 // x in [e^-20, e^20]: largest relative error 1.1e-16 for both) with three fused ops, not four.
+// rcp64(0) is NaN, not inf (the seed is inf and e = fma(-0, inf, 1) is NaN): callers either
+// never pass 0 or discard the result, as the Gaussian's coincident pair does.
 __device__ __forceinline__ double rcp64(double x) {
   const double r = __builtin_amdgcn_rcp(x);
   const double e = __builtin_fma(-x, r, 1.0);

@@ -94,8 +94,8 @@ std::vector<double> panels_on_panels_coeff(const Surfaces<S>& src, Surfaces<S>& 
                                            const ExecEnv& env) {
   const int32_t nunk = (int32_t)targ.num_unknowns_per_panel();
   const o2d_env e = env.c_env(velonly);
-  const auto tx = as_double(targ.get_pos()[0]), ty = as_double(targ.get_pos()[1]);
-  const auto uu = as_double(targ.get_vel()[0]), uv = as_double(targ.get_vel()[1]);
+  const DoubleView<S> tx(targ.get_pos()[0]), ty(targ.get_pos()[1]);
+  const DoubleView<S> uu(targ.get_vel()[0]), uv(targ.get_vel()[1]);
   if (&src == &targ) {
     const bool aug = targ.is_augmented();
     const int64_t n = (int64_t)targ.get_num_rows();
@@ -111,7 +111,7 @@ std::vector<double> panels_on_panels_coeff(const Surfaces<S>& src, Surfaces<S>& 
     throw std::runtime_error("BEM blocks between bodies need the same unknowns per panel");
   const int64_t nr = (int64_t)targ.get_num_rows(), nc = (int64_t)src.get_num_rows();
   std::vector<double> coeffs((size_t)(nr * nc), 0.0);
-  const auto sx = as_double(src.get_pos()[0]), sy = as_double(src.get_pos()[1]);
+  const DoubleView<S> sx(src.get_pos()[0]), sy(src.get_pos()[1]);
   o2d_check(o2d_hip_bem_assemble_pair_d(&e, (int64_t)src.get_n(), sx.data(), sy.data(),
                                         (int64_t)src.get_npanels(), src.get_idx().data(),
                                         (int64_t)targ.get_n(), tx.data(), ty.data(),

@@ -17,29 +17,54 @@
 namespace omega2d {
 
 // x += dt * sum_k w[k] * u_k (src/ElementBase.h:276-359), stage velocities taken from `stages`
+// (stages[0] is p itself). With two or three stages the reference also leaves the combined
+// velocity u = sum_k w[k] u_k in the element (src/ElementBase.h:308-312, 340-345); so does this.
+// Positions advance in fp64 on the device and are rounded to S once.
 template <class S>
 void move_points(Points<S>& p, double dt, const std::vector<double>& w,
                  const std::vector<const Points<S>*>& stages, int32_t device) {
   if (p.get_movet() != lagrangian) return;
   const size_t n = p.get_n();
-  auto x = as_double(p.get_pos()[0]), y = as_double(p.get_pos()[1]);
-  std::vector<std::vector<double>> us, vs;
+  const size_t ns = stages.size();
+  std::vector<DoubleView<S>> us, vs;
+  us.reserve(ns);
+  vs.reserve(ns);
   std::vector<const double*> up, vp;
   for (const Points<S>* q : stages) {
-    us.push_back(as_double(q->get_vel()[0]));
-    vs.push_back(as_double(q->get_vel()[1]));
-  }
-  for (size_t k = 0; k < stages.size(); ++k) {
-    up.push_back(us[k].data());
-    vp.push_back(vs[k].data());
+    us.emplace_back(q->get_vel()[0]);
+    vs.emplace_back(q->get_vel()[1]);
+    up.push_back(us.back().data());
+    vp.push_back(vs.back().data());
   }
   o2d_env e = {O2D_CORE_V2, O2D_PLAIN_FP64, O2D_VELONLY, device, nullptr};
-  o2d_check(o2d_hip_rk_move_d(&e, (int64_t)n, dt, (int32_t)stages.size(), w.data(), up.data(),
-                              vp.data(), x.data(), y.data()),
-            "o2d_hip_rk_move");
-  for (size_t i = 0; i < n; ++i) {
-    p.get_pos()[0][i] = S(x[i]);
-    p.get_pos()[1][i] = S(y[i]);
+  if constexpr (std::is_same_v<S, double>) {
+    o2d_check(o2d_hip_rk_move_d(&e, (int64_t)n, dt, (int32_t)ns, w.data(), up.data(), vp.data(),
+                                p.get_pos()[0].data(), p.get_pos()[1].data()),
+              "o2d_hip_rk_move");
+  } else {
+    auto x = as_double(p.get_pos()[0]), y = as_double(p.get_pos()[1]);
+    o2d_check(o2d_hip_rk_move_d(&e, (int64_t)n, dt, (int32_t)ns, w.data(), up.data(), vp.data(),
+                                x.data(), y.data()),
+              "o2d_hip_rk_move");
+    for (size_t i = 0; i < n; ++i) {
+      p.get_pos()[0][i] = S(x[i]);
+      p.get_pos()[1][i] = S(y[i]);
+    }
+  }
+  if (ns >= 2) {
+    // the combination the device used for the positions, in the same order of operations;
+    // stages[0] may be p, so every stage value is read before p's velocity is overwritten
+    auto& pu = p.get_vel();
+    for (size_t i = 0; i < n; ++i) {
+      double cu = w[0] * up[0][i] + w[1] * up[1][i];
+      double cv = w[0] * vp[0][i] + w[1] * vp[1][i];
+      if (ns == 3) {
+        cu = cu + w[2] * up[2][i];
+        cv = cv + w[2] * vp[2][i];
+      }
+      pu[0][i] = S(cu);
+      pu[1][i] = S(cv);
+    }
   }
 }
 

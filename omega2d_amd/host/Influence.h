@@ -43,6 +43,84 @@ std::vector<double> as_double(const std::vector<S>& v) {
   return std::vector<double>(v.begin(), v.end());
 }
 
+// A read-only fp64 view of a storage array for the _d entry points: the array itself when the
+// storage is double, else a converted copy.
+template <class S>
+class DoubleView {
+ public:
+  explicit DoubleView(const std::vector<S>& v) {
+    if constexpr (std::is_same_v<S, double>) {
+      p = v.data();
+    } else {
+      copy.assign(v.begin(), v.end());
+      p = copy.data();
+    }
+  }
+  const double* data() const { return p; }
+
+ private:
+  std::vector<double> copy;
+  const double* p = nullptr;
+};
+
+// An fp64 in/out array for the _d entry points that update in place: the storage array itself
+// when it is double, else a converted copy that commit() rounds back into the storage.
+template <class S>
+class DoubleInOut {
+ public:
+  explicit DoubleInOut(std::vector<S>& v) : v(v) {
+    if constexpr (!std::is_same_v<S, double>) copy.assign(v.begin(), v.end());
+  }
+  double* data() {
+    if constexpr (std::is_same_v<S, double>) return v.data();
+    else return copy.data();
+  }
+  void commit() {
+    if constexpr (!std::is_same_v<S, double>)
+      for (size_t i = 0; i < v.size(); ++i) v[i] = S(copy[i]);
+  }
+
+ private:
+  std::vector<S>& v;
+  std::vector<double> copy;
+};
+
+// Accumulating fp64 outputs for the _d entry points: the storage arrays themselves when they
+// are double (the entry points add into them), else zeroed fp64 arrays that commit() adds into
+// the storage, each sum rounded once to S.
+template <class S>
+class DoubleAccum {
+ public:
+  DoubleAccum(std::vector<S>& u, std::vector<S>& v, size_t n) : u(u), v(v), n(n) {
+    if constexpr (!std::is_same_v<S, double>) {
+      tu.assign(n, 0.0);
+      tv.assign(n, 0.0);
+    }
+  }
+  double* du() {
+    if constexpr (std::is_same_v<S, double>) return u.data();
+    else return tu.data();
+  }
+  double* dv() {
+    if constexpr (std::is_same_v<S, double>) return v.data();
+    else return tv.data();
+  }
+  void commit() {
+    if constexpr (!std::is_same_v<S, double>) {
+      for (size_t i = 0; i < n; ++i) {
+        u[i] = S(double(u[i]) + tu[i]);
+        v[i] = S(double(v[i]) + tv[i]);
+      }
+    }
+  }
+
+ private:
+  std::vector<S>& u;
+  std::vector<S>& v;
+  size_t n;
+  std::vector<double> tu, tv;
+};
+
 // ---------------------------------------------------------------------- points on points
 
 template <class S, class A>
@@ -62,21 +140,15 @@ void points_affect_points(const Points<S>& src, Points<S>& targ, const ResultsTy
   if (env.get_summation() == barneshut && restype.get_type() == velonly) {
     // the treecode (fp64, double arrays), summation_t barneshut; vorticity and shear outputs
     // stay on the direct sum below
-    const auto sxd = as_double(sx[0]), syd = as_double(sx[1]);
-    const auto ssd = as_double(src.get_str()), srd = as_double(src.get_rad());
-    const auto txd = as_double(tx[0]), tyd = as_double(tx[1]);
-    std::vector<double> trd;
-    if (tr) trd = as_double(targ.get_rad());
-    auto ud = as_double(tu[0]), vd = as_double(tu[1]);
+    const DoubleView<S> sxd(sx[0]), syd(sx[1]), ssd(src.get_str()), srd(src.get_rad());
+    const DoubleView<S> txd(tx[0]), tyd(tx[1]), trd(targ.get_rad());
+    DoubleAccum<S> acc(tu[0], tu[1], targ.get_n());
     o2d_check(o2d_hip_points_on_points_tree_d(&e, &env.get_tree(), (int64_t)src.get_n(),
                                               sxd.data(), syd.data(), ssd.data(), srd.data(),
                                               (int64_t)targ.get_n(), txd.data(), tyd.data(),
-                                              tr ? trd.data() : nullptr, ud.data(), vd.data()),
+                                              tr ? trd.data() : nullptr, acc.du(), acc.dv()),
               "o2d_hip_points_on_points_tree");
-    for (size_t i = 0; i < targ.get_n(); ++i) {
-      tu[0][i] = S(ud[i]);
-      tu[1][i] = S(vd[i]);
-    }
+    acc.commit();
     // the log reports the rate of the direct sum the treecode stands in for
     const int per = o2d_hip_flops_per_pair(env.get_core(), targ.is_inert() ? 1 : 0, 0);
     log_rate("points_affect_points", start,
@@ -114,22 +186,17 @@ void panels_affect_points(const Surfaces<S>& src, Points<S>& targ, const Results
     throw std::runtime_error("surfaces cannot compute streamfunction or velocity gradients");
   const auto start = Clock::now();
   const o2d_env e = env.c_env(velonly);
-  const auto nx = as_double(src.get_pos()[0]), ny = as_double(src.get_pos()[1]);
-  const auto vs = as_double(src.get_str());
-  const auto ss = as_double(src.get_src_str());
-  const auto tx = as_double(targ.get_pos()[0]), ty = as_double(targ.get_pos()[1]);
-  std::vector<double> tu(targ.get_n(), 0.0), tv(targ.get_n(), 0.0);
+  const DoubleView<S> nx(src.get_pos()[0]), ny(src.get_pos()[1]);
+  const DoubleView<S> vs(src.get_str()), ss(src.get_src_str());
+  const DoubleView<S> tx(targ.get_pos()[0]), ty(targ.get_pos()[1]);
+  DoubleAccum<S> acc(targ.get_vel()[0], targ.get_vel()[1], targ.get_n());
   o2d_check(o2d_hip_panels_on_points_d(&e, (int64_t)src.get_n(), nx.data(), ny.data(),
                                        (int64_t)src.get_npanels(), src.get_idx().data(),
                                        vs.data(), src.have_src_str() ? ss.data() : nullptr,
-                                       (int64_t)targ.get_n(), tx.data(), ty.data(), tu.data(),
-                                       tv.data()),
+                                       (int64_t)targ.get_n(), tx.data(), ty.data(), acc.du(),
+                                       acc.dv()),
             "o2d_hip_panels_on_points");
-  auto& u = targ.get_vel();
-  for (size_t i = 0; i < targ.get_n(); ++i) {
-    u[0][i] = S(double(u[0][i]) + tu[i]);
-    u[1][i] = S(double(u[1][i]) + tv[i]);
-  }
+  acc.commit();
   log_rate("panels_affect_points", start,
            (double)targ.get_n() *
                (2.0 + o2d_hip_flops_per_pair(env.get_core(), 1, src.have_src_str() ? 2 : 1) *
@@ -147,20 +214,15 @@ void points_affect_panels(const Points<S>& src, Surfaces<S>& targ, const Results
     throw std::runtime_error("points cannot compute streamfunction or velocity gradients");
   const auto start = Clock::now();
   const o2d_env e = env.c_env(velonly);
-  const auto sx = as_double(src.get_pos()[0]), sy = as_double(src.get_pos()[1]);
-  const auto ss = as_double(src.get_str());
-  const auto nx = as_double(targ.get_pos()[0]), ny = as_double(targ.get_pos()[1]);
-  std::vector<double> pu(targ.get_npanels(), 0.0), pv(targ.get_npanels(), 0.0);
+  const DoubleView<S> sx(src.get_pos()[0]), sy(src.get_pos()[1]), ss(src.get_str());
+  const DoubleView<S> nx(targ.get_pos()[0]), ny(targ.get_pos()[1]);
+  DoubleAccum<S> acc(targ.get_vel()[0], targ.get_vel()[1], targ.get_npanels());
   o2d_check(o2d_hip_points_on_panels_d(&e, (int64_t)src.get_n(), sx.data(), sy.data(), ss.data(),
                                        (int64_t)targ.get_n(), nx.data(), ny.data(),
                                        (int64_t)targ.get_npanels(), targ.get_idx().data(),
-                                       pu.data(), pv.data()),
+                                       acc.du(), acc.dv()),
             "o2d_hip_points_on_panels");
-  auto& u = targ.get_vel();
-  for (size_t i = 0; i < targ.get_npanels(); ++i) {
-    u[0][i] = S(double(u[0][i]) + pu[i]);
-    u[1][i] = S(double(u[1][i]) + pv[i]);
-  }
+  acc.commit();
   log_rate("points_affect_panels", start,
            (double)targ.get_npanels() * (11.0 + 39.0 * (double)src.get_n()));
 }

@@ -19,11 +19,11 @@ namespace detail {
 
 template <class S>
 struct PanelArgs {
-  std::vector<double> nx, ny, pnx, pny;
+  DoubleView<S> nx, ny, pnx, pny;
   const std::vector<Int>* idx;
   explicit PanelArgs(const Surfaces<S>& s)
-      : nx(as_double(s.get_pos()[0])), ny(as_double(s.get_pos()[1])),
-        pnx(as_double(s.get_norm()[0])), pny(as_double(s.get_norm()[1])), idx(&s.get_idx()) {}
+      : nx(s.get_pos()[0]), ny(s.get_pos()[1]), pnx(s.get_norm()[0]), pny(s.get_norm()[1]),
+        idx(&s.get_idx()) {}
 };
 
 template <class S>
@@ -40,15 +40,15 @@ void reflect_panp2(const Surfaces<S>& src, Points<S>& targ, const ExecEnv& env) 
   const auto start = Clock::now();
   std::printf("  Reflecting %zu particles from near %zu panels\n", targ.get_n(), src.get_npanels());
   const detail::PanelArgs<S> pa(src);
-  auto x = as_double(targ.get_pos()[0]), y = as_double(targ.get_pos()[1]);
+  DoubleInOut<S> x(targ.get_pos()[0]), y(targ.get_pos()[1]);
   const o2d_env e = env.c_env(O2D_VELONLY);
   int64_t nref = 0;
   o2d_check(o2d_hip_reflect_d(&e, (int64_t)src.get_n(), pa.nx.data(), pa.ny.data(),
                               (int64_t)src.get_npanels(), pa.idx->data(), pa.pnx.data(),
                               pa.pny.data(), (int64_t)targ.get_n(), x.data(), y.data(), &nref),
             "o2d_hip_reflect");
-  detail::store(targ.get_pos()[0], x);
-  detail::store(targ.get_pos()[1], y);
+  x.commit();
+  y.commit();
   std::printf("    reflected %lld particles\n", (long long)nref);
   log_rate("reflect_panp2", start,
            (double)targ.get_n() * (62.0 + 27.0 * (double)src.get_npanels()));
@@ -73,13 +73,11 @@ S clear_inner_panp2(int method, const Surfaces<S>& src, Points<S>& targ, S cutof
   const auto start = Clock::now();
   std::printf("  Clearing %zu particles from near %zu panels\n", targ.get_n(), src.get_npanels());
   const detail::PanelArgs<S> pa(src);
-  auto x = as_double(targ.get_pos()[0]), y = as_double(targ.get_pos()[1]);
+  DoubleInOut<S> x(targ.get_pos()[0]), y(targ.get_pos()[1]);
   const bool fldpts = targ.is_inert();
-  std::vector<double> s, r;
-  if (!fldpts) {
-    s = as_double(targ.get_str());
-    r = as_double(targ.get_rad());
-  }
+  // field points carry no strength or radius (their arrays are empty)
+  DoubleInOut<S> s(targ.get_str());
+  const DoubleView<S> r(targ.get_rad());
   const o2d_env e = env.c_env(O2D_VELONLY);
   double circ = 0.0;
   int64_t ncrop = 0;
@@ -90,9 +88,9 @@ S clear_inner_panp2(int method, const Surfaces<S>& src, Points<S>& targ, S cutof
                                   fldpts ? nullptr : s.data(), fldpts ? nullptr : r.data(),
                                   &circ, &ncrop),
             "o2d_hip_clear_inner");
-  detail::store(targ.get_pos()[0], x);
-  detail::store(targ.get_pos()[1], y);
-  if (!fldpts && method == 0) detail::store(targ.get_str(), s);
+  x.commit();
+  y.commit();
+  if (!fldpts && method == 0) s.commit();
   std::printf("    %s %lld particles\n", method == 0 ? "cropped" : "pushed", (long long)ncrop);
   log_rate("clear_inner_panp2", start,
            (double)(targ.get_n() + ncrop) * (62.0 + 27.0 * (double)src.get_npanels()));
@@ -123,7 +121,7 @@ Vector<S> get_nearest_distances(const Surfaces<S>& src, const Points<S>& targ,
   require_gpu_hip(env);
   const auto start = Clock::now();
   const detail::PanelArgs<S> pa(src);
-  const auto x = as_double(targ.get_pos()[0]), y = as_double(targ.get_pos()[1]);
+  const DoubleView<S> x(targ.get_pos()[0]), y(targ.get_pos()[1]);
   std::vector<double> d(targ.get_n());
   const o2d_env e = env.c_env(O2D_VELONLY);
   o2d_check(o2d_hip_nearest_distances_d(&e, (int64_t)src.get_n(), pa.nx.data(), pa.ny.data(),

@@ -90,6 +90,9 @@ void run_rk(Reader& in, Writer& out, int64_t n, int order, int core, int prec,
   const auto& p = std::get<Points<S>>(vort[0]);
   out.vec(p.get_pos()[0]);
   out.vec(p.get_pos()[1]);
+  // the velocity the step leaves in the particles (the stage combination for RK2 and RK3)
+  out.vec(p.get_vel()[0]);
+  out.vec(p.get_vel()[1]);
 }
 
 // header: npan, nnodes, nwake, nunk, core, order; then fsx, fsy, dt, ips, nodes x/y
@@ -140,6 +143,8 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
     const auto& p = std::get<Points<double>>(vort[0]);
     out.vec(p.get_pos()[0]);
     out.vec(p.get_pos()[1]);
+    out.vec(p.get_vel()[0]);
+    out.vec(p.get_vel()[1]);
   }
 }
 

@@ -67,3 +67,24 @@ def test_probe_objective_gradients(inside):
             fd = (_objective("gauss", *args_p, px, py) - _objective("gauss", *args_m, px, py)) / (2 * h)
             scale = max(abs(fd), 1e-3 * np.max(np.abs(grads[k])))
             assert abs(grads[k][j] - fd) <= 1e-5 * scale, (k, j, grads[k][j], fd)
+
+
+@pytest.mark.parametrize("core", ["rm", "v2"])
+def test_padding_probes_leave_a_source_at_the_origin_finite(core):
+    # 300 probes (not a whole tile of 256): the padded slots repeat a real probe with zero
+    # upstream gradient. A zero-radius source at the origin, where a padding probe placed at
+    # (0, 0) would coincide with it, must keep finite gradients equal to the oracle's.
+    import omega2d_amd as o2d
+    rng = np.random.default_rng(3)
+    x = np.concatenate([[0.0], rng.uniform(-1, 1, 99)])
+    y = np.concatenate([[0.0], rng.uniform(-1, 1, 99)])
+    s = rng.normal(0, 1, 100)
+    r = np.concatenate([[0.0], rng.uniform(0.01, 0.05, 99)])
+    th = np.linspace(0, 2 * np.pi, 300, endpoint=False)
+    px, py = 2.0 * np.cos(th), 2.0 * np.sin(th)
+    gu, gv = rng.normal(size=300), rng.normal(size=300)
+    got = o2d.points_on_points_adjoint(x, y, s, r, px, py, gu, gv, core=core)
+    ref = O.points_on_points_adjoint(core, x, y, s, r, px, py, gu, gv)
+    for g, rg in zip(got, ref):
+        assert np.all(np.isfinite(g))
+        assert rel_l2(g, rg) <= 1e-12

@@ -42,16 +42,21 @@ def test_rk_step_free_particles(tmp_path, order):
     w = W.c1_disc(n=3000)
     x, y, s, r = w["x"], w["y"], w["s"], w["r"]
     dt = 0.01
-    got = run_driver(tmp_path, "rk", [len(x), order, CORE_ID["rm"], 0, 1], [np.float64(dt), x, y, s, r],
-                     2 * len(x))
-    # the Ralston schemes of src/Convection.h with oracle velocities
+    n = len(x)
+    got = run_driver(tmp_path, "rk", [n, order, CORE_ID["rm"], 0, 1], [np.float64(dt), x, y, s, r],
+                     4 * n)
+    # the Ralston schemes of src/Convection.h with oracle velocities; after two or three stages
+    # the particles hold the stage combination (src/ElementBase.h:308-312, 340-345), after one
+    # the stage velocity itself
     u0 = oracle_vel("rm", x, y, s, r)
     if order == 1:
         ex, ey = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], dt)
+        eu, ev = u0
     elif order == 2:
         x1, y1 = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * dt)
         u1 = oracle_vel("rm", x1, y1, s, r)
         ex, ey = O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt)
+        eu, ev = 0.25 * u0[0] + 0.75 * u1[0], 0.25 * u0[1] + 0.75 * u1[1]
     else:
         x1, y1 = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 0.5 * dt)
         u1 = oracle_vel("rm", x1, y1, s, r)
@@ -59,8 +64,11 @@ def test_rk_step_free_particles(tmp_path, order):
         u2 = oracle_vel("rm", x2, y2, s, r)
         ex, ey = O.rk_combine(x, y, [u0[0], u1[0], u2[0]], [u0[1], u1[1], u2[1]],
                               [2 / 9, 3 / 9, 4 / 9], dt)
-    gx, gy = got[: len(x)], got[len(x):]
+        eu = 2 / 9 * u0[0] + 3 / 9 * u1[0] + 4 / 9 * u2[0]
+        ev = 2 / 9 * u0[1] + 3 / 9 * u1[1] + 4 / 9 * u2[1]
+    gx, gy, gu, gv = got[:n], got[n:2 * n], got[2 * n:3 * n], got[3 * n:]
     assert np.max(np.abs(gx - ex)) <= 1e-14 and np.max(np.abs(gy - ey)) <= 1e-14
+    assert rel_l2(gu, eu) <= 1e-12 and rel_l2(gv, ev) <= 1e-12
 
 
 def test_rk2_step_with_treecode_summation(tmp_path):
@@ -72,28 +80,32 @@ def test_rk2_step_with_treecode_summation(tmp_path):
     s, r = rng.normal(0, 1.0 / n, n), np.full(n, 2e-3)
     dt = 0.01
     got = run_driver(tmp_path, "rktree", [n, 2, CORE_ID["gauss"], 0, 1],
-                     [np.float64(dt), x, y, s, r], 2 * n)
+                     [np.float64(dt), x, y, s, r], 4 * n)
     u0 = oracle_vel("gauss", x, y, s, r)
     x1, y1 = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * dt)
     u1 = oracle_vel("gauss", x1, y1, s, r)
     ex, ey = O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt)
     scale = dt * np.max(np.hypot(u0[0], u0[1]))
     assert np.max(np.abs(got[:n] - ex)) <= 1e-9 * scale
-    assert np.max(np.abs(got[n:] - ey)) <= 1e-9 * scale
+    assert np.max(np.abs(got[n:2 * n] - ey)) <= 1e-9 * scale
 
 
 def test_rk_step_float_storage(tmp_path):
     # the reference application's STORE=float through the _f entry point
     w = W.c1_disc(n=2000)
     x, y, s, r = (w[k].astype(np.float32).astype(np.float64) for k in ("x", "y", "s", "r"))
-    got = run_driver(tmp_path, "rk", [len(x), 2, CORE_ID["rm"], 1, 1], [np.float64(0.01), x, y, s, r],
-                     2 * len(x))
+    n = len(x)
+    got = run_driver(tmp_path, "rk", [n, 2, CORE_ID["rm"], 1, 1], [np.float64(0.01), x, y, s, r],
+                     4 * n)
     u0 = oracle_vel("rm", x, y, s, r)
     x1, y1 = O.rk_combine(x, y, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * 0.01)
     u1 = oracle_vel("rm", x1, y1, s, r)
     ex, ey = O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], 0.01)
     # positions stored as float: agreement to float rounding of the positions
-    assert np.max(np.abs(got[: len(x)] - ex)) <= 1e-7 and np.max(np.abs(got[len(x):] - ey)) <= 1e-7
+    assert np.max(np.abs(got[:n] - ex)) <= 1e-7 and np.max(np.abs(got[n:2 * n] - ey)) <= 1e-7
+    # and the combined velocity, stored as float
+    eu, ev = 0.25 * u0[0] + 0.75 * u1[0], 0.25 * u0[1] + 0.75 * u1[1]
+    assert rel_l2(got[2 * n:3 * n], eu) <= 1e-6 and rel_l2(got[3 * n:], ev) <= 1e-6
 
 
 class PyBem:
@@ -205,7 +217,7 @@ def test_bem_rk2_step_with_wake(tmp_path):
     dt, ips = 0.01, 0.5  # a layer 0.1 thick: wake particles start from radius 0.52
     header = [len(idx) // 2, len(nx), nw, 2, CORE_ID["v2"], 2]
     payload = [np.array([1.0, 0.0, dt, ips]), nx, ny, idx, w["x"], w["y"], w["s"], w["r"]]
-    got = run_driver(tmp_path, "bemstep", header, payload, 2 * nw)
+    got = run_driver(tmp_path, "bemstep", header, payload, 4 * nw)
     ref = PyBem(nx, ny, idx, 2, "v2", (1.0, 0.0))
     x, y, s, r = w["x"], w["y"], w["s"], w["r"]
     n64x, n64y = nx.astype(np.float64), ny.astype(np.float64)
@@ -225,7 +237,10 @@ def test_bem_rk2_step_with_wake(tmp_path):
     u1 = ref.wake_vel(x1, y1, s, r)
     ex, ey = clear(*O.rk_combine(x, y, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt))
     assert min(pushed) > 0
-    assert np.max(np.abs(got[:nw] - ex)) <= 1e-12 and np.max(np.abs(got[nw:] - ey)) <= 1e-12
+    assert np.max(np.abs(got[:nw] - ex)) <= 1e-12 and np.max(np.abs(got[nw:2 * nw] - ey)) <= 1e-12
+    # the particles hold the stage combination 0.25 u0 + 0.75 u1 (src/ElementBase.h:308-312)
+    assert rel_l2(got[2 * nw:3 * nw], 0.25 * u0[0] + 0.75 * u1[0]) <= 1e-10
+    assert rel_l2(got[3 * nw:], 0.25 * u0[1] + 0.75 * u1[1]) <= 1e-10
 
 
 def test_find_vort_self(tmp_path):
