@@ -131,6 +131,18 @@ int o2d_hip_points_on_points_dev_d(const o2d_env* env, int64_t ns, const double*
                                    int64_t nt, const double* tx, const double* ty,
                                    const double* trad, double* tu, double* tv, double* tw,
                                    double* te);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same sum as
+ * o2d_hip_points_on_points_dev_d (same arguments, both precision forms, every core and result
+ * type) with its temporaries in a fixed 64 MiB array per device instead of the library's
+ * buffer pool; it calls no hipMalloc, hipFree or allocating library routine. At most 262,144
+ * sources and 262,144 targets (O2D_ERR_SIZE beyond). Calls that overlap on one device race on
+ * that array. */
+int o2d_hip_points_on_points_noalloc_dev_d(const o2d_env* env, int64_t ns, const double* sx,
+                                           const double* sy, const double* sstr,
+                                           const double* srad, int64_t nt, const double* tx,
+                                           const double* ty, const double* trad, double* tu,
+                                           double* tv, double* tw, double* te);
 
 /* ---- panels on points: panels_affect_points (src/Influence.h:422-614) -------------------- */
 

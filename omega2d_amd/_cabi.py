@@ -65,6 +65,8 @@ SIGNATURES = {
                                    _vp, _vp],
     "o2d_hip_points_on_points_dev_d": [_envp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
                                        _vp, _vp, _vp],
+    "o2d_hip_points_on_points_noalloc_dev_d": [_envp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                                               _vp, _vp, _vp, _vp, _vp],
     "o2d_hip_panels_on_points_d": [_envp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                    _vp],
     "o2d_hip_panels_on_points_dev_d": [_envp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp,

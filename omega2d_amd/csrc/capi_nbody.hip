@@ -42,7 +42,7 @@ int download_d2f(Device* d, Slot stage, float* h, const double* dptr, int64_t n,
   return download<float>(h, static_cast<const float*>(p), n, s);
 }
 
-static int check_pp(const o2d_env* env, int64_t ns, const void* sx, const void* sy,
+int check_pp(const o2d_env* env, int64_t ns, const void* sx, const void* sy,
                     const void* ss, const void* sr, int64_t nt, const void* tx, const void* ty,
                     const void* tu, const void* tv, const void* tw, const void* te) {
   O2D_TRY(check_env(env));

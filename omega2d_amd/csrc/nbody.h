@@ -9,15 +9,23 @@
 
 namespace o2d {
 
-// particles on points, device pointers, enqueued on s (tw is the vorticity or shear output)
+// particles on points, device pointers, enqueued on s (tw is the vorticity or shear output);
+// temporaries come from the device's buffer pool, or from *ws when it is given
 int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns, const double* sx,
               const double* sy, const double* ss, const double* sr, int64_t nt, const double* tx,
-              const double* ty, const double* tr, double* tu, double* tv, double* tw);
+              const double* ty, const double* tr, double* tu, double* tv, double* tw,
+              Workspace* ws = nullptr);
+
+// argument checks of the particles-on-points entry points (counts, NULL arrays, outputs the
+// result type needs)
+int check_pp(const o2d_env* env, int64_t ns, const void* sx, const void* sy, const void* ss,
+             const void* sr, int64_t nt, const void* tx, const void* ty, const void* tu,
+             const void* tv, const void* tw, const void* te);
 
 // 32-bit Morton keys and stable order (bbox: derive the box on device from x, y)
 int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const double* y,
                  bool bbox, double x0, double y0, double inv, uint32_t* keys_out,
-                 uint32_t* order_out);
+                 uint32_t* order_out, Workspace* ws = nullptr);
 
 int panels_on_points_dev(Device* d, hipStream_t s, int64_t nnodes, const double* nx,
                          const double* ny, int64_t np, const uint32_t* idx, const double* vs,

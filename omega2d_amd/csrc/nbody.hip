@@ -650,8 +650,9 @@ struct ChunkPlan {
 
 // Splits the source tiles into chunks (grid.y) so that the launch has per_cu workgroups per CU.
 // Beyond 8 per CU a chunk keeps at least kMinChunkTiles tiles, so that few targets do not turn
-// into thousands of tiny chunks and partial sums.
-ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles, int per_cu) {
+// into thousands of tiny chunks and partial sums. At most max_chunks chunks.
+ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles, int per_cu,
+                      int64_t max_chunks) {
   const auto chunks_for = [&](int64_t wgs_per_cu) {
     const int64_t want = wgs_per_cu * (int64_t)d->num_cu;
     return target_wgs >= want ? (int64_t)1 : (want + target_wgs - 1) / target_wgs;
@@ -659,6 +660,7 @@ ChunkPlan plan_chunks(const Device* d, int64_t target_wgs, int64_t ntiles, int p
   int64_t nchunk = chunks_for(per_cu);
   const int64_t most = std::max(chunks_for(8), (ntiles + kMinChunkTiles - 1) / kMinChunkTiles);
   if (nchunk > most) nchunk = most;
+  if (nchunk > max_chunks) nchunk = max_chunks;
   if (nchunk > ntiles) nchunk = ntiles;
   if (nchunk < 1) nchunk = 1;
   const int64_t per = (ntiles + nchunk - 1) / nchunk;
@@ -767,15 +769,15 @@ int gather_targets(hipStream_t s, int core, int64_t n, const uint32_t* order, co
 // Morton keys and the stable key order; bbox == true derives the box from the data on device.
 int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const double* y,
                  bool bbox, double x0, double y0, double inv, uint32_t* keys_out,
-                 uint32_t* order_out) {
+                 uint32_t* order_out, Workspace* ws) {
   if (n <= 0) return 0;
   void *kA, *oA, *bbp = nullptr;
-  O2D_TRY(slot(d, kKeysA, n * sizeof(uint32_t), s, &kA));
-  O2D_TRY(slot(d, kOrderA, n * sizeof(uint32_t), s, &oA));
+  O2D_TRY(take(d, ws, kKeysA, n * sizeof(uint32_t), s, &kA));
+  O2D_TRY(take(d, ws, kOrderA, n * sizeof(uint32_t), s, &oA));
   BBox* bb = nullptr;
   if (bbox) {
     const int nblk = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 1024);
-    O2D_TRY(slot(d, kBBox, sizeof(double) * (4 * nblk + 4), s, &bbp));
+    O2D_TRY(take(d, ws, kBBox, sizeof(double) * (4 * nblk + 4), s, &bbp));
     double* part = static_cast<double*>(bbp);
     bb = reinterpret_cast<BBox*>(part + 4 * nblk);
     bbox_partial_kernel<<<nblk, kBlock, 0, s>>>(n, x, y, part);
@@ -790,11 +792,19 @@ int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const dou
                                              keys_out, static_cast<uint32_t*>(oA), order_out,
                                              (int)n, 0, 32, s));
   void* tmp;
-  O2D_TRY(slot(d, kSortTemp, tbytes, s, &tmp));
+  O2D_TRY(take(d, ws, kSortTemp, tbytes, s, &tmp));
   O2D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, static_cast<uint32_t*>(kA), keys_out,
                                              static_cast<uint32_t*>(oA), order_out, (int)n, 0, 32,
                                              s));
   return 0;
+}
+
+// The most source chunks whose fp64 partials (nf per target) fit what is left of *ws; no limit
+// when the temporaries come from the buffer pool.
+int64_t max_chunks(const Workspace* ws, int nf, int64_t nt) {
+  if (!ws) return INT64_MAX;
+  const int64_t c = (int64_t)(ws->left() / (sizeof(double) * (size_t)nf * (size_t)nt));
+  return c < 1 ? 1 : c;
 }
 
 // Targets for the kernels: their positions and radius terms gathered into *pos and *w, and
@@ -802,21 +812,21 @@ int morton_order(Device* d, hipStream_t s, int64_t n, const double* x, const dou
 // themselves (the same position arrays) reuse the sources' order src_order; other targets are
 // put in Morton order when `sort`, else kept in input order (*order null).
 template <class W>
-int prep_targets(Device* d, hipStream_t s, int core, int64_t nt, const double* tx,
+int prep_targets(Device* d, hipStream_t s, Workspace* ws, int core, int64_t nt, const double* tx,
                  const double* ty, const double* tr, bool sort, const uint32_t* src_order,
                  const uint32_t** order, const double2** pos, const W** w) {
   const uint32_t* o = src_order;
   if (!o && sort) {
     void *kT, *oT;
-    O2D_TRY(slot(d, kTrgKeys, nt * sizeof(uint32_t), s, &kT));
-    O2D_TRY(slot(d, kTrgOrder, nt * sizeof(uint32_t), s, &oT));
+    O2D_TRY(take(d, ws, kTrgKeys, nt * sizeof(uint32_t), s, &kT));
+    O2D_TRY(take(d, ws, kTrgOrder, nt * sizeof(uint32_t), s, &oT));
     O2D_TRY(morton_order(d, s, nt, tx, ty, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kT),
-                         static_cast<uint32_t*>(oT)));
+                         static_cast<uint32_t*>(oT), ws));
     o = static_cast<const uint32_t*>(oT);
   }
   void *pp, *wp = nullptr;
-  O2D_TRY(slot(d, kTrgPos, nt * sizeof(double2), s, &pp));
-  if (tr) O2D_TRY(slot(d, kTrgW, nt * sizeof(W), s, &wp));
+  O2D_TRY(take(d, ws, kTrgPos, nt * sizeof(double2), s, &pp));
+  if (tr) O2D_TRY(take(d, ws, kTrgW, nt * sizeof(W), s, &wp));
   O2D_TRY(gather_targets<W>(s, core, nt, o, tx, ty, tr, static_cast<double2*>(pp),
                             static_cast<W*>(wp)));
   *order = o;
@@ -827,7 +837,8 @@ int prep_targets(Device* d, hipStream_t s, int core, int64_t nt, const double* t
 
 int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns, const double* sx,
               const double* sy, const double* ss, const double* sr, int64_t nt, const double* tx,
-              const double* ty, const double* tr, double* tu, double* tv, double* tw) {
+              const double* ty, const double* tr, double* tu, double* tv, double* tw,
+              Workspace* ws) {
   if (nt == 0 || ns == 0) return 0;
   // the radix sort of the Morton keys takes an int count
   if (ns > 0x7fffffffLL || nt > 0x7fffffffLL)
@@ -837,19 +848,19 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   const bool self = tx == sx && ty == sy && nt == ns;
   // sources in Morton order (both forms): the stable order of 32-bit Morton keys
   void *kB, *oB;
-  O2D_TRY(slot(d, kKeysB, ns * sizeof(uint32_t), s, &kB));
-  O2D_TRY(slot(d, kOrderB, ns * sizeof(uint32_t), s, &oB));
+  O2D_TRY(take(d, ws, kKeysB, ns * sizeof(uint32_t), s, &kB));
+  O2D_TRY(take(d, ws, kOrderB, ns * sizeof(uint32_t), s, &oB));
 
   if (prec == O2D_PLAIN_FP64) {
     // only the Gaussian's velocity sum uses locality (its far-field tiles); every other fp64
     // sum keeps the input order and skips the sort
     const bool sort = core == O2D_CORE_GAUSSIAN && res == O2D_VELONLY;
     void* sp;
-    O2D_TRY(slot(d, kSrc64, ns * sizeof(Src64), s, &sp));
+    O2D_TRY(take(d, ws, kSrc64, ns * sizeof(Src64), s, &sp));
     const uint32_t* order = nullptr;
     if (sort) {
       O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
-                           static_cast<uint32_t*>(oB)));
+                           static_cast<uint32_t*>(oB), ws));
       order = static_cast<const uint32_t*>(oB);
     }
     Src64* src = static_cast<Src64*>(sp);
@@ -864,20 +875,20 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
     const uint32_t* torder;
     const double2* tq;
     const double* twv;
-    O2D_TRY(prep_targets<double>(d, s, core, nt, tx, ty, tr, sort, self ? order : nullptr,
+    O2D_TRY(prep_targets<double>(d, s, ws, core, nt, tx, ty, tr, sort, self ? order : nullptr,
                                  &torder, &tq, &twv));
 
     const int64_t twgs = (nt + kBlock * kTpl64 - 1) / (kBlock * kTpl64);
     const int64_t ntiles = (ns + kTile - 1) / kTile;
-    ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT64);
+    const int nf = res == O2D_VELONLY ? 2 : 3;
+    ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT64, max_chunks(ws, nf, nt));
     const int64_t chunk = cp.per_chunk * kTile;
     const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
     double *ou = tu, *ov = tv, *ow = tw;
     const int direct = cp.nchunk == 1;
     if (!direct) {
       void* p;
-      const int nf = res == O2D_VELONLY ? 2 : 3;
-      O2D_TRY(slot(d, kPartials, sizeof(double) * nf * cp.nchunk * nt, s, &p));
+      O2D_TRY(take(d, ws, kPartials, sizeof(double) * nf * cp.nchunk * nt, s, &p));
       ou = static_cast<double*>(p);
       ov = ou + cp.nchunk * nt;
       ow = res == O2D_VELONLY ? nullptr : ov + cp.nchunk * nt;
@@ -905,11 +916,11 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   const int64_t npad = ntiles * kTile;
   const int64_t ngroups = npad / kGroup;
   void *tsrc, *tcent;
-  O2D_TRY(slot(d, kTileSrc, npad * sizeof(float4), s, &tsrc));
-  O2D_TRY(slot(d, kTileCent, ngroups * sizeof(GroupInfo), s, &tcent));
+  O2D_TRY(take(d, ws, kTileSrc, npad * sizeof(float4), s, &tsrc));
+  O2D_TRY(take(d, ws, kTileCent, ngroups * sizeof(GroupInfo), s, &tcent));
   KernelTimer kprep(O2D_KT_SPLIT_PREP, s);
   O2D_TRY(morton_order(d, s, ns, sx, sy, true, 0.0, 0.0, 0.0, static_cast<uint32_t*>(kB),
-                       static_cast<uint32_t*>(oB)));
+                       static_cast<uint32_t*>(oB), ws));
   const uint32_t* order = static_cast<const uint32_t*>(oB);
   GroupInfo* cent = static_cast<GroupInfo*>(tcent);
   float4* src = static_cast<float4*>(tsrc);
@@ -925,19 +936,19 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
   const double2* tq;
   const float* twv;
   // the target order matters to the Gaussian's wave-uniform far-field test only
-  O2D_TRY(prep_targets<float>(d, s, core, nt, tx, ty, tr, core == O2D_CORE_GAUSSIAN,
+  O2D_TRY(prep_targets<float>(d, s, ws, core, nt, tx, ty, tr, core == O2D_CORE_GAUSSIAN,
                               self ? order : nullptr, &torder, &tq, &twv));
   kprep.stop();
 
   const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);
-  ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT32);
+  const int nf = res == O2D_VELONLY ? 2 : 3;
+  ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT32, max_chunks(ws, nf, nt));
   const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);
   double *ou = tu, *ov = tv, *ow = tw;
   const int direct = cp.nchunk == 1;
   if (!direct) {
     void* p;
-    const int nf = res == O2D_VELONLY ? 2 : 3;
-    O2D_TRY(slot(d, kPartials, sizeof(double) * nf * cp.nchunk * nt, s, &p));
+    O2D_TRY(take(d, ws, kPartials, sizeof(double) * nf * cp.nchunk * nt, s, &p));
     ou = static_cast<double*>(p);
     ov = ou + cp.nchunk * nt;
     ow = res == O2D_VELONLY ? nullptr : ov + cp.nchunk * nt;

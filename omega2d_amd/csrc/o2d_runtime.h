@@ -69,6 +69,22 @@ inline hipStream_t dev_stream(const o2d_env* env) { return static_cast<hipStream
 // Device buffer for `slot` with at least `bytes`; grows (after draining `s`) when too small.
 int slot(Device* d, Slot which, size_t bytes, hipStream_t s, void** out);
 
+// A fixed device buffer handed out in consecutive 256-byte-aligned pieces, for calls that
+// must not allocate.
+struct Workspace {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  // bytes still free after aligning the next piece
+  size_t left() const {
+    const size_t off = (used + 255) & ~size_t(255);
+    return off < cap ? cap - off : 0;
+  }
+};
+// Temporary device memory: `slot` when ws is null, else the next piece of *ws (an error when
+// the workspace is too small; nothing is allocated).
+int take(Device* d, Workspace* ws, Slot which, size_t bytes, hipStream_t s, void** out);
+
 int check_env(const o2d_env* env);
 
 // Optional per-kernel timing: KernelTimer t(cls, s); ... launch ...; t.stop();

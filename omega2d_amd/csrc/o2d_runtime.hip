@@ -70,6 +70,18 @@ int slot(Device* d, Slot which, size_t bytes, hipStream_t s, void** out) {
   return 0;
 }
 
+int take(Device* d, Workspace* ws, Slot which, size_t bytes, hipStream_t s, void** out) {
+  if (!ws) return slot(d, which, bytes, s, out);
+  if (bytes == 0) bytes = 256;
+  const size_t off = (ws->used + 255) & ~size_t(255);
+  if (off > ws->cap || bytes > ws->cap - off)
+    return fail(O2D_ERR_SIZE, "the %zu-byte workspace is too small: %zu bytes are in use and "
+                              "%zu more are needed", ws->cap, off, bytes);
+  *out = ws->base + off;
+  ws->used = off + bytes;
+  return 0;
+}
+
 int check_env(const o2d_env* env) {
   if (!env) return fail(O2D_ERR_ARG, "env is NULL");
   if (env->core < O2D_CORE_RM || env->core > O2D_CORE_V3)
