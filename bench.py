@@ -523,7 +523,7 @@ def cpu_reference_c3(seconds):
     per-stage pieces of the C3 workload (2,048 panels, 200,000 wake particles, Vatistas n=2, the
     reference's compiled-in core), each on a bounded target sample, in (target, source) pairs
     per second. The library's times for the same pieces come from
-    `python -m omega2d_amd.bench_configs --configs c3`."""
+    `python bench_configs.py --configs c3`."""
     try:
         import oracle as O
         from omega2d_amd import workloads as wl

@@ -1,6 +1,6 @@
 """Per-configuration measurements beside the headline bench line (one JSON object per config).
 
-    python -m omega2d_amd.bench_configs [--configs c1,c2host,c2tree,c3,c3step,c4,c5] [--out FILE]
+    python bench_configs.py [--configs c1,c2host,c2tree,c3,c3step,c4,c5,...] [--out FILE]
 
 C1  10k disc, RM core, one Ralston RK2 step: split and plain velocity evaluations, fp64 update;
     the Gaussian vorticity of the disc on itself
@@ -10,9 +10,21 @@ C3  circle of 2048 panels + 200k wake, V2 core: BEM assembly (vortex-only + augm
     and vortex+source + augmented 4097^2) with HBM GB/s, LU factor + solve, RHS (points on
     panels), panels on particles, particles on particles (split and plain), and the
     particle/panel proximity scans run after every move (clearing pass, nearest distances)
+C3step  whole RK2 steps of C3 through the C++ host mirror (tests/cpp/host_driver.cpp, built
+    by python -m omega2d_amd.build), host arrays over PCIe
 C4  250k particles -> 4096 probes, Gaussian core: forward and adjoint (fp64), ratio
 C5  4M particles in a shear layer, RM core, one Ralston RK3 step in the split form with the
     fp64 position update; error of the split velocity against the fp64 form on 16,384 targets
+
+Short drivers for profiler runs (each prints one JSON line; run them under rocprofv3):
+lu        three factorisations of a random 2049^2 matrix, then three of a 4097^2 one
+assemble  the C3 assembly: vortex-only + augmented 2049^2, then vortex+source + augmented
+          4097^2, three times each
+treewalk  the C2 treecode evaluation (theta 0.5, order 20) twice on the particle targets
+near      the C3 proximity scans: three clearing passes (method 1) and three nearest-distance
+          scans of the 200k wake against the 2,048 panels
+panels    the C3 panel sums: three points-on-panels (RHS) and three panels-on-points launches
+fp64c2    two fp64 evaluations of the C2 particles on themselves (Gaussian core)
 
 Timing: torch.cuda.synchronize() around each measured piece; kernel times from the library's
 HIP-event timers where a single kernel dominates. Inputs are resident in HBM.
@@ -20,17 +32,22 @@ HIP-event timers where a single kernel dominates. Inputs are resident in HBM.
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
-import numpy as np
-import torch
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
 
-from . import device as dv
-from . import flops_per_pair, workloads as wl
-from ._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN, KT_NBODY_PLAIN_PT,
-                    KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS, KT_PANELS, KT_TREE_BUILD,
-                    KT_TREE_EVAL, KT_VORT, timing_enable, timing_read)
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+from omega2d_amd import device as dv  # noqa: E402
+from omega2d_amd import flops_per_pair, workloads as wl  # noqa: E402
+from omega2d_amd._cabi import (KT_ADJOINT, KT_BEM_ASSEMBLE, KT_NBODY_PLAIN,  # noqa: E402
+                               KT_NBODY_PLAIN_PT, KT_NBODY_SPLIT, KT_NEAR, KT_PANEL_RHS,
+                               KT_PANELS, KT_TREE_BUILD, KT_TREE_EVAL, KT_VORT, timing_enable,
+                               timing_read)
 
 PEAK_FP32, PEAK_FP64, PEAK_HBM = 157.3e12, 78.6e12, 8.0e12
 
@@ -270,10 +287,9 @@ def c3step(nsteps=10):
     them and clears the inner layer (method 1). The driver passes host arrays, so every call
     copies its inputs over PCIe, as the reference's call would. Wall time per step after three
     untimed steps, split and fp64 pair sums."""
-    import os
     import subprocess
     import tempfile
-    from . import build as B
+    from omega2d_amd import build as B
     w = wl.c3_circle_wake()
     npan, nw = len(w["idx"]) // 2, len(w["x"])
     res = {"config": "C3step", "name": w["name"], "panels": npan, "wake": nw, "core": "v2",
@@ -368,13 +384,113 @@ def c2tree():
     return res
 
 
+def _c3_panels():
+    w = wl.c3_circle_wake()
+    nx, ny = T(w["nx"]), T(w["ny"])
+    idx = torch.from_numpy(w["idx"].astype(np.int32)).cuda()
+    return w, nx, ny, idx
+
+
+def lu():
+    """Profiler driver: three LU factorisations of a random 2049^2 matrix, then of a 4097^2
+    one (the wall time of each third factorisation is reported)."""
+    rng = np.random.default_rng(0)
+    res = {"config": "lu"}
+    for n in (2049, 4097):
+        A0 = T(rng.normal(size=(n, n)))
+        for _ in range(3):
+            A = A0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dv.bem_factor(A)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+        res[f"factor_{n}_ms"] = round(t * 1e3, 3)
+    return res
+
+
+def assemble():
+    """Profiler driver: the C3 assembly, vortex-only + augmented 2049^2 and vortex+source +
+    augmented 4097^2, three launches each."""
+    w, nx, ny, idx = _c3_panels()
+    npan = len(w["idx"]) // 2
+    rng = np.random.default_rng(0)
+    uu, uv = T(rng.normal(size=npan)), T(rng.normal(size=npan))
+    res = {"config": "assemble"}
+    for nunk in (1, 2):
+        n = npan * nunk + 1
+        A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        ms, _ = kernel_ms(KT_BEM_ASSEMBLE, lambda: dv.bem_assemble(nx, ny, idx, nunk, True, uu,
+                                                                  uv, 0.78, out=A), reps=3)
+        res[f"assemble_{n}_kernel_ms"] = round(ms, 4)
+        res[f"assemble_{n}_bytes_written"] = 8 * n * n
+    return res
+
+
+def treewalk():
+    """Profiler driver: the C2 treecode evaluation (theta 0.5, order 20) on the 1M particle
+    targets, twice."""
+    w = wl.c2_blobs()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    for _ in range(2):
+        dv.points_on_points_tree(x, y, s, r, x, y, r, theta=0.5, order=20)
+    torch.cuda.synchronize()
+    return {"config": "treewalk", "particles": x.numel()}
+
+
+def near():
+    """Profiler driver: three clearing passes (method 1, the reference's layer at spacing
+    0.005) and three nearest-distance scans of the C3 wake against its 2,048 panels."""
+    w, nx, ny, idx = _c3_panels()
+    x, y = T(w["x"]), T(w["y"])
+    a, b = w["idx"][0::2], w["idx"][1::2]
+    tx_, ty_ = w["nx"][b] - w["nx"][a], w["ny"][b] - w["ny"][a]
+    ln = np.sqrt(tx_ * tx_ + ty_ * ty_)
+    pnx, pny = T(-ty_ * (1.0 / ln)), T(tx_ * (1.0 / ln))
+    cm, ips = 0.5 / math.sqrt(2.0 * math.pi), 0.005
+    res = {"config": "near"}
+    for name, fn in (("clear_inner", lambda: dv.clear_inner(1, nx, ny, idx, pnx, pny, cm, ips,
+                                                            x.clone(), y.clone())),
+                     ("nearest_distances", lambda: dv.nearest_distances(nx, ny, idx, x, y))):
+        ms, _ = kernel_ms(KT_NEAR, fn, reps=3)
+        res[name + "_kernel_ms"] = round(ms, 4)
+    return res
+
+
+def panels():
+    """Profiler driver: three points-on-panels (the BEM right-hand side from the 200k wake) and
+    three panels-on-points (the 2,048 panels on the wake) launches."""
+    w, nx, ny, idx = _c3_panels()
+    npan = len(w["idx"]) // 2
+    x, y, s = T(w["x"]), T(w["y"]), T(w["s"])
+    rng = np.random.default_rng(0)
+    vs, ss = T(rng.normal(size=npan)), T(rng.normal(size=npan))
+    res = {"config": "panels"}
+    ms, _ = kernel_ms(KT_PANEL_RHS, lambda: dv.points_on_panels(x, y, s, nx, ny, idx), reps=3)
+    res["points_on_panels_kernel_ms"] = round(ms, 4)
+    ms, _ = kernel_ms(KT_PANELS, lambda: dv.panels_on_points(nx, ny, idx, vs, ss, x, y), reps=3)
+    res["panels_on_points_kernel_ms"] = round(ms, 4)
+    return res
+
+
+def fp64c2():
+    """Profiler driver: two fp64 evaluations of the C2 particles on themselves (Gaussian)."""
+    w = wl.c2_blobs()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    ms, n = kernel_ms(KT_NBODY_PLAIN, lambda: dv.points_on_points(x, y, s, r, x, y, r,
+                                                                  core="gauss",
+                                                                  precision="fp64"))
+    return {"config": "fp64c2", "particles": x.numel(), "kernel_ms": round(ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--configs", default="c1,c3,c4,c5")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     fns = {"c1": c1, "c2host": c2host, "c2tree": c2tree, "c3": c3, "c3step": c3step,
-           "c4": c4, "c5": c5}
+           "c4": c4, "c5": c5, "lu": lu, "assemble": assemble, "treewalk": treewalk,
+           "near": near, "panels": panels, "fp64c2": fp64c2}
     out = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()

@@ -60,6 +60,7 @@ constexpr int kGroup = 64;   // sources sharing one centroid in the split form
 #define O2D_WANT64 128
 #endif
 constexpr int kMinChunkTiles = 16;
+constexpr int kStage = 4;  // source tiles brought into LDS per stage of the split kernel
 constexpr int kTpl64 = O2D_TPL64;  // targets per lane, plain form
 constexpr int kTpl32 = O2D_TPL32;  // targets per lane, split form
 
@@ -283,7 +284,14 @@ __device__ __forceinline__ void group_pairs(const float4* __restrict__ q0, const
 }
 
 // sources prepared by the split prep: fp32 (x - c, y - c, strength, radius term) in Morton
-// order, padded to whole tiles; grp[g] describes group g
+// order, padded to whole tiles; grp[g] describes group g.
+//
+// A workgroup owns 64 * TPL targets (TPL per lane), and its kWaves waves all hold those same
+// targets: each stage brings kStage tiles (kStage * kTile sources) into LDS, and wave w takes
+// groups w, w + kWaves, ... of the stage. After the chunk's last stage the waves' fp64 sums of a
+// target are added in LDS in wave order, so one workgroup writes one partial per target per
+// chunk: a quarter of the chunk partials that a workgroup of four waves with a target set each
+// would write, for the same work per wave.
 template <int CORE, int RES, bool TR, int TPL, int UNROLL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(O2D_WAVES32, 8)))
 void nbody32_kernel(
@@ -291,18 +299,25 @@ void nbody32_kernel(
     int64_t nt, const double2* __restrict__ tq, const float* __restrict__ tw,
     const uint32_t* __restrict__ torder, int64_t tiles_per_chunk, double* __restrict__ ou,
     double* __restrict__ ov, double* __restrict__ ow, int direct) {
-  constexpr int kGroups = kTile / kGroup;
+  constexpr int kGroups = kTile / kGroup;  // groups per tile
+  constexpr int kWaves = kBlock / 64;
+  constexpr int kTargets = 64 * TPL;       // targets per workgroup
   constexpr bool kAux = RES != O2D_VELONLY;
-  __shared__ float4 tile[kTile];
-  __shared__ GroupInfo tgrp[kGroups];
-  const int tid = threadIdx.x;
-  const int64_t tbase = (int64_t)blockIdx.x * (kBlock * TPL) + tid;
+  constexpr int kComp = kAux ? 3 : 2;
+  // the staged tiles, and after the last stage the waves' sums [component][wave][target]
+  constexpr int kRedDoubles = kComp * kWaves * kTargets;
+  constexpr int kTileDoubles = kStage * kTile * (int)(sizeof(float4) / sizeof(double));
+  __shared__ __align__(16) double lds[kRedDoubles > kTileDoubles ? kRedDoubles : kTileDoubles];
+  __shared__ GroupInfo tgrp[kStage * kGroups];
+  float4* tile = reinterpret_cast<float4*>(lds);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t t0 = (int64_t)blockIdx.x * kTargets;
 
   double px[TPL], py[TPL], au[TPL], av[TPL], aw[TPL];
   float pw[TPL];
 #pragma unroll
   for (int t = 0; t < TPL; ++t) {
-    const int64_t i = tbase + (int64_t)t * kBlock;
+    const int64_t i = t0 + lane + 64 * t;
     const int64_t ii = i < nt ? i : nt - 1;
     const double2 p = tq[ii];
     px[t] = p.x;
@@ -310,14 +325,13 @@ void nbody32_kernel(
     pw[t] = TR ? tw[ii] : 0.0f;
     au[t] = av[t] = aw[t] = 0.0;
   }
-  // when every target of the block has the same radius term w0, it is added to each source's
-  // term once as the tile is staged, and the pair loop runs without the per-pair add (the same
-  // fp32 sum, so the same result)
+  // when every target of the workgroup has the same radius term w0, it is added to each
+  // source's term once as the tile is staged, and the pair loop runs without the per-pair add
+  // (the same fp32 sum, so the same result)
   float w0 = 0.0f;
   bool uni = false;
   if constexpr (TR) {
-    const int64_t i0 = (int64_t)blockIdx.x * (kBlock * TPL);
-    w0 = tw[i0 < nt ? i0 : nt - 1];
+    w0 = tw[t0 < nt ? t0 : nt - 1];
     bool same = true;
 #pragma unroll
     for (int t = 0; t < TPL; ++t) same = same && pw[t] == w0;
@@ -326,15 +340,25 @@ void nbody32_kernel(
 
   const int64_t tb = (int64_t)blockIdx.y * tiles_per_chunk;
   const int64_t te = tb + tiles_per_chunk < ntiles ? tb + tiles_per_chunk : ntiles;
-  for (int64_t it = tb; it < te; ++it) {
-    float4 s = src[it * kTile + tid];
-    if (uni) s.w = wsum(s.w, w0);
+  for (int64_t it = tb; it < te; it += kStage) {
+    const int nst = te - it < kStage ? (int)(te - it) : kStage;  // tiles in this stage
+    // loaded after the barrier, straight into LDS: sixteen staging registers held across it
+    // would push the kernel past its register budget (the other resident workgroups of the CU
+    // cover the load latency)
     __syncthreads();
-    tile[tid] = s;
-    if (tid < kGroups) tgrp[tid] = grp[it * kGroups + tid];
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+      if (k < nst) {
+        float4 q = src[(it + k) * kTile + tid];
+        if (uni) q.w = wsum(q.w, w0);
+        tile[k * kTile + tid] = q;
+      }
+    }
+    if (tid < nst * kGroups) tgrp[tid] = grp[it * kGroups + tid];
     __syncthreads();
+    const int ng = nst * kGroups;
 #pragma unroll 1
-    for (int g = 0; g < kGroups; ++g) {
+    for (int g = wave; g < ng; g += kWaves) {
       const GroupInfo cg = tgrp[g];
       float ex[TPL], ey[TPL], fu[TPL], fv[TPL], fw[TPL];
 #pragma unroll
@@ -345,6 +369,7 @@ void nbody32_kernel(
         fv[t] = 0.0f;
         fw[t] = 0.0f;
       }
+      const float4* q0 = tile + g * kGroup;
       if constexpr (CORE == O2D_CORE_GAUSSIAN) {
         // every pair of the group is outside the compact core (d^2 > 16 w) when the target's
         // distance to the centroid exceeds the group radius plus 4 sqrt(wmax + w_target);
@@ -357,16 +382,15 @@ void nbody32_kernel(
           far = far && (e2 * (1.0f - 1e-5f) > lim * lim * (1.0f + 1e-5f));
         }
         if (__all(far))
-          group_pairs<CORE, RES, TR, TPL, 16, false>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
+          group_pairs<CORE, RES, TR, TPL, 16, false>(q0, ex, ey, pw, fu, fv, fw);
         else if (uni)
-          group_pairs<CORE, RES, false, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
+          group_pairs<CORE, RES, false, TPL, 1, true>(q0, ex, ey, pw, fu, fv, fw);
         else
-          group_pairs<CORE, RES, TR, TPL, 1, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
+          group_pairs<CORE, RES, TR, TPL, 1, true>(q0, ex, ey, pw, fu, fv, fw);
       } else if (uni) {
-        group_pairs<CORE, RES, false, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv,
-                                                         fw);
+        group_pairs<CORE, RES, false, TPL, UNROLL, true>(q0, ex, ey, pw, fu, fv, fw);
       } else {
-        group_pairs<CORE, RES, TR, TPL, UNROLL, true>(tile + g * kGroup, ex, ey, pw, fu, fv, fw);
+        group_pairs<CORE, RES, TR, TPL, UNROLL, true>(q0, ex, ey, pw, fu, fv, fw);
       }
 #pragma unroll
       for (int t = 0; t < TPL; ++t) {
@@ -377,20 +401,38 @@ void nbody32_kernel(
     }
   }
 
+  // the waves' sums of each target, added in wave order
+  __syncthreads();  // every wave is done with the last stage's tiles
+  double* red = lds;
 #pragma unroll
   for (int t = 0; t < TPL; ++t) {
-    const int64_t i = tbase + (int64_t)t * kBlock;
+    const int j = lane + 64 * t;
+    red[(0 * kWaves + wave) * kTargets + j] = au[t];
+    red[(1 * kWaves + wave) * kTargets + j] = av[t];
+    if (kAux) red[(2 * kWaves + wave) * kTargets + j] = aw[t];
+  }
+  __syncthreads();
+  for (int j = tid; j < kTargets; j += kBlock) {
+    const int64_t i = t0 + j;
     if (i >= nt) continue;
+    double su = red[j], sv = red[kWaves * kTargets + j], sw = 0.0;
+    if (kAux) sw = red[2 * kWaves * kTargets + j];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+      su += red[w * kTargets + j];
+      sv += red[(kWaves + w) * kTargets + j];
+      if (kAux) sw += red[(2 * kWaves + w) * kTargets + j];
+    }
     if (direct) {
       const int64_t o = torder ? (int64_t)torder[i] : i;
-      ou[o] += au[t];
-      ov[o] += av[t];
-      if (kAux) ow[o] += aw[t];
+      ou[o] += su;
+      ov[o] += sv;
+      if (kAux) ow[o] += sw;
     } else {
       const int64_t o = (int64_t)blockIdx.y * nt + i;
-      ou[o] = au[t];
-      ov[o] = av[t];
-      if (kAux) ow[o] = aw[t];
+      ou[o] = su;
+      ov[o] = sv;
+      if (kAux) ow[o] = sw;
     }
   }
 }
@@ -940,7 +982,7 @@ int nbody_dev(Device* d, hipStream_t s, int core, int prec, int res, int64_t ns,
                               self ? order : nullptr, &torder, &tq, &twv));
   kprep.stop();
 
-  const int64_t twgs = (nt + kBlock * kTpl32 - 1) / (kBlock * kTpl32);
+  const int64_t twgs = (nt + 64 * kTpl32 - 1) / (64 * kTpl32);  // one target set per workgroup
   const int nf = res == O2D_VELONLY ? 2 : 3;
   ChunkPlan cp = plan_chunks(d, twgs, ntiles, O2D_WANT32, max_chunks(ws, nf, nt));
   const dim3 grid((unsigned)twgs, (unsigned)cp.nchunk);

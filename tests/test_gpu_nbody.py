@@ -125,7 +125,7 @@ def test_split_form_keeps_its_accuracy_far_from_the_origin(core):
 @pytest.mark.parametrize("results", ["vel", "vort"])
 @pytest.mark.parametrize("precision", ["split", "fp64"])
 def test_uniform_target_radius_fold_is_bit_identical(core, results, precision):
-    """When every target of a workgroup (1,024 targets split, 512 fp64) has the same radius, the
+    """When every target of a workgroup (256 targets split, 512 fp64) has the same radius, the
     kernel adds that radius term to each staged source once instead of per pair: the same sum,
     so the same bits. Changing one target's radius in the first workgroup takes that workgroup
     off the fold and leaves every other target's result unchanged, bit for bit."""
