@@ -213,6 +213,11 @@ int o2d_hip_bem_assemble_pair_dev_d(const o2d_env* env, int64_t snn, const doubl
  * src/BEM.h:182-195). A is a host array (_d) or device array (_dev_d, not modified). */
 int o2d_hip_bem_factor_d(const o2d_env* env, int64_t n, const double* A, int64_t lda);
 int o2d_hip_bem_factor_dev_d(const o2d_env* env, int64_t n, const double* A, int64_t lda);
+/* The stored factors to host memory in LAPACK getrf's layout: lu (n x n, column-major) holds
+ * L (unit diagonal, not stored) and U, ipiv the 1-based row interchanges. No counterpart in the
+ * reference, whose GMRES (src/BEM.h:182-202) keeps no factors: it lets a caller check the
+ * factors and pivots against a library getrf. */
+int o2d_hip_bem_factors_d(const o2d_env* env, int64_t n, double* lu, int32_t* ipiv);
 /* x = A^-1 b with the stored factors (x may alias b). */
 int o2d_hip_bem_solve_d(const o2d_env* env, int64_t n, const double* b, double* x);
 int o2d_hip_bem_solve_dev_d(const o2d_env* env, int64_t n, const double* b, double* x);

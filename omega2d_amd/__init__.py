@@ -148,6 +148,15 @@ def bem_factor(A, *, device=0):
     call("o2d_hip_bem_factor_d", ctypes.byref(env), n, _p(colmajor), n)
 
 
+def bem_factors(n, *, device=0):
+    """The kept factors as LAPACK getrf returns them: (lu[row, col], 1-based ipiv)."""
+    lu = np.empty(n * n, dtype=np.float64)
+    ipiv = np.empty(n, dtype=np.int32)
+    env = make_env("v2", "fp64", "vel", device)
+    call("o2d_hip_bem_factors_d", ctypes.byref(env), n, _p(lu), _p(ipiv))
+    return lu.reshape(n, n).T.copy(), ipiv
+
+
 def bem_solve(b, *, device=0):
     b = _arr(b, np.float64).copy()
     env = make_env("v2", "fp64", "vel", device)

@@ -84,6 +84,7 @@ SIGNATURES = {
                                         _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i64],
     "o2d_hip_bem_factor_d": [_envp, _i64, _vp, _i64],
     "o2d_hip_bem_factor_dev_d": [_envp, _i64, _vp, _i64],
+    "o2d_hip_bem_factors_d": [_envp, _i64, _vp, _vp],
     "o2d_hip_bem_solve_d": [_envp, _i64, _vp, _vp],
     "o2d_hip_bem_solve_dev_d": [_envp, _i64, _vp, _vp],
     "o2d_hip_rk_move_d": [_envp, _i64, _dbl, _i32, _vp, _vp, _vp, _vp, _vp],

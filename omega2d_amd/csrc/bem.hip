@@ -496,6 +496,21 @@ int o2d_hip_bem_factor_d(const o2d_env* env, int64_t n, const double* A, int64_t
   return bem_factor_dev(d, s, n, static_cast<double*>(dA), n);
 }
 
+int o2d_hip_bem_factors_d(const o2d_env* env, int64_t n, double* lu, int32_t* ipiv) {
+  clear_error();
+  if (!env) return fail(O2D_ERR_ARG, "env is NULL");
+  if (!lu || !ipiv) return fail(O2D_ERR_ARG, "NULL output");
+  Device* d;
+  O2D_TRY(device(env->device, &d));
+  if (!d->lu_ready || d->lu_n != n)
+    return fail(O2D_ERR_STATE, "no LU factors of size %lld (factor first)", (long long)n);
+  hipStream_t s = stream_of(env, d);
+  O2D_HIP(hipMemcpyAsync(lu, d->buf[kLU], sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
+  O2D_HIP(hipMemcpyAsync(ipiv, d->buf[kPivots], sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  O2D_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 int o2d_hip_bem_solve_dev_d(const o2d_env* env, int64_t n, const double* b, double* x) {
   clear_error();
   if (!env) return fail(O2D_ERR_ARG, "env is NULL");

@@ -4,24 +4,26 @@
 //
 // A library dgetrf on a matrix of a few thousand rows spends its time in three kernel launches
 // per column (pivot search, singularity check, scale and update), microseconds each. Here the
-// factorisation is right-looking in blocks of kNB columns, each factored in leaves of kLeaf:
-//   lu_leaf_kernel    one workgroup (kLeafThreads) holds the leaf's rows in registers and
-//                     does the pivot search, interchange, scaling and rank-1 updates of its
-//                     columns with two workgroup barriers per column; then it applies the
-//                     leaf's interchanges to the block's other columns (as one permutation of at
-//                     most 2 kLeaf rows), solves the block's top rows right of the leaf
-//                     (U12 = L11^-1 A12), and after the block's last leaf lists the block's
-//                     interchanges as (destination, source) row pairs; a pair's source is
-//                     found by undoing the interchanges in reverse order, so no lane waits
-//                     on another;
-//   lu_update_kernel  the rank-kLeaf update of the block's columns right of the leaf;
-//   lu_laswp_kernel   the block's row pairs applied to every column outside the block, all
-//                     loads of a column before its stores;
-//   lu_trsm_kernel    the block's top rows of the trailing columns, L11^-1 A12 (one column per
-//                     lane, L11 in LDS);
+// factorisation is right-looking in blocks of kNB columns, with four launches per block:
+//   lu_panel_kernel   one workgroup factors the block's panel (rows P..n-1, kNB columns) in
+//                     leaves of kLeaf columns, left-looking: a leaf's columns are loaded into
+//                     registers, take the updates of the block's earlier leaves (the U rows by
+//                     forward substitution in one wave, the rows below as one streamed update),
+//                     and are factored with one workgroup barrier per column (pivot searches
+//                     reduce on the DPP paths). Row interchanges
+//                     are not applied in memory: the workgroup keeps the map from logical to
+//                     physical row in LDS, reads and writes through it, and lists the block's
+//                     interchanges at the end as (destination, source) row pairs;
+//   lu_laswp_kernel   those row pairs applied to every column of the matrix, all loads of a
+//                     column before its stores (rows that move form closed cycles);
+//   lu_trsm_kernel    the block's top rows of the trailing columns, L11^-1 A12 (one wave per
+//                     column, the forward substitution as a wavefront over its lanes);
 //   rocBLAS dgemm     the trailing update on the fp64 matrix cores.
-// Pivots are chosen as LAPACK's idamax chooses them (largest magnitude, first index on ties), so
-// the factors match a library dgetrf up to rounding.
+// Every entry takes its updates in the same order and with the same fused multiply-adds as a
+// right-looking factorisation that applies each leaf's rank-kLeaf update as soon as the leaf is
+// factored, so the factors do not depend on the loop order. Pivots are chosen as LAPACK's idamax
+// chooses them (largest magnitude, first index on ties), so the factors match a library dgetrf
+// up to rounding.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -36,17 +38,14 @@ int blas_of(Device* d, hipStream_t s, rocblas_handle* out);  // bem.hip
 
 namespace {
 
-constexpr int kNB = 64;          // columns per block (trailing update by dgemm)
-constexpr int kLeaf = 8;         // columns per leaf (one workgroup)
-#ifndef O2D_LU_THREADS
-#define O2D_LU_THREADS 512
-#endif
-constexpr int kLeafThreads = O2D_LU_THREADS;  // one workgroup per leaf
-constexpr int kWaves = kLeafThreads / 64;
-constexpr int kMaxOrder = 6144;               // leaves of up to 6144 rows
-constexpr int kMaxRowsPerThread = kMaxOrder / kLeafThreads;
+constexpr int kNB = 64;                  // columns per block (trailing update by dgemm)
+constexpr int kLeaf = 8;                 // columns per leaf
+constexpr int kMaxOrder = 6144;          // panel rows held in registers
+constexpr int kPT = 512;                 // threads of the panel kernel (two waves per SIMD)
+constexpr int kPW = kPT / 64;            // its waves
+static_assert(kPW <= 8, "the waves' pivot candidates are picked within eight lanes");
+constexpr int kMaxQ = (kMaxOrder + kPT - 1) / kPT;
 constexpr int kMaxPairs = 2 * kNB;
-constexpr int kBlock = 256;
 
 // (value, row) of the larger magnitude; the smaller row on ties (idamax's first index)
 __device__ __forceinline__ void pick(double& v, int& i, double ov, int oi) {
@@ -56,271 +55,380 @@ __device__ __forceinline__ void pick(double& v, int& i, double ov, int oi) {
   }
 }
 
-// Leaf of columns [c, c + l) of block [P, pend): rows c..n-1 (m = n - c), thread t holding rows
-// t, t + kLeafThreads, ... (Q of them). bpairs: after the block's last leaf, [count, (dst, src)...].
+// ---- wave-wide reductions on the DPP paths (no LDS crossbar): xor 1, xor 2, half-row mirror
+// and row mirror leave every lane with its 16-lane row's result; four lane reads combine the
+// rows, so every lane returns the wave's result
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_i32<CTRL>((int)b);
+  const int hi = dpp_i32<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// the largest x over the wave (values are magnitudes or -1; no NaN reaches a pick)
+__device__ __forceinline__ double wave_max(double x) {
+  x = fmax(x, dpp_f64<0xB1>(x));
+  x = fmax(x, dpp_f64<0x4E>(x));
+  x = fmax(x, dpp_f64<0x141>(x));
+  x = fmax(x, dpp_f64<0x140>(x));
+  return fmax(fmax(lane_f64(x, 0), lane_f64(x, 16)), fmax(lane_f64(x, 32), lane_f64(x, 48)));
+}
+__device__ __forceinline__ int wave_min(int x) {
+  x = min(x, dpp_i32<0xB1>(x));
+  x = min(x, dpp_i32<0x4E>(x));
+  x = min(x, dpp_i32<0x141>(x));
+  x = min(x, dpp_i32<0x140>(x));
+  return min(min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+             min(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+}
+// (largest value, smallest index among the lanes holding it) over the wave: the pick order
+__device__ __forceinline__ void wave_pick(double& v, int& i) {
+  const double vm = wave_max(v);
+  const unsigned long long hit = __ballot(v == vm);
+  if (__popcll(hit) == 1)  // the common case: one lane holds the largest value
+    i = __builtin_amdgcn_readlane(i, __ffsll(hit) - 1);
+  else
+    i = wave_min(v == vm ? i : INT_MAX);
+  v = vm;
+}
+
+// LDS writes of this wave visible to its own later reads, and no compiler reordering across
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The panel of block [P, P + nb): rows P..n-1 (m = n - P of them, relative row r = logical row
+// P + r), thread t holding relative rows t, t + kPT, ... (Q of them). Physical row of relative
+// row r: P + perm[r]. Writes ipiv (1-based) for the block's columns, *info on a zero pivot, and
+// bpairs = [count, (destination, source)...], the block's row permutation.
 template <int Q>
-__global__ __launch_bounds__(kLeafThreads) void lu_leaf_kernel(int n, int c, int l, int P,
-                                                                int pend, double* __restrict__ A,
-                                                                int lda, int* __restrict__ ipiv,
-                                                                int* __restrict__ info,
-                                                                int* __restrict__ bpairs) {
-  // double-buffered by column parity, so a column needs two barriers
-  __shared__ double red_v[2][kWaves];
-  __shared__ int red_i[2][kWaves];
-  __shared__ double prow[2][kLeaf];  // the pivot row (leaf columns)
-  __shared__ double jrow[2][kLeaf];  // row j before the interchange
-  __shared__ double sl[kLeaf][kLeaf];  // L11 of the leaf
-  __shared__ int spiv[kLeaf];
-  __shared__ int pdst[2 * kLeaf], psrc[2 * kLeaf];
-  __shared__ int s_np;
-  __shared__ int sbpiv[kNB];  // the block's pivot rows (0-based)
+__global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
+                                                       double* __restrict__ A, int lda,
+                                                       int* __restrict__ ipiv,
+                                                       int* __restrict__ info,
+                                                       int* __restrict__ bpairs) {
+  __shared__ int perm[kMaxOrder];
+  __shared__ double L11[kNB][kNB + 1];   // final rows of the panel's top kNB rows
+  __shared__ double X[kNB][kLeaf];       // the U rows of the current leaf's columns
+  __shared__ double cand_v[2][kPW];      // per wave: the pivot candidate, double-buffered
+  __shared__ int cand_i[2][kPW];         // by column parity (one barrier per column)
+  __shared__ double cand_row[2][kPW][kLeaf];
+  __shared__ double diag_row[2][kLeaf];  // the diagonal row before the interchange
+  __shared__ int spiv[kNB];              // the block's pivots (relative logical rows)
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int m = n - c;
-  int szero = -1;  // first zero pivot of the leaf (thread 0)
-  double a[Q][kLeaf];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int r = t + q * kLeafThreads;
-#pragma unroll
-    for (int k = 0; k < kLeaf; ++k)
-      a[q][k] = (r < m && k < l) ? A[(size_t)(c + k) * lda + c + r] : 0.0;
-  }
-#pragma unroll
-  for (int j = 0; j < kLeaf; ++j) {
-    if (j >= l) break;
-    const int b = j & 1;
-    // pivot search over rows j..m-1 of column j
-    double best = -1.0;
-    int bi = INT_MAX;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int r = t + q * kLeafThreads;
-      if (r < m && r >= j) pick(best, bi, fabs(a[q][j]), r);
-    }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) pick(best, bi, __shfl_xor(best, o), __shfl_xor(bi, o));
-    if (lane == 0) {
-      red_v[b][wave] = best;
-      red_i[b][wave] = bi;
-    }
-    __syncthreads();
-    int p = red_i[b][0];
-    {
-      double v = red_v[b][0];
-#pragma unroll
-      for (int w = 1; w < kWaves; ++w) pick(v, p, red_v[b][w], red_i[b][w]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int r = t + q * kLeafThreads;
-      if (r == p) {
-#pragma unroll
-        for (int k = 0; k < kLeaf; ++k) prow[b][k] = a[q][k];
-      }
-      if (r == j) {
-#pragma unroll
-        for (int k = 0; k < kLeaf; ++k) jrow[b][k] = a[q][k];
-      }
-    }
-    if (t == 0) spiv[j] = p;  // to global memory after the loop: a pending global store
-                              // would hold the next barrier
-    __syncthreads();
-    if (p != j) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int r = t + q * kLeafThreads;
-        if (r == j) {
-#pragma unroll
-          for (int k = 0; k < kLeaf; ++k) a[q][k] = prow[b][k];
-        } else if (r == p) {
-#pragma unroll
-          for (int k = 0; k < kLeaf; ++k) a[q][k] = jrow[b][k];
-        }
-      }
-    }
-    const double piv = prow[b][j];
-    if (piv != 0.0) {
-      const double rp = 1.0 / piv;
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int r = t + q * kLeafThreads;
-        if (r < m && r > j) {
-          const double lr = a[q][j] * rp;
-          a[q][j] = lr;
-#pragma unroll
-          for (int k = j + 1; k < kLeaf; ++k)
-            if (k < l) a[q][k] = __builtin_fma(-lr, prow[b][k], a[q][k]);
-        }
-      }
-    } else if (t == 0 && szero < 0) {
-      szero = j;  // singular: the column is left as it is, like dgetf2
-    }
-  }
-  if (t == 0 && szero >= 0 && *info == 0) *info = c + szero + 1;
-  // the factored leaf back to memory; its first l rows (L11 below the diagonal) to LDS; the
-  // leaf's interchanges as one permutation (destination row <- source row, leaf-relative)
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int r = t + q * kLeafThreads;
-    if (r < m) {
-#pragma unroll
-      for (int k = 0; k < kLeaf; ++k)
-        if (k < l) A[(size_t)(c + k) * lda + c + r] = a[q][k];
-    }
-    if (q == 0 && t < l) {
-#pragma unroll
-      for (int k = 0; k < kLeaf; ++k) sl[t][k] = a[0][k];
-    }
-  }
-  if (t < l) ipiv[c + t] = c + spiv[t] + 1;
-  // The leaf's interchanges as one permutation: only rows 0..l-1 and the pivot rows move. For
-  // each of those 2l candidate rows, the row whose content ends there is found by undoing the
-  // interchanges in reverse order (a repeated candidate repeats its pair, which is harmless).
-  if (t < 2 * l) {
-    const int r = t < l ? t : spiv[t - l];
-    int src = r;
-    for (int j = l - 1; j >= 0; --j) {
-      const int p = spiv[j];
-      src = src == j ? p : (src == p ? j : src);
-    }
-    pdst[t] = r;
-    psrc[t] = src;
-  }
-  if (t == 0) s_np = 2 * l;
+  const int m = n - P;
+  const size_t base = (size_t)P * lda + P;  // A + base: the panel's top-left entry
+  for (int r = t; r < m; r += kPT) perm[r] = r;
+  int szero = -1;
   __syncthreads();
-  // the same rows permuted in the block's other columns: every load before any store
-  const int np = s_np;
-  if (np > 0) {
-    const int col = P + t;
-    if (t < pend - P && (col < c || col >= c + l)) {
-      double* cp = A + (size_t)col * lda + c;
-      double v[2 * kLeaf];
+
+  const int nleaf = (nb + kLeaf - 1) / kLeaf;
+  for (int k = 0; k < nleaf; ++k) {
+    const int c0 = kLeaf * k;          // the leaf's first column and diagonal row (relative)
+    const int l = nb - c0 < kLeaf ? nb - c0 : kLeaf;
+    // 1. the leaf's columns, rows in the current logical order
+    double a[Q][kLeaf];
 #pragma unroll
-      for (int i = 0; i < 2 * kLeaf; ++i)
-        if (i < np) v[i] = cp[psrc[i]];
+    for (int q = 0; q < Q; ++q) {
+      const int r = t + q * kPT;
+      const int pr = r < m ? perm[r] : 0;
 #pragma unroll
-      for (int i = 0; i < 2 * kLeaf; ++i)
-        if (i < np) cp[pdst[i]] = v[i];
+      for (int c = 0; c < kLeaf; ++c)
+        a[q][c] = (r < m && c < l) ? A[base + (size_t)(c0 + c) * lda + pr] : 0.0;
     }
-  }
-  __syncthreads();  // the permuted rows are read by other threads below
-  // U12 = L11^-1 A12 for the block's columns right of the leaf: one column per thread
-  {
-    const int col = c + l + t;
-    if (col < pend) {
-      double* cp = A + (size_t)col * lda + c;
-      double x[kLeaf];
+    if (k > 0) {
+      // 2. the U rows 0..c0-1 of the leaf's columns, in wave 0 (its lanes hold rows 0..63):
+      //    for each earlier leaf j, forward substitution with its unit-lower diagonal block,
+      //    then its rank-kLeaf update of the later U rows
+      if (wave == 0) {
+        if (lane < c0) {
 #pragma unroll
-      for (int i = 0; i < kLeaf; ++i) {
-        if (i < l) {
-          double v = cp[i];
+          for (int c = 0; c < kLeaf; ++c) X[lane][c] = a[0][c];
+        }
+        wave_sync();
+        for (int j = 0; j < k; ++j) {
+          const int rj = kLeaf * j;
+          if (lane < l) {
+            const int c = lane;
+            double xv[kLeaf];
 #pragma unroll
-          for (int k = 0; k < i; ++k) v = __builtin_fma(-sl[i][k], x[k], v);
-          x[i] = v;
-          cp[i] = v;
+            for (int i = 0; i < kLeaf; ++i) xv[i] = X[rj + i][c];
+#pragma unroll
+            for (int i = 1; i < kLeaf; ++i) {
+              double v = xv[i];
+#pragma unroll
+              for (int kk = 0; kk < i; ++kk) v = __builtin_fma(-L11[rj + i][rj + kk], xv[kk], v);
+              xv[i] = v;
+            }
+#pragma unroll
+            for (int i = 0; i < kLeaf; ++i) X[rj + i][c] = xv[i];
+          }
+          wave_sync();
+          const int rb = rj + kLeaf;
+          for (int e = lane; e < (c0 - rb) * kLeaf; e += 64) {
+            const int r = rb + e / kLeaf, c = e % kLeaf;
+            if (c < l) {
+              double v = X[r][c];
+#pragma unroll
+              for (int kk = 0; kk < kLeaf; ++kk)
+                v = __builtin_fma(-L11[r][rj + kk], X[rj + kk][c], v);
+              X[r][c] = v;
+            }
+          }
+          wave_sync();
+        }
+      }
+      __syncthreads();
+      // 3. the rows below take every earlier leaf's update, column by column of L; the L
+      //    entries of kCh columns for all of a thread's rows are loaded before they are used
+      constexpr int kCh = Q <= 4 ? 8 : (Q <= 6 ? 4 : 2);
+      int prq[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int r = t + q * kPT;
+        prq[q] = (r >= c0 && r < m) ? perm[r] : -1;
+        if (r < c0) {
+#pragma unroll
+          for (int c = 0; c < kLeaf; ++c) a[q][c] = X[r][c];
+        }
+      }
+      for (int m0 = 0; m0 < c0; m0 += kCh) {
+        double lv[Q][kCh];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+#pragma unroll
+          for (int cc = 0; cc < kCh; ++cc)
+            lv[q][cc] = prq[q] >= 0 ? A[base + (size_t)(m0 + cc) * lda + prq[q]] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (prq[q] >= 0) {
+#pragma unroll
+            for (int cc = 0; cc < kCh; ++cc) {
+#pragma unroll
+              for (int c = 0; c < kLeaf; ++c)
+                a[q][c] = __builtin_fma(-lv[q][cc], X[m0 + cc][c], a[q][c]);
+            }
+          }
         }
       }
     }
-  }
-  // after the block's last leaf: the block's interchanges (rows P..pend-1, in order) as row
-  // pairs for the columns outside the block, found in the same way over the 2 nb candidates
-  if (c + l == pend && pend - P < n) {
-    const int nb = pend - P;
-    __syncthreads();  // the leaf's own pivots are in global memory
-    if (t < nb) sbpiv[t] = ipiv[P + t] - 1;
-    __syncthreads();
-    if (t < 2 * nb) {
-      const int r = t < nb ? P + t : sbpiv[t - nb];
-      int src = r;
-      for (int k = nb - 1; k >= 0; --k) {
-        const int p = sbpiv[k];
-        src = src == P + k ? p : (src == p ? P + k : src);
-      }
-      bpairs[1 + 2 * t] = r;
-      bpairs[2 + 2 * t] = src;
-    }
-    if (t == 0) bpairs[0] = 2 * nb;
-  }
-}
-
-// A[r][j] -= sum_k L21[r][k] U12[k][j] for rows r >= c + l and the block's columns
-// j in [c + l, pend), after the leaf [c, c + l)
-__global__ __launch_bounds__(kBlock) void lu_update_kernel(int n, int c, int l,
-                                                           double* __restrict__ A, int lda) {
-  const int col = c + l + blockIdx.x;
-  const int r = c + l + blockIdx.y * kBlock + threadIdx.x;
-  if (r >= n) return;
-  const double* u = A + (size_t)col * lda + c;  // U12 column (uniform)
-  double v = A[(size_t)col * lda + r];
+    // 4. the leaf's columns factored, one workgroup barrier per column
 #pragma unroll
-  for (int k = 0; k < kLeaf; ++k)
-    if (k < l) v = __builtin_fma(-A[(size_t)(c + k) * lda + r], u[k], v);
-  A[(size_t)col * lda + r] = v;
+    for (int jj = 0; jj < kLeaf; ++jj) {
+      if (jj >= l) break;
+      const int cj = c0 + jj;
+      const int b = jj & 1;
+      // the thread's candidate: its rows come in increasing order, so a strictly larger
+      // magnitude replaces the pick and ties keep the first row
+      double best = -1.0;
+      int bi = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int r = t + q * kPT;
+        if (r < m && r >= cj) {
+          const double v = fabs(a[q][jj]);
+          if (v > best) {
+            best = v;
+            bi = r;
+          }
+        }
+      }
+      wave_pick(best, bi);
+      // the wave's candidate row (held by thread bi % kPT as its row bi / kPT) and the
+      // diagonal row (thread cj, its first row), to LDS
+      if (bi != INT_MAX && t == bi % kPT) {
+        const int qb = bi / kPT;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (q == qb) {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) cand_row[b][wave][c] = a[q][c];
+          }
+        }
+      }
+      if (t == cj) {
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c) diag_row[b][c] = a[0][c];
+      }
+      if (lane == 0) {
+        cand_v[b][wave] = best;
+        cand_i[b][wave] = bi;
+      }
+      __syncthreads();
+      // the waves' candidates in lanes 0..kPW-1, picked the same way (xor 1, xor 2 and the
+      // half-row mirror cover eight lanes)
+      double v = -1.0;
+      int p = INT_MAX;
+      if (lane < kPW) {
+        v = cand_v[b][lane];
+        p = cand_i[b][lane];
+      }
+      pick(v, p, dpp_f64<0xB1>(v), dpp_i32<0xB1>(p));
+      pick(v, p, dpp_f64<0x4E>(v), dpp_i32<0x4E>(p));
+      pick(v, p, dpp_f64<0x141>(v), dpp_i32<0x141>(p));
+      p = __builtin_amdgcn_readlane(p, 0);
+      const int pt = p % kPT;  // the thread holding pivot row p
+      const double* prow = cand_row[b][pt >> 6];
+      double pr[kLeaf], dr[kLeaf];
+#pragma unroll
+      for (int c = 0; c < kLeaf; ++c) {
+        pr[c] = prow[c];
+        dr[c] = diag_row[b][c];
+      }
+      // the interchange of rows cj and p, in registers (a thread can hold both)
+      if (t == cj) {
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c) a[0][c] = pr[c];
+      }
+      if (p != cj && t == pt) {
+        const int qp = p / kPT;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (q == qp) {
+#pragma unroll
+            for (int c = 0; c < kLeaf; ++c) a[q][c] = dr[c];
+          }
+        }
+      }
+      if (t == 0) spiv[cj] = p;
+      const double piv = pr[jj];
+      if (piv != 0.0) {
+        const double rp = 1.0 / piv;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const int r = t + q * kPT;
+          if (r < m && r > cj) {
+            const double lr = a[q][jj] * rp;
+            a[q][jj] = lr;
+#pragma unroll
+            for (int c = jj + 1; c < kLeaf; ++c)
+              if (c < l) a[q][c] = __builtin_fma(-lr, pr[c], a[q][c]);
+          }
+        }
+      } else if (t == 0 && szero < 0) {
+        szero = cj;  // singular: the column is left as it is, like dgetf2
+      }
+    }
+    // the leaf's interchanges into perm at once: the swaps (c0 + j, spiv[c0 + j]) in order j
+    // leave perm[x] = old perm[tau_0(tau_1(...tau_{l-1}(x)))], tau_j that swap, and move only
+    // the diagonal and pivot rows
+    __syncthreads();
+    int px = -1, pv = 0;
+    if (t < 2 * l) {
+      px = t < l ? c0 + t : spiv[c0 + t - l];
+      int y = px;
+      for (int j = l - 1; j >= 0; --j) {
+        const int cj = c0 + j, pj = spiv[cj];
+        y = y == cj ? pj : (y == pj ? cj : y);
+      }
+      pv = perm[y];
+    }
+    __syncthreads();
+    if (px >= 0) perm[px] = pv;  // threads with the same px write the same entry
+    __syncthreads();
+    // 5. the leaf's columns back to their physical rows; its rows are now final: their L
+    //    entries (earlier leaves' from memory, this leaf's from registers) to L11
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int r = t + q * kPT;
+      if (r < m) {
+        const size_t pr = perm[r];
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c)
+          if (c < l) A[base + (size_t)(c0 + c) * lda + pr] = a[q][c];
+      }
+    }
+    if (t >= c0 && t < c0 + l) {
+#pragma unroll
+      for (int c = 0; c < kLeaf; ++c) L11[t][c0 + c] = a[0][c];
+    }
+    for (int e = t; e < l * c0; e += kPT) {
+      const int i = e / c0, mm = e % c0;
+      L11[c0 + i][mm] = A[base + (size_t)mm * lda + perm[c0 + i]];
+    }
+    if (t < l) ipiv[P + c0 + t] = P + spiv[c0 + t] + 1;
+    __syncthreads();
+  }
+  if (t == 0 && szero >= 0 && *info == 0) *info = P + szero + 1;
+  // the block's row permutation: only the diagonal rows and the pivot rows can have moved
+  if (t < 2 * nb) {
+    const int r = t < nb ? t : spiv[t - nb];
+    bpairs[1 + 2 * t] = P + r;
+    bpairs[2 + 2 * t] = P + perm[r];
+  }
+  if (t == 0) bpairs[0] = 2 * nb;
 }
 
-// the block's row pairs on every column outside [P, pend): two columns per workgroup, one
-// pair per thread; a column's loads all land in LDS before any of its stores
-__global__ __launch_bounds__(2 * kMaxPairs) void lu_laswp_kernel(int n, int P, int pend,
-                                                                 double* __restrict__ A, int lda,
+// the block's row pairs on every column: two columns per workgroup, one pair per thread; a
+// column's loads all land in LDS before any of its stores
+__global__ __launch_bounds__(2 * kMaxPairs) void lu_laswp_kernel(int n, double* __restrict__ A,
+                                                                 int lda,
                                                                  const int* __restrict__ bpairs) {
   __shared__ double vals[2][kMaxPairs];
   const int cnt = bpairs[0];
   const int half = threadIdx.x / kMaxPairs, i = threadIdx.x % kMaxPairs;
   const int j = blockIdx.x * 2 + half;
-  const int ncol = n - (pend - P);
-  const bool ok = j < ncol && i < cnt;
-  double* cp = A + (size_t)(j < P ? j : j + (pend - P)) * lda;
+  const bool ok = j < n && i < cnt;
+  double* cp = A + (size_t)(j < n ? j : 0) * lda;
   if (ok) vals[half][i] = cp[bpairs[2 + 2 * i]];
   __syncthreads();
   if (ok) cp[bpairs[1 + 2 * i]] = vals[half][i];
 }
 
 // A12 <- L11^-1 A12 for the block rows P..P+kNB-1 and the trailing columns pend..n-1 (L11 unit
-// lower, kNB x kNB): one column per lane, L11 staged in LDS
-__global__ __launch_bounds__(64) void lu_trsm_kernel(int n, int P, double* __restrict__ A,
-                                                     int lda) {
-  __shared__ double L[kNB][kNB + 1];
-  const int pend = P + kNB;
-  for (int e = threadIdx.x; e < kNB * kNB; e += 64) {
-    const int i = e % kNB, k = e / kNB;
-    L[i][k] = A[(size_t)(P + k) * lda + P + i];
-  }
-  __syncthreads();
-  const int col = pend + blockIdx.x * 64 + threadIdx.x;
-  if (col >= n) return;
+// lower, kNB x kNB): one wave per column, lane i holding row i. The forward substitution runs as
+// a wavefront: at step s, row s is final (it has taken its terms 0..s-1) and every later row takes
+// its term s, so each row adds its terms in index order, as a sequential solve does.
+__global__ __launch_bounds__(256) void lu_trsm_kernel(int n, int P, double* __restrict__ A,
+                                                      int lda) {
+  static_assert(kNB == 64, "one lane per block row");
+  const int lane = threadIdx.x & 63;
+  const int col = P + kNB + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (col >= n) return;  // the whole wave
+  double lc[kNB];  // L[lane][s]
+#pragma unroll
+  for (int s = 0; s < kNB; ++s) lc[s] = A[(size_t)(P + s) * lda + P + lane];
   double* cp = A + (size_t)col * lda + P;
-  double x[kNB];
+  double v = cp[lane];
 #pragma unroll
-  for (int i = 0; i < kNB; ++i) x[i] = cp[i];
-#pragma unroll
-  for (int i = 1; i < kNB; ++i) {
-    double v = x[i];
-#pragma unroll
-    for (int k = 0; k < i; ++k) v = __builtin_fma(-L[i][k], x[k], v);
-    x[i] = v;
+  for (int s = 0; s < kNB - 1; ++s) {
+    const double xs = lane_f64(v, s);
+    if (lane > s) v = __builtin_fma(-lc[s], xs, v);
   }
-#pragma unroll
-  for (int i = 0; i < kNB; ++i) cp[i] = x[i];
+  cp[lane] = v;
 }
 
 __global__ void lu_zero_info_kernel(int* info) { *info = 0; }
 
-// the leaf kernel for q rows per thread, q <= Q
+// the panel kernel for q rows per thread, q <= Q
 template <int Q>
-void launch_leaf(int q, hipStream_t s, int n, int c, int l, int P, int pend, double* A, int lda,
-                 int* ipiv, int* info, int* bpairs) {
+void launch_panel(int q, hipStream_t s, int n, int P, int nb, double* A, int lda, int* ipiv,
+                  int* info, int* bpairs) {
   if constexpr (Q > 1) {
     if (q < Q) {
-      launch_leaf<Q - 1>(q, s, n, c, l, P, pend, A, lda, ipiv, info, bpairs);
+      launch_panel<Q - 1>(q, s, n, P, nb, A, lda, ipiv, info, bpairs);
       return;
     }
   }
-  lu_leaf_kernel<Q><<<1, kLeafThreads, 0, s>>>(n, c, l, P, pend, A, lda, ipiv, info, bpairs);
+  lu_panel_kernel<Q><<<1, kPT, 0, s>>>(n, P, nb, A, lda, ipiv, info, bpairs);
 }
 
 }  // namespace
 
-// Largest order the blocked factorisation takes (a leaf's rows must fit one workgroup).
+// Largest order the blocked factorisation takes (a panel's rows must fit one workgroup).
 int64_t lu_max_order() { return kMaxOrder; }
 
 // In-place LU of the n x n column-major matrix A (leading dimension lda), pivots to ipiv
@@ -339,28 +447,16 @@ int lu_factor_dev(Device* d, hipStream_t s, int n, double* A, int lda, int* ipiv
   O2D_LAUNCH_CHECK();
   const double one = 1.0, minus_one = -1.0;
   for (int P = 0; P < n; P += kNB) {
-    const int pend = std::min(n, P + kNB);
-    for (int c = P; c < pend; c += kLeaf) {
-      const int l = std::min(kLeaf, pend - c);
-      const int q = (n - c + kLeafThreads - 1) / kLeafThreads;
-      launch_leaf<kMaxRowsPerThread>(q, s, n, c, l, P, pend, A, lda, ipiv, info, bpairs);
-      O2D_LAUNCH_CHECK();
-      const int rest = pend - (c + l), rows = n - (c + l);
-      if (rest > 0 && rows > 0) {
-        lu_update_kernel<<<dim3((unsigned)rest, (unsigned)((rows + kBlock - 1) / kBlock)), kBlock,
-                           0, s>>>(n, c, l, A, lda);
-        O2D_LAUNCH_CHECK();
-      }
-    }
-    const int ncol = n - (pend - P);
-    if (ncol > 0) {
-      lu_laswp_kernel<<<(unsigned)((ncol + 1) / 2), 2 * kMaxPairs, 0, s>>>(n, P, pend, A, lda,
-                                                                           bpairs);
-      O2D_LAUNCH_CHECK();
-    }
+    const int nb = std::min(kNB, n - P);
+    const int q = (n - P + kPT - 1) / kPT;
+    launch_panel<kMaxQ>(q, s, n, P, nb, A, lda, ipiv, info, bpairs);
+    O2D_LAUNCH_CHECK();
+    lu_laswp_kernel<<<(unsigned)((n + 1) / 2), 2 * kMaxPairs, 0, s>>>(n, A, lda, bpairs);
+    O2D_LAUNCH_CHECK();
+    const int pend = P + nb;
     const int nr = n - pend;
-    if (nr > 0) {  // pend = P + kNB here
-      lu_trsm_kernel<<<(unsigned)((nr + 63) / 64), 64, 0, s>>>(n, P, A, lda);
+    if (nr > 0) {  // nb = kNB here
+      lu_trsm_kernel<<<(unsigned)((nr + 3) / 4), 256, 0, s>>>(n, P, A, lda);
       O2D_LAUNCH_CHECK();
       if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nr, nr, kNB,
                         &minus_one, A + (size_t)P * lda + pend, lda, A + (size_t)pend * lda + P,

@@ -132,6 +132,55 @@ def test_lu_of_general_matrices_matches_lapack(n):
         assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(A) * np.linalg.norm(x)
 
 
+@pytest.mark.parametrize("n", [65, 700, 2049, 4097])
+def test_lu_factors_and_pivots_equal_lapack(n):
+    """The kept factors are LAPACK getrf's: the same interchanges at every column and the same
+    L and U up to rounding (a different pivot anywhere would change whole rows of both)."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(n + 11)
+    A = rng.normal(size=(n, n))
+    o2d.bem_factor(A)
+    lu, piv = o2d.bem_factors(n)
+    rlu, rpiv = sl.lu_factor(A)
+    assert np.array_equal(piv, rpiv + 1)
+    assert np.max(np.abs(lu - rlu)) <= 1e-9 * np.max(np.abs(rlu))
+
+
+@pytest.mark.parametrize("K", [0, 11, 70])
+@pytest.mark.parametrize("rows", [(3, 10, 70), (5, 517, 1029), (130, 700, 1300), (600, 1100),
+                                  (1590,)])
+def test_lu_pivot_ties_take_the_first_row(K, rows):
+    """Exact ties for the largest magnitude in pivot column K: idamax takes the first row. The
+    panel kernel's thread t holds rows t, t + 512, ...; the tied rows sit in one thread, in
+    one wave and in different waves. Columns k < K pivot on row 1599 - 2k (entry 10, zero in
+    the other pivot rows), and column K is zero in those rows, so no update reaches it before
+    its own pivot search and the tie stays exact."""
+    import scipy.linalg as sl
+    n = 1600
+    rows = [r + 80 if r < K else r for r in rows]
+    rng = np.random.default_rng(K + len(rows))
+    A = rng.uniform(-1.0, 1.0, size=(n, n))
+    prow = [n - 1 - 2 * k for k in range(K)]
+    for k, r in enumerate(prow):
+        A[prow, k] = 0.0
+        A[r, k] = 10.0
+    A[prow, K] = 0.0
+    for i, r in enumerate(rows):
+        A[r, K] = 5.0 if i % 2 == 0 else -5.0
+    o2d.bem_factor(A)
+    lu, piv = o2d.bem_factors(n)
+    rlu, rpiv = sl.lu_factor(A)
+    assert piv[K] == min(rows) + 1
+    assert np.array_equal(piv, rpiv + 1)
+    assert np.max(np.abs(lu - rlu)) <= 1e-9 * np.max(np.abs(rlu))
+
+
+def test_bem_factors_need_factors_of_that_size():
+    o2d.bem_factor(np.eye(3) * 2.0)
+    with pytest.raises(o2d.O2DError):
+        o2d.bem_factors(4)
+
+
 @pytest.mark.parametrize("n", [70, 300])
 def test_lu_reports_a_singular_matrix(n):
     rng = np.random.default_rng(1)
