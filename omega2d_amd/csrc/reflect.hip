@@ -19,6 +19,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -32,7 +33,6 @@ namespace o2d {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPanTile = 128;     // panels per LDS tile
 constexpr int kMaxPasses = 64;    // clearing passes per target before it counts as unsettled
 constexpr int kCutCap = 64;       // rows of the cut table
 enum Mode { kNearest = 0, kReflect = 1, kClear = 2 };
@@ -264,15 +264,16 @@ struct Partial {
   double pad;
 };
 
+template <int NB>
 __device__ void block_partial(long long c0, long long c1, double v, Partial* out) {
-  __shared__ long long s0[kBlock], s1[kBlock];
-  __shared__ double sv[kBlock];
+  __shared__ long long s0[NB], s1[NB];
+  __shared__ double sv[NB];
   const int t = threadIdx.x;
   s0[t] = c0;
   s1[t] = c1;
   sv[t] = v;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
+  for (int w = NB / 2; w > 0; w >>= 1) {
     if (t < w) {
       s0[t] += s0[t + w];
       s1[t] += s1[t + w];
@@ -283,18 +284,136 @@ __device__ void block_partial(long long c0, long long c1, double v, Partial* out
   if (t == 0) out[blockIdx.x] = Partial{s0[0], s1[0], sv[0], 0.0};
 }
 
+// ---- the split scan. A workgroup takes kGroupTargets targets (one per lane) and its kWaves waves
+// split the panels: wave w scans the w-th of kWaves consecutive ranges, staged through its own
+// LDS tile. The sequential scan is a state machine whose result depends on panel order, so each
+// wave keeps an exact summary of its share instead. Let M be the smallest distance over all
+// panels and T > M a value at which no panel lies. Every panel before the first one below T is
+// at least next_up(T), so that first one replaces whatever set the scan held; from then on the
+// bound is below T and no panel at or above T can replace or join. So the sequential scan ends
+// exactly as a scan of the panels below T alone, in index order, ends. The summary keeps what
+// that needs for T = next_up(next_up(M)): the wave's smallest distance m, its panels at m and
+// at next_up(m) (up to kHits of each, in index order) and whether one lies at
+// next_up(next_up(m)). The waves' lists of panels below T are merged by index and replayed with
+// the sequential code. A target with a panel at T (three panels at consecutive doubles), or
+// more than kHits panels of one wave below T, is rescanned sequentially over all panels.
+// Results are bit-identical to the sequential scan either way.
+#ifndef O2D_NEAR_WAVES
+#define O2D_NEAR_WAVES 4
+#endif
+constexpr int kWaves = O2D_NEAR_WAVES;
+constexpr int kNearBlock = 64 * kWaves;
+constexpr int kGroupTargets = 64;
+constexpr int kHits = 4;
+constexpr int kWaveTile = 32;                           // panels per wave tile
+constexpr int kTileLoads = kWaveTile * kRecDoubles / 64;  // doubles per lane per tile
+static_assert(kTileLoads * 64 == kWaveTile * kRecDoubles, "whole tile loads");
+
+// LDS writes of this wave visible to its own later reads, and no compiler reordering across
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Distances are non-negative doubles, so their bit patterns order them as integers and
+// next_up(d) is the pattern plus one: one 64-bit difference of patterns classifies a distance
+// against the running minimum.
+struct Chunk {
+  long long bm;  // bit pattern of the smallest distance m
+  double thr;    // a bound above which an interior distance cannot matter (above m + 2 ulps)
+  int n0, n1;    // panels at m and at next_up(m)
+  int f2;        // a panel at next_up(next_up(m)) was seen
+  int sel;       // which hit list holds the panels at m
+  int k0, k1;    // the first panel at m and at next_up(m); later ones go to the LDS lists
+};
+
+__device__ __forceinline__ void chunk_start(Chunk& c, double m0) {
+  c.bm = __double_as_longlong(m0);
+  c.thr = __longlong_as_double(c.bm + 2) * (1.0 + 0x1p-40) + 0x1p-1000;
+  c.n0 = c.n1 = c.f2 = c.sel = 0;
+  c.k0 = c.k1 = 0;
+}
+
+// one panel of a wave's share: the distance exactly as scan_panel computes it, then the
+// summary. hits[(list * kHits + j) * kGroupTargets] is entry j (j >= 1) of hit list 0 or 1.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void near_kernel(int64_t np, const PanelRec* __restrict__ pan,
+__device__ __forceinline__ void chunk_panel(const PanelRec& p, double x, double y, int k,
+                                            Chunk& c, int* __restrict__ hits) {
+  const double ax = x - p.x0, ay = y - p.y0;
+  const double dot = ax * p.bx + ay * p.by;
+  bool in = (dot > 0.0) && (dot < p.blensq);
+  if (in && dot < p.blensq * 0x1p-1000) {
+    const double t = dot / p.blensq;
+    in = (t > 0.0) && (t < 1.0);
+  }
+  double d;
+  if (in) {
+    const double cc = ay * p.bx - ax * p.by;
+    const double sq = cc * cc;
+    if (sq * p.rbl > c.thr) return;  // more than two ulps above m: it cannot matter
+    d = sq / p.blensq;
+  } else {
+    const double rij2 = ax * ax + ay * ay;
+    const double ex = x - p.x1, ey = y - p.y1;
+    const double rij12 = ex * ex + ey * ey;
+    d = rij2 < rij12 ? rij2 : rij12;
+  }
+  const long long bd = __double_as_longlong(d);
+  const long long dl = bd - c.bm;
+  if (dl > 2) return;
+  if (dl < 0) {
+    c.bm = bd;
+    c.thr = __longlong_as_double(bd + 2) * (1.0 + 0x1p-40) + 0x1p-1000;
+    if (dl == -1) {  // the panels at the old minimum are now at next_up(m)
+      c.f2 = c.n1 > 0;
+      c.n1 = c.n0;
+      c.k1 = c.k0;
+      c.sel ^= 1;
+    } else {
+      c.f2 = dl == -2;
+      c.n1 = 0;
+    }
+    c.n0 = 1;
+    c.k0 = k;
+  } else if (dl == 0) {
+    if (c.n0 == 0) c.k0 = k;
+    else if (c.n0 < kHits) hits[(c.sel * kHits + c.n0) * kGroupTargets] = k;
+    ++c.n0;
+  } else if (dl == 1) {
+    if (c.n1 == 0) c.k1 = k;
+    else if (c.n1 < kHits) hits[((c.sel ^ 1) * kHits + c.n1) * kGroupTargets] = k;
+    ++c.n1;
+  } else {
+    c.f2 = 1;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kNearBlock) void near_kernel(int64_t np, const PanelRec* __restrict__ pan,
                                                       int64_t nt, double* __restrict__ tx,
                                                       double* __restrict__ ty,
                                                       double* __restrict__ ts,
                                                       const double* __restrict__ tr,
                                                       double* __restrict__ dist, ClearParams cp,
                                                       Partial* __restrict__ part) {
-  __shared__ double tile[kPanTile * kRecDoubles];
-  const PanelRec* lp = reinterpret_cast<const PanelRec*>(tile);
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = i < nt;
+  __shared__ double wtile[kWaves][kWaveTile * kRecDoubles];  // each wave's panel tile
+  __shared__ double gx[kGroupTargets], gy[kGroupTargets];   // the targets of this pass
+  __shared__ int gact[kGroupTargets];
+  __shared__ long long cm[kWaves][kGroupTargets];            // the waves' summaries
+  __shared__ int c0n[kWaves][kGroupTargets], c1n[kWaves][kGroupTargets];
+  __shared__ int cfs[kWaves][kGroupTargets];                 // f2 | sel << 1
+  __shared__ int ch[kWaves][2][kHits][kGroupTargets];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t i = (int64_t)blockIdx.x * kGroupTargets + lane;
+  double* mytile = wtile[wave];
+  const PanelRec* lp = reinterpret_cast<const PanelRec*>(mytile);
+  const int64_t per = (np + kWaves - 1) / kWaves;  // wave w scans panels [k0, k1)
+  const int64_t k0 = wave * per < np ? wave * per : np;
+  const int64_t k1 = k0 + per < np ? k0 + per : np;
+  // the target's state lives in wave 0
+  const bool valid = wave == 0 && i < nt;
   double x = valid ? tx[i] : 0.0, y = valid ? ty[i] : 0.0;
   double s = (valid && ts && tr) ? ts[i] : 0.0;
   const double rad = (valid && tr) ? tr[i] : cp.ips;
@@ -307,22 +426,116 @@ __global__ __launch_bounds__(kBlock) void near_kernel(int64_t np, const PanelRec
   const double m0 = (MODE == kClear) ? 0.1 * DBL_MAX : DBL_MAX;
   for (;;) {
     const bool act = untested && passes < maxp;
+    if (wave == 0) {
+      gx[lane] = x;
+      gy[lane] = y;
+      gact[lane] = act;
+    }
     if (!__syncthreads_or(act)) break;
-    Scan st;
-    st.start(m0);
-    for (int64_t base = 0; base < np; base += kPanTile) {
-      const int n = (int)((np - base) < kPanTile ? (np - base) : kPanTile);
-      const double* src = reinterpret_cast<const double*>(pan + base);
-      __syncthreads();
-      for (int e = threadIdx.x; e < n * kRecDoubles; e += kBlock) tile[e] = src[e];
-      __syncthreads();
-      if (act) {
+    const double xw = gx[lane], yw = gy[lane];
+    const bool aw = gact[lane] != 0;
+    Chunk c;
+    chunk_start(c, m0);
+    int* hits = &ch[wave][0][0][lane];
+    if (__ballot(aw) != 0 && k0 < k1) {
+      // the wave's share in tiles of kWaveTile panels through its own LDS tile; the next
+      // tile's loads are in flight while the current one is scanned
+      const double* src = reinterpret_cast<const double*>(pan);
+      double reg[kTileLoads];
+      int64_t t0 = k0;
+      int n = (int)(k1 - t0 < kWaveTile ? k1 - t0 : kWaveTile);
+#pragma unroll
+      for (int j = 0; j < kTileLoads; ++j) {
+        const int e = lane + 64 * j;
+        reg[j] = e < n * kRecDoubles ? src[t0 * kRecDoubles + e] : 0.0;
+      }
+      for (;;) {
+        wave_sync();  // the previous tile's reads are done
+#pragma unroll
+        for (int j = 0; j < kTileLoads; ++j) mytile[lane + 64 * j] = reg[j];
+        wave_sync();
+        const int64_t t1 = t0 + n;
+        const int n1 = (int)(k1 - t1 < kWaveTile ? k1 - t1 : kWaveTile);
+        if (n1 > 0) {
+#pragma unroll
+          for (int j = 0; j < kTileLoads; ++j) {
+            const int e = lane + 64 * j;
+            reg[j] = e < n1 * kRecDoubles ? src[t1 * kRecDoubles + e] : 0.0;
+          }
+        }
+        if (aw) {
 #pragma unroll 4
-        for (int k = 0; k < n; ++k) scan_panel<MODE>(lp[k], x, y, st);
+          for (int k = 0; k < n; ++k) chunk_panel<MODE>(lp[k], xw, yw, (int)t0 + k, c, hits);
+        }
+        if (n1 <= 0) break;
+        t0 = t1;
+        n = n1;
       }
     }
-    if (!act) continue;
+    if (c.n0 > 0) hits[(c.sel * kHits) * kGroupTargets] = c.k0;
+    if (c.n1 > 0) hits[((c.sel ^ 1) * kHits) * kGroupTargets] = c.k1;
+    cm[wave][lane] = c.bm;
+    c0n[wave][lane] = c.n0;
+    c1n[wave][lane] = c.n1;
+    cfs[wave][lane] = c.f2 | (c.sel << 1);
+    __syncthreads();
+    if (!act) continue;  // only wave 0 lanes with an active target go on
     ++passes;
+    long long M = cm[0][lane];  // bit patterns: M, M + 1 = next_up(M), M + 2
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) M = cm[w][lane] < M ? cm[w][lane] : M;
+    const long long M1 = M + 1, M2 = M + 2;
+    // the panels below T = M2 are the waves' lists at M and M1; a panel at M2 or an overflowed
+    // list sends the target to the sequential rescan (a kNearest scan ends at the distance of
+    // the first panel below T, which every list keeps)
+    bool exact = M2 < __double_as_longlong(m0);
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {  // c0n, c1n become the lengths of lists 0 and 1 below T
+      const long long mw = cm[w][lane];
+      const int n0 = c0n[w][lane], n1 = c1n[w][lane], fs = cfs[w][lane];
+      const int sel = fs >> 1;
+      int l0 = 0, l1 = 0;  // entries of the lists at m and at next_up(m) that lie below T
+      if (mw == M) {
+        if (fs & 1) exact = false;
+        if (MODE != kNearest && (n0 > kHits || n1 > kHits)) exact = false;
+        l0 = n0 < kHits ? n0 : kHits;
+        l1 = n1 < kHits ? n1 : kHits;
+      } else if (mw == M1) {
+        if (n1 > 0) exact = false;
+        if (MODE != kNearest && n0 > kHits) exact = false;
+        l0 = n0 < kHits ? n0 : kHits;
+      } else if (mw == M2) {
+        exact = false;
+      }
+      c0n[w][lane] = sel ? l1 : l0;
+      c1n[w][lane] = sel ? l0 : l1;
+    }
+    Scan st;
+    st.start(m0);
+    if (exact) {
+      // the listed panels in index order (each list ascends), replayed with the sequential code
+      int last = -1;
+      for (;;) {
+        int bk = INT_MAX;
+        for (int w = 0; w < kWaves; ++w) {
+          const int la = c0n[w][lane], lb = c1n[w][lane];
+          for (int j = 0; j < la; ++j) {
+            const int k = ch[w][0][j][lane];
+            if (k > last && k < bk) bk = k;
+          }
+          for (int j = 0; j < lb; ++j) {
+            const int k = ch[w][1][j][lane];
+            if (k > last && k < bk) bk = k;
+          }
+        }
+        if (bk == INT_MAX) break;
+        scan_panel<MODE>(pan[bk], x, y, st);
+        last = bk;
+        if (MODE == kNearest) break;  // the first panel below T fixes the distance
+      }
+    } else {
+      for (int64_t k = 0; k < np; ++k) scan_panel<MODE>(pan[k], x, y, st);
+    }
     if (MODE == kNearest) {
       dist[i] = sqrt(st.m);
       untested = false;
@@ -389,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void near_kernel(int64_t np, const PanelRec
     ty[i] = y;
     if (MODE == kClear && ts && tr) ts[i] = s;
   }
-  block_partial(c0, (valid && untested) ? 1 : 0, removed, part);
+  block_partial<kNearBlock>(c0, (valid && untested) ? 1 : 0, removed, part);
 }
 
 // fixed-order sum of the block partials: counts[0..1], *value; counts[2] = bad panels
@@ -430,6 +643,9 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(int nblk, const Partial*
 }
 
 inline unsigned blocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned groups(int64_t n) {
+  return (unsigned)((n + kGroupTargets - 1) / kGroupTargets);
+}
 
 // init_cut_tables<double>(0.1) (src/Reflect.h:333-394), computed once on the host in IEEE double
 // without fused operations, in the reference's order
@@ -489,7 +705,7 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
              const uint32_t* idx, const double* pnx, const double* pny, int64_t nt, double* tx,
              double* ty, double* ts, const double* tr, double* dist, double* value,
              int64_t* counts) {
-  const unsigned nblk = blocks(nt);
+  const unsigned nblk = groups(nt);
   void* pb;
   O2D_TRY(slot(d, kNearPart, sizeof(Partial) * (nblk > 0 ? nblk : 1) + 64, s, &pb));
   Partial* part = static_cast<Partial*>(pb);
@@ -557,11 +773,11 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
     if (run) {
       KernelTimer kt(O2D_KT_NEAR, s);
       if (mode == kNearest)
-        near_kernel<kNearest><<<nblk, kBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+        near_kernel<kNearest><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
       else if (mode == kReflect)
-        near_kernel<kReflect><<<nblk, kBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+        near_kernel<kReflect><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
       else
-        near_kernel<kClear><<<nblk, kBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+        near_kernel<kClear><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
       kt.stop();
       O2D_LAUNCH_CHECK();
     }

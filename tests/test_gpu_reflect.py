@@ -187,3 +187,79 @@ def test_device_entry_counts_bad_panels_without_reading_out_of_range():
     x, y = cu(np.array([0.0, 0.9])), cu(np.array([0.0, 0.0]))
     c = dv.reflect(cu(nx), cu(ny), cu(bad), cu(pnx), cu(pny), x, y)
     assert c.cpu().tolist()[2] == 1
+
+
+def _one_ulp_nodes():
+    """Node A = (0.3, 0) at squared distance M = 0.09 from the origin, and node B with squared
+    distance exactly next_up(M): the pair the scan's one-ulp windows treat by panel order."""
+    a = 0.3
+    M = (0.0 - a) * (0.0 - a)
+    want = np.nextafter(M, np.inf)
+    rng = np.random.default_rng(5)
+    for _ in range(20000):
+        c = rng.uniform(0.05, 0.25)
+        e = np.sqrt(M - c * c)
+        for k in range(-6, 7):
+            ee = e + k * np.spacing(e)
+            if (0.0 - c) * (0.0 - c) + (0.0 - ee) * (0.0 - ee) == want:
+                return (a, 0.0), (c, ee), M
+    raise AssertionError("no node pair found")
+
+
+def _ulp_geometry(case):
+    """Panels on a ring of radius 2 (far from the origin) with panels at the nearest distances
+    put in at chosen indices. A near panel starts at its node and points away from the origin,
+    so the origin's distance to it is the node's. The scan gives wave w panels w, w + 4, ...:
+      succ_first  a panel at next_up(M) (index 5) before one at M (index 10): the sequential
+                  scan keeps next_up(M) as its minimum;
+      m_first     the same two the other way round;
+      spread      eight panels at M, two in each wave's share;
+      crowd       six panels at M in one wave's share (more than it keeps);
+      both        panels at M and at next_up(M) interleaved across the waves."""
+    (ax, ay), (bx, by), M = _one_ulp_nodes()
+    nring = 40
+    ang = np.linspace(0.0, 2 * np.pi, nring, endpoint=False)
+    xs, ys = list(2.0 * np.cos(ang)), list(2.0 * np.sin(ang))
+    ring = [(k, (k + 1) % nring) for k in range(nring)]
+    place = {"succ_first": {5: "B", 10: "A"}, "m_first": {5: "A", 10: "B"},
+             "spread": {k: "A" for k in range(3, 11)},
+             "crowd": {k: "A" for k in (0, 4, 8, 12, 16, 20)},
+             "both": {3: "B", 4: "A", 6: "A", 9: "B", 13: "A", 14: "B"}}[case]
+    xs += [ax, bx]
+    ys += [ay, by]
+    na, nb = nring, nring + 1
+    pans, r = [], 0
+    for j in range(nring + len(place)):
+        if j in place:
+            n0 = na if place[j] == "A" else nb
+            xs.append(1.2 * xs[n0] + 0.01 * len(pans))
+            ys.append(1.2 * ys[n0])
+            pans.append((n0, len(xs) - 1))
+        else:
+            pans.append(ring[r])
+            r += 1
+    nx, ny = np.array(xs), np.array(ys)
+    idx = np.array(pans, dtype=np.uint32).ravel()
+    _, _, pnx, pny, _ = O.panel_geometry(nx, ny, idx)
+    return nx, ny, idx, pnx, pny, M
+
+
+@pytest.mark.parametrize("case", ["succ_first", "m_first", "spread", "crowd", "both"])
+def test_one_ulp_windows_follow_panel_order(case):
+    """Nearest distance, reflection and clearing where panels lie exactly one ulp apart in
+    squared distance: the split scan must end where the sequential scan ends."""
+    nx, ny, idx, pnx, pny, M = _ulp_geometry(case)
+    rng = np.random.default_rng(3)
+    tx = np.concatenate([[0.0, 0.0], rng.uniform(-0.5, 0.5, 300)])
+    ty = np.concatenate([[0.0, 0.0], rng.uniform(-0.5, 0.5, 300)])
+    ref = O.nearest_distances(nx, ny, idx, tx, ty)
+    if case == "succ_first":  # the case is live: the scan's minimum is not the smallest distance
+        assert ref[0] == np.sqrt(np.nextafter(M, np.inf))
+    assert np.array_equal(o2d.nearest_distances(nx, ny, idx, tx, ty), ref)
+    gx, gy, gn = o2d.reflect(nx, ny, idx, pnx, pny, tx, ty)
+    rx, ry, rn = O.reflect(nx, ny, idx, pnx, pny, tx, ty)
+    assert gn == rn and np.array_equal(gx, rx) and np.array_equal(gy, ry)
+    got = o2d.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, tx, ty)
+    ref = O.clear_inner(1, nx, ny, idx, pnx, pny, CM, IPS, tx, ty)
+    assert got["num_cropped"] == ref["num_cropped"]
+    assert np.array_equal(got["x"], ref["x"]) and np.array_equal(got["y"], ref["y"])
