@@ -298,10 +298,7 @@ __device__ void block_partial(long long c0, long long c1, double v, Partial* out
 // the sequential code. A target with a panel at T (three panels at consecutive doubles), or
 // more than kHits panels of one wave below T, is rescanned sequentially over all panels.
 // Results are bit-identical to the sequential scan either way.
-#ifndef O2D_NEAR_WAVES
-#define O2D_NEAR_WAVES 4
-#endif
-constexpr int kWaves = O2D_NEAR_WAVES;
+constexpr int kWaves = 4;
 constexpr int kNearBlock = 64 * kWaves;
 constexpr int kGroupTargets = 64;
 constexpr int kHits = 4;
