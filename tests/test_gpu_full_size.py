@@ -4,9 +4,14 @@ depend on size, plus oracle checks on target (or panel) subsets of the full sums
 - C2 (1M Gaussian particles + 65,536 tracers): split against fp64 over every target; fp64
   against the oracle on a target subset; exact linearity in the strengths; momentum balance of
   the self-interaction (the 0b kernel is symmetric, so sum_i G_i u_i = 0).
-- C5 (4M particles, RM): split against fp64 on a 16,384-target subsample.
+- C1 (10,000-particle disc, RM): one Ralston RK2 step, split and fp64, against the oracle's
+  step at full size.
+- C5 (4M particles, RM): split against fp64 on a 16,384-target subsample (the RK3 step itself
+  is checked against the oracle at 20,000 particles in test_gpu_device_step.py).
 - C3 (2,048 panels + 200,000 particles): the 2049^2 and 4097^2 systems against LAPACK, the
-  2049^2 matrix against the oracle; panel sums against the oracle on panel/target subsets.
+  2049^2 matrix against the oracle; panel sums against the oracle on panel/target subsets; the
+  wake's V2 self-interaction, split against fp64 on every particle and fp64 against the oracle
+  on a subset.
 - C4 (250,000 particles, 4,096 probes): the strength adjoint is the transposed sum,
   <dJ/dG, G> = <gu, u> + <gv, v>.
 """
@@ -184,3 +189,42 @@ def test_c4_strength_adjoint_is_the_transposed_sum():
     print("C4 <dJ/dG, G> vs <g, u>:", lhs, rhs)
     assert abs(lhs - rhs) <= 1e-12 * scale
     assert np.all(np.isfinite(_np(gx))) and np.all(np.isfinite(_np(gy)))
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 1e-12), ("split", 1e-6)])
+def test_c1_rk2_step_at_full_size_against_the_oracle(precision, bar):
+    import omega2d_amd.device_step as ds
+    w = W.c1_disc()
+    x0, y0, s, r, dt = w["x"], w["y"], w["s"], w["r"], w["dt"]
+    x, y = _cu(x0), _cu(y0)
+    st = ds.Stepper(x, y, _cu(s), _cu(r), core="rm", precision=precision)
+    u, v = st.step(dt, 2)
+
+    def vel(px, py):
+        return O.finalize(*O.points_on_points("rm", px, py, s, r, px, py, r))
+    u0 = vel(x0, y0)
+    x1, y1 = O.rk_combine(x0, y0, [u0[0]], [u0[1]], [1.0], 2.0 / 3.0 * dt)
+    u1 = vel(x1, y1)
+    ex, ey = O.rk_combine(x0, y0, [u0[0], u1[0]], [u0[1], u1[1]], [0.25, 0.75], dt)
+    eu, ev = 0.25 * u0[0] + 0.75 * u1[0], 0.25 * u0[1] + 0.75 * u1[1]
+    assert _err2((_np(u), _np(v)), (eu, ev)) <= bar
+    tol = 1e-14 if precision == "fp64" else 1e-5 * dt * np.max(np.hypot(eu, ev))
+    assert np.max(np.abs(_np(x) - ex)) <= tol and np.max(np.abs(_np(y) - ey)) <= tol
+
+
+def test_c3_wake_split_against_fp64_and_the_oracle():
+    import omega2d_amd.device as dv
+    w = W.c3_circle_wake()
+    d = {k: _cu(w[k]) for k in ("x", "y", "s", "r")}
+    res = {}
+    for prec in ("split", "fp64"):
+        u, v = dv.points_on_points(d["x"], d["y"], d["s"], d["r"], d["x"], d["y"], d["r"],
+                                   core="v2", precision=prec)
+        res[prec] = (_np(u), _np(v))
+    err = _err2(res["split"], res["fp64"])
+    print("C3 wake split vs fp64 over 200,000 particles:", err)
+    assert err <= 1e-6
+    t = np.linspace(0, len(w["x"]) - 1, 256).astype(np.int64)
+    ru, rv = O.points_on_points("v2", w["x"], w["y"], w["s"], w["r"], w["x"][t], w["y"][t],
+                                w["r"][t])
+    assert _err2((res["fp64"][0][t], res["fp64"][1][t]), (ru, rv)) <= 1e-12
