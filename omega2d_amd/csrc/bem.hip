@@ -195,6 +195,16 @@ void release_blas(Device* d) {
     rocblas_destroy_handle(static_cast<rocblas_handle>(d->blas));
     d->blas = nullptr;
   }
+  if (d->blas_aux) {
+    rocblas_destroy_handle(static_cast<rocblas_handle>(d->blas_aux));
+    d->blas_aux = nullptr;
+  }
+  for (hipEvent_t& e : d->aux_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (d->aux) (void)hipStreamDestroy(d->aux);
+  d->aux = nullptr;
 }
 
 int bem_assemble_dev(Device* d, hipStream_t s, int64_t nnodes, const double* nx, const double* ny,

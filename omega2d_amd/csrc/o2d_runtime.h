@@ -52,6 +52,11 @@ struct Device {
   int64_t lu_solves = 0;
   bool inv_ready = false;
   void* blas = nullptr;  // rocblas_handle
+  // the LU's second stream (the trailing update beside the next panel), its rocBLAS handle and
+  // two events; created on first use
+  hipStream_t aux = nullptr;
+  void* blas_aux = nullptr;
+  hipEvent_t aux_ev[2] = {};
 };
 
 // Error handling: every entry point returns fail(code, fmt, ...) on failure.
