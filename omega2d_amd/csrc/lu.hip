@@ -8,7 +8,8 @@
 //   lu_panel_kernel   one workgroup factors the block's panel (rows P..n-1, kNB columns) in
 //                     leaves of kLeaf columns, left-looking: a leaf's columns are loaded into
 //                     registers, take the updates of the block's earlier leaves (the U rows by
-//                     forward substitution in one wave, the rows below as one streamed update),
+//                     a wavefront forward substitution, one wave per column, the rows below as
+//                     one streamed update),
 //                     and are factored with one workgroup barrier per column (pivot searches
 //                     reduce on the DPP paths). Row interchanges
 //                     are not applied in memory: the workgroup keeps the map from logical to
@@ -44,6 +45,7 @@ constexpr int kMaxOrder = 6144;          // panel rows held in registers
 constexpr int kPT = 512;                 // threads of the panel kernel (two waves per SIMD)
 constexpr int kPW = kPT / 64;            // its waves
 static_assert(kPW <= 8, "the waves' pivot candidates are picked within eight lanes");
+static_assert(kPW >= kLeaf, "one wave per leaf column in the U-row solve");
 constexpr int kMaxQ = (kMaxOrder + kPT - 1) / kPT;
 constexpr int kMaxPairs = 2 * kNB;
 constexpr int kLookAhead = 2048;  // trailing columns beside the next panel, at least
@@ -150,46 +152,23 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
         a[q][c] = (r < m && c < l) ? A[base + (size_t)(c0 + c) * lda + pr] : 0.0;
     }
     if (k > 0) {
-      // 2. the U rows 0..c0-1 of the leaf's columns, in wave 0 (its lanes hold rows 0..63):
-      //    for each earlier leaf j, forward substitution with its unit-lower diagonal block,
-      //    then its rank-kLeaf update of the later U rows
-      if (wave == 0) {
-        if (lane < c0) {
+      // 2. the U rows 0..c0-1 of the leaf's columns: wave w takes column w, lane i its row i,
+      //    and the forward substitution with the unit-lower L11 runs as a wavefront over the
+      //    lanes (at step s row s is final and every later row takes its term s, so each row
+      //    adds its terms in index order, as the sequential solve does)
+      if (wave == 0 && lane < c0) {
 #pragma unroll
-          for (int c = 0; c < kLeaf; ++c) X[lane][c] = a[0][c];
+        for (int c = 0; c < kLeaf; ++c) X[lane][c] = a[0][c];
+      }
+      __syncthreads();
+      if (wave < l) {
+        double v = lane < c0 ? X[lane][wave] : 0.0;
+#pragma unroll 8
+        for (int s = 0; s < c0 - 1; ++s) {
+          const double xs = lane_f64(v, s);
+          if (lane > s && lane < c0) v = __builtin_fma(-L11[lane][s], xs, v);
         }
-        wave_sync();
-        for (int j = 0; j < k; ++j) {
-          const int rj = kLeaf * j;
-          if (lane < l) {
-            const int c = lane;
-            double xv[kLeaf];
-#pragma unroll
-            for (int i = 0; i < kLeaf; ++i) xv[i] = X[rj + i][c];
-#pragma unroll
-            for (int i = 1; i < kLeaf; ++i) {
-              double v = xv[i];
-#pragma unroll
-              for (int kk = 0; kk < i; ++kk) v = __builtin_fma(-L11[rj + i][rj + kk], xv[kk], v);
-              xv[i] = v;
-            }
-#pragma unroll
-            for (int i = 0; i < kLeaf; ++i) X[rj + i][c] = xv[i];
-          }
-          wave_sync();
-          const int rb = rj + kLeaf;
-          for (int e = lane; e < (c0 - rb) * kLeaf; e += 64) {
-            const int r = rb + e / kLeaf, c = e % kLeaf;
-            if (c < l) {
-              double v = X[r][c];
-#pragma unroll
-              for (int kk = 0; kk < kLeaf; ++kk)
-                v = __builtin_fma(-L11[r][rj + kk], X[rj + kk][c], v);
-              X[r][c] = v;
-            }
-          }
-          wave_sync();
-        }
+        if (lane < c0) X[lane][wave] = v;
       }
       __syncthreads();
       // 3. the rows below take every earlier leaf's update, column by column of L; the L
