@@ -292,11 +292,12 @@ __device__ void block_partial(long long c0, long long c1, double v, Partial* out
 // at least next_up(T), so that first one replaces whatever set the scan held; from then on the
 // bound is below T and no panel at or above T can replace or join. So the sequential scan ends
 // exactly as a scan of the panels below T alone, in index order, ends. The summary keeps what
-// that needs for T = next_up(next_up(M)): the wave's smallest distance m, its panels at m and
-// at next_up(m) (up to kHits of each, in index order) and whether one lies at
-// next_up(next_up(m)). The waves' lists of panels below T are merged by index and replayed with
-// the sequential code. A target with a panel at T (three panels at consecutive doubles), or
-// more than kHits panels of one wave below T, is rescanned sequentially over all panels.
+// that needs for T = next_up(M) or T = next_up(next_up(M)): the wave's smallest distance m, its
+// panels at m and at next_up(m) (up to kHits of each, in index order) and whether one lies at
+// next_up(next_up(m)). T is next_up(M) when no panel lies there, else next_up(next_up(M)). The
+// waves' lists of panels below T are merged by index and replayed with the sequential code. A
+// target with panels at all three of M, next_up(M) and next_up(next_up(M)), or more than kHits
+// panels of one wave below T, is rescanned sequentially over all panels.
 // Results are bit-identical to the sequential scan either way.
 constexpr int kWaves = 4;
 constexpr int kNearBlock = 64 * kWaves;
@@ -485,24 +486,38 @@ __global__ __launch_bounds__(kNearBlock) void near_kernel(int64_t np, const Pane
     // the panels below T = M2 are the waves' lists at M and M1; a panel at M2 or an overflowed
     // list sends the target to the sequential rescan (a kNearest scan ends at the distance of
     // the first panel below T, which every list keeps)
-    bool exact = M2 < __double_as_longlong(m0);
+    // T = M1 when no panel lies at M1 (the lists at M alone), else T = M2 when none lies at M2
+    // (the lists at M and M1); T must stay below the starting bound m0
+    bool at1 = false, at2 = false;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const long long mw = cm[w][lane];
+      const int n1 = c1n[w][lane], f2 = cfs[w][lane] & 1;
+      if (mw == M) {
+        at1 = at1 || n1 > 0;
+        at2 = at2 || f2;
+      } else if (mw == M1) {
+        at1 = true;
+        at2 = at2 || n1 > 0;
+      } else if (mw == M2) {
+        at2 = true;
+      }
+    }
+    const bool two = at1;  // T = M2
+    bool exact = (two ? !at2 && M2 < __double_as_longlong(m0) : M1 < __double_as_longlong(m0));
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {  // c0n, c1n become the lengths of lists 0 and 1 below T
       const long long mw = cm[w][lane];
-      const int n0 = c0n[w][lane], n1 = c1n[w][lane], fs = cfs[w][lane];
-      const int sel = fs >> 1;
+      const int n0 = c0n[w][lane], n1 = c1n[w][lane];
+      const int sel = cfs[w][lane] >> 1;
       int l0 = 0, l1 = 0;  // entries of the lists at m and at next_up(m) that lie below T
       if (mw == M) {
-        if (fs & 1) exact = false;
-        if (MODE != kNearest && (n0 > kHits || n1 > kHits)) exact = false;
+        if (MODE != kNearest && (n0 > kHits || (two && n1 > kHits))) exact = false;
         l0 = n0 < kHits ? n0 : kHits;
-        l1 = n1 < kHits ? n1 : kHits;
-      } else if (mw == M1) {
-        if (n1 > 0) exact = false;
+        l1 = two ? (n1 < kHits ? n1 : kHits) : 0;
+      } else if (two && mw == M1) {
         if (MODE != kNearest && n0 > kHits) exact = false;
         l0 = n0 < kHits ? n0 : kHits;
-      } else if (mw == M2) {
-        exact = false;
       }
       c0n[w][lane] = sel ? l1 : l0;
       c1n[w][lane] = sel ? l0 : l1;

@@ -189,21 +189,27 @@ def test_device_entry_counts_bad_panels_without_reading_out_of_range():
     assert c.cpu().tolist()[2] == 1
 
 
-def _one_ulp_nodes():
-    """Node A = (0.3, 0) at squared distance M = 0.09 from the origin, and node B with squared
-    distance exactly next_up(M): the pair the scan's one-ulp windows treat by panel order."""
-    a = 0.3
-    M = (0.0 - a) * (0.0 - a)
-    want = np.nextafter(M, np.inf)
-    rng = np.random.default_rng(5)
+def _node_at(want, seed):
+    """A node whose squared distance from the origin, as the scan computes it, is exactly want."""
+    rng = np.random.default_rng(seed)
     for _ in range(20000):
         c = rng.uniform(0.05, 0.25)
-        e = np.sqrt(M - c * c)
+        e = np.sqrt(want - c * c)
         for k in range(-6, 7):
             ee = e + k * np.spacing(e)
             if (0.0 - c) * (0.0 - c) + (0.0 - ee) * (0.0 - ee) == want:
-                return (a, 0.0), (c, ee), M
-    raise AssertionError("no node pair found")
+                return (c, ee)
+    raise AssertionError("no node found")
+
+
+def _one_ulp_nodes():
+    """Node A = (0.3, 0) at squared distance M = 0.09 from the origin, node B at exactly
+    next_up(M) and node C at next_up(next_up(M)): the distances the scan's one-ulp windows
+    treat by panel order."""
+    a = 0.3
+    M = (0.0 - a) * (0.0 - a)
+    M1 = np.nextafter(M, np.inf)
+    return (a, 0.0), _node_at(M1, 5), _node_at(np.nextafter(M1, np.inf), 6), M
 
 
 def _ulp_geometry(case):
@@ -215,8 +221,11 @@ def _ulp_geometry(case):
       m_first     the same two the other way round;
       spread      eight panels at M, two in each wave's share;
       crowd       six panels at M in one wave's share (more than it keeps);
-      both        panels at M and at next_up(M) interleaved across the waves."""
-    (ax, ay), (bx, by), M = _one_ulp_nodes()
+      both        panels at M and at next_up(M) interleaved across the waves;
+      gap_first   a panel at next_up(next_up(M)) (index 5) before one at M (index 20), none
+                  between them;
+      three       panels at M, next_up(M) and next_up(next_up(M)) (the sequential rescan)."""
+    (ax, ay), (bx, by), (cx, cy), M = _one_ulp_nodes()
     nring = 40
     ang = np.linspace(0.0, 2 * np.pi, nring, endpoint=False)
     xs, ys = list(2.0 * np.cos(ang)), list(2.0 * np.sin(ang))
@@ -224,14 +233,16 @@ def _ulp_geometry(case):
     place = {"succ_first": {5: "B", 10: "A"}, "m_first": {5: "A", 10: "B"},
              "spread": {k: "A" for k in range(3, 11)},
              "crowd": {k: "A" for k in (0, 4, 8, 12, 16, 20)},
-             "both": {3: "B", 4: "A", 6: "A", 9: "B", 13: "A", 14: "B"}}[case]
-    xs += [ax, bx]
-    ys += [ay, by]
-    na, nb = nring, nring + 1
+             "both": {3: "B", 4: "A", 6: "A", 9: "B", 13: "A", 14: "B"},
+             "gap_first": {5: "C", 20: "A"},
+             "three": {2: "C", 7: "B", 11: "A", 30: "B"}}[case]
+    xs += [ax, bx, cx]
+    ys += [ay, by, cy]
+    nodes = {"A": nring, "B": nring + 1, "C": nring + 2}
     pans, r = [], 0
     for j in range(nring + len(place)):
         if j in place:
-            n0 = na if place[j] == "A" else nb
+            n0 = nodes[place[j]]
             xs.append(1.2 * xs[n0] + 0.01 * len(pans))
             ys.append(1.2 * ys[n0])
             pans.append((n0, len(xs) - 1))
@@ -244,7 +255,8 @@ def _ulp_geometry(case):
     return nx, ny, idx, pnx, pny, M
 
 
-@pytest.mark.parametrize("case", ["succ_first", "m_first", "spread", "crowd", "both"])
+@pytest.mark.parametrize("case", ["succ_first", "m_first", "spread", "crowd", "both",
+                                  "gap_first", "three"])
 def test_one_ulp_windows_follow_panel_order(case):
     """Nearest distance, reflection and clearing where panels lie exactly one ulp apart in
     squared distance: the split scan must end where the sequential scan ends."""
