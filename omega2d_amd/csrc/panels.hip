@@ -34,7 +34,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPTile = 256;  // panels per LDS stage (panels on points)
 #ifndef O2D_PANTILE
-#define O2D_PANTILE 8
+#define O2D_PANTILE 4
 #endif
 constexpr int kPanTile = O2D_PANTILE;  // panels per workgroup (points on panels)
 
