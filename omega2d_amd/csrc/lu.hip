@@ -46,6 +46,7 @@ constexpr int kPT = 512;                 // threads of the panel kernel (two wav
 constexpr int kPW = kPT / 64;            // its waves
 static_assert(kPW <= 8, "the waves' pivot candidates are picked within eight lanes");
 static_assert(kPW >= kLeaf, "one wave per leaf column in the U-row solve");
+static_assert(kLeaf * (kNB - kLeaf) <= kPT, "one earlier L entry of a leaf's rows per thread");
 constexpr int kMaxQ = (kMaxOrder + kPT - 1) / kPT;
 constexpr int kMaxPairs = 2 * kNB;
 constexpr int kLookAhead = 2048;  // trailing columns beside the next panel, at least
@@ -320,7 +321,16 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
     if (px >= 0) perm[px] = pv;  // threads with the same px write the same entry
     __syncthreads();
     // 5. the leaf's columns back to their physical rows; its rows are now final: their L
-    //    entries (earlier leaves' from memory, this leaf's from registers) to L11
+    //    entries (earlier leaves' from memory, this leaf's from registers) to L11. The loads
+    //    go first: loads and stores share one in-order counter, so a load issued after the
+    //    stores would wait for them as well
+    double lprev = 0.0;
+    int ei = -1, em = 0;
+    if (t < l * c0) {
+      ei = t / c0;
+      em = t % c0;
+      lprev = A[base + (size_t)em * lda + perm[c0 + ei]];
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int r = t + q * kPT;
@@ -335,10 +345,7 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
       for (int c = 0; c < kLeaf; ++c) L11[t][c0 + c] = a[0][c];
     }
-    for (int e = t; e < l * c0; e += kPT) {
-      const int i = e / c0, mm = e % c0;
-      L11[c0 + i][mm] = A[base + (size_t)mm * lda + perm[c0 + i]];
-    }
+    if (ei >= 0) L11[c0 + ei][em] = lprev;
     if (t < l) ipiv[P + c0 + t] = P + spiv[c0 + t] + 1;
     __syncthreads();
   }
