@@ -47,6 +47,7 @@ constexpr int kPW = kPT / 64;            // its waves
 static_assert(kPW <= 8, "the waves' pivot candidates are picked within eight lanes");
 static_assert(kPW >= kLeaf, "one wave per leaf column in the U-row solve");
 static_assert(kLeaf * (kNB - kLeaf) <= kPT, "one earlier L entry of a leaf's rows per thread");
+static_assert(kNB <= kPT, "rows past the first kPT lie below every diagonal row of the block");
 constexpr int kMaxQ = (kMaxOrder + kPT - 1) / kPT;
 constexpr int kMaxPairs = 2 * kNB;
 constexpr int kLookAhead = 2048;  // trailing columns beside the next panel, at least
@@ -138,19 +139,36 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
   int szero = -1;
   __syncthreads();
 
+  // whether row t + q * kPT lies in the panel: with Q = ceil(m / kPT) every row of q < Q - 1 does
+  auto exists = [&](int q) { return q < Q - 1 || t + q * kPT < m; };
+  // the leaf of columns [c0, c0 + l) in logical row order, through perm
+  auto load_leaf = [&](double (&dst)[Q][kLeaf], int c0, int l) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int r = t + q * kPT;
+      const int pr = exists(q) ? perm[r] : 0;
+#pragma unroll
+      for (int c = 0; c < kLeaf; ++c)
+        dst[q][c] = (exists(q) && c < l) ? A[base + (size_t)(c0 + c) * lda + pr] : 0.0;
+    }
+  };
+  constexpr bool kPrefetch = Q <= 4;
+  double an[kPrefetch ? Q : 1][kLeaf];  // the next leaf, loaded before this leaf's stores
   const int nleaf = (nb + kLeaf - 1) / kLeaf;
   for (int k = 0; k < nleaf; ++k) {
     const int c0 = kLeaf * k;          // the leaf's first column and diagonal row (relative)
     const int l = nb - c0 < kLeaf ? nb - c0 : kLeaf;
-    // 1. the leaf's columns, rows in the current logical order
+    // 1. the leaf's columns, rows in the current logical order (loaded at the end of the
+    //    previous leaf when the rows are few enough to hold two leaves in registers)
     double a[Q][kLeaf];
+    if (kPrefetch && k > 0) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int r = t + q * kPT;
-      const int pr = r < m ? perm[r] : 0;
+      for (int q = 0; q < Q; ++q) {
 #pragma unroll
-      for (int c = 0; c < kLeaf; ++c)
-        a[q][c] = (r < m && c < l) ? A[base + (size_t)(c0 + c) * lda + pr] : 0.0;
+        for (int c = 0; c < kLeaf; ++c) a[q][c] = an[q][c];
+      }
+    } else {
+      load_leaf(a, c0, l);
     }
     if (k > 0) {
       // 2. the U rows 0..c0-1 of the leaf's columns: wave w takes column w, lane i its row i,
@@ -164,10 +182,18 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
       __syncthreads();
       if (wave < l) {
         double v = lane < c0 ? X[lane][wave] : 0.0;
-#pragma unroll 8
-        for (int s = 0; s < c0 - 1; ++s) {
-          const double xs = lane_f64(v, s);
-          if (lane > s && lane < c0) v = __builtin_fma(-L11[lane][s], xs, v);
+        for (int s0 = 0; s0 < c0 - 1; s0 += kLeaf) {
+          double lc[kLeaf];  // this lane's L11 entries for the next kLeaf steps
+#pragma unroll
+          for (int j = 0; j < kLeaf; ++j) lc[j] = lane < c0 ? L11[lane][s0 + j] : 0.0;
+#pragma unroll
+          for (int j = 0; j < kLeaf; ++j) {
+            const int s = s0 + j;
+            if (s < c0 - 1) {
+              const double nv = __builtin_fma(-lc[j], lane_f64(v, s), v);
+              v = (lane > s && lane < c0) ? nv : v;
+            }
+          }
         }
         if (lane < c0) X[lane][wave] = v;
       }
@@ -179,7 +205,7 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int r = t + q * kPT;
-        prq[q] = (r >= c0 && r < m) ? perm[r] : -1;
+        prq[q] = (exists(q) && (q > 0 || r >= c0)) ? perm[r] : -1;
         if (r < c0) {
 #pragma unroll
           for (int c = 0; c < kLeaf; ++c) a[q][c] = X[r][c];
@@ -219,7 +245,7 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int r = t + q * kPT;
-        if (r < m && r >= cj) {
+        if (exists(q) && (q > 0 || r >= cj)) {
           const double v = fabs(a[q][jj]);
           if (v > best) {
             best = v;
@@ -291,7 +317,7 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
           const int r = t + q * kPT;
-          if (r < m && r > cj) {
+          if (exists(q) && (q > 0 || r > cj)) {
             const double lr = a[q][jj] * rp;
             a[q][jj] = lr;
 #pragma unroll
@@ -324,6 +350,9 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
     //    entries (earlier leaves' from memory, this leaf's from registers) to L11. The loads
     //    go first: loads and stores share one in-order counter, so a load issued after the
     //    stores would wait for them as well
+    if constexpr (kPrefetch) {
+      if (k + 1 < nleaf) load_leaf(an, c0 + kLeaf, nb - c0 - kLeaf < kLeaf ? nb - c0 - kLeaf : kLeaf);
+    }
     double lprev = 0.0;
     int ei = -1, em = 0;
     if (t < l * c0) {
@@ -334,7 +363,7 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int r = t + q * kPT;
-      if (r < m) {
+      if (exists(q)) {
         const size_t pr = perm[r];
 #pragma unroll
         for (int c = 0; c < kLeaf; ++c)
