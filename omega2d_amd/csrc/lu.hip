@@ -54,10 +54,9 @@ constexpr int kLookAhead = 2048;  // trailing columns beside the next panel, at 
 
 // (value, row) of the larger magnitude; the smaller row on ties (idamax's first index)
 __device__ __forceinline__ void pick(double& v, int& i, double ov, int oi) {
-  if (ov > v || (ov == v && oi < i)) {
-    v = ov;
-    i = oi;
-  }
+  const bool take = ov > v || (ov == v && oi < i);  // selects, not a branch
+  v = take ? ov : v;
+  i = take ? oi : i;
 }
 
 // ---- wave-wide reductions on the DPP paths (no LDS crossbar): xor 1, xor 2, half-row mirror
@@ -320,9 +319,10 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
           if (exists(q) && (q > 0 || r > cj)) {
             const double lr = a[q][jj] * rp;
             a[q][jj] = lr;
+            // columns past a short last leaf (c >= l) hold zeros in every row and in the
+            // pivot row, are never searched and never stored: they need no test here
 #pragma unroll
-            for (int c = jj + 1; c < kLeaf; ++c)
-              if (c < l) a[q][c] = __builtin_fma(-lr, pr[c], a[q][c]);
+            for (int c = jj + 1; c < kLeaf; ++c) a[q][c] = __builtin_fma(-lr, pr[c], a[q][c]);
           }
         }
       } else if (t == 0 && szero < 0) {
