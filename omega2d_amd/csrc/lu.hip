@@ -198,29 +198,32 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
       }
       __syncthreads();
       // 3. the rows below take every earlier leaf's update, column by column of L; the L
-      //    entries of kCh columns for all of a thread's rows are loaded before they are used
-      constexpr int kCh = Q <= 4 ? 8 : (Q <= 6 ? 4 : 2);
+      //    entries of kCh columns for all of a thread's rows are loaded before they are used.
+      //    Every load reads a row of the panel (rows past m read row m - 1) so no load needs a
+      //    branch; a row past m only feeds its own padding. The U rows (r < c0, in wave 0)
+      //    skip the update.
+      constexpr int kCh = Q <= 2 ? 8 : (Q == 3 ? 4 : (Q <= 8 ? 2 : 1));
       int prq[Q];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int r = t + q * kPT;
-        prq[q] = (exists(q) && (q > 0 || r >= c0)) ? perm[r] : -1;
+        prq[q] = perm[exists(q) ? r : m - 1];
         if (r < c0) {
 #pragma unroll
           for (int c = 0; c < kLeaf; ++c) a[q][c] = X[r][c];
         }
       }
+      const bool urow = t < c0;  // this thread's first row is a U row
       for (int m0 = 0; m0 < c0; m0 += kCh) {
         double lv[Q][kCh];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
 #pragma unroll
-          for (int cc = 0; cc < kCh; ++cc)
-            lv[q][cc] = prq[q] >= 0 ? A[base + (size_t)(m0 + cc) * lda + prq[q]] : 0.0;
+          for (int cc = 0; cc < kCh; ++cc) lv[q][cc] = A[base + (size_t)(m0 + cc) * lda + prq[q]];
         }
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          if (prq[q] >= 0) {
+          if (q > 0 || !urow) {
 #pragma unroll
             for (int cc = 0; cc < kCh; ++cc) {
 #pragma unroll
