@@ -54,7 +54,7 @@ constexpr int kLookAhead = 2048;  // trailing columns beside the next panel, at 
 
 // (value, row) of the larger magnitude; the smaller row on ties (idamax's first index)
 __device__ __forceinline__ void pick(double& v, int& i, double ov, int oi) {
-  const bool take = ov > v || (ov == v && oi < i);  // selects, not a branch
+  const bool take = (ov > v) | ((ov == v) & (oi < i));  // selects, no short-circuit branch
   v = take ? ov : v;
   i = take ? oi : i;
 }
@@ -247,13 +247,10 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int r = t + q * kPT;
-        if (exists(q) && (q > 0 || r >= cj)) {
-          const double v = fabs(a[q][jj]);
-          if (v > best) {
-            best = v;
-            bi = r;
-          }
-        }
+        const double v = fabs(a[q][jj]);
+        const bool take = exists(q) & ((q > 0) | (r >= cj)) & (v > best);
+        best = take ? v : best;
+        bi = take ? r : bi;
       }
       wave_pick(best, bi);
       // the wave's candidate row (held by thread bi % kPT as its row bi / kPT) and the
