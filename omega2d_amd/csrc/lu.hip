@@ -331,20 +331,24 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
     }
     // the leaf's interchanges into perm at once: the swaps (c0 + j, spiv[c0 + j]) in order j
     // leave perm[x] = old perm[tau_0(tau_1(...tau_{l-1}(x)))], tau_j that swap, and move only
-    // the diagonal and pivot rows
-    __syncthreads();
-    int px = -1, pv = 0;
-    if (t < 2 * l) {
-      px = t < l ? c0 + t : spiv[c0 + t - l];
-      int y = px;
-      for (int j = l - 1; j >= 0; --j) {
-        const int cj = c0 + j, pj = spiv[cj];
-        y = y == cj ? pj : (y == pj ? cj : y);
+    // the diagonal and pivot rows. Only wave 0 takes part (spiv is written by thread 0 and
+    // 2 l <= 16 threads recompose perm, which nobody reads during the column loop), so wave
+    // syncs order it and one workgroup barrier publishes the result.
+    if (wave == 0) {
+      wave_sync();
+      int px = -1, pv = 0;
+      if (t < 2 * l) {
+        px = t < l ? c0 + t : spiv[c0 + t - l];
+        int y = px;
+        for (int j = l - 1; j >= 0; --j) {
+          const int cj = c0 + j, pj = spiv[cj];
+          y = y == cj ? pj : (y == pj ? cj : y);
+        }
+        pv = perm[y];
       }
-      pv = perm[y];
+      wave_sync();
+      if (px >= 0) perm[px] = pv;  // threads with the same px write the same entry
     }
-    __syncthreads();
-    if (px >= 0) perm[px] = pv;  // threads with the same px write the same entry
     __syncthreads();
     // 5. the leaf's columns back to their physical rows; its rows are now final: their L
     //    entries (earlier leaves' from memory, this leaf's from registers) to L11. The loads
