@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
       //    Every load reads a row of the panel (rows past m read row m - 1) so no load needs a
       //    branch; a row past m only feeds its own padding. The U rows (r < c0, in wave 0)
       //    skip the update.
-      constexpr int kCh = Q <= 2 ? 8 : (Q == 3 ? 4 : (Q <= 8 ? 2 : 1));
+      constexpr int kCh = Q <= 2 ? 8 : (Q == 3 ? 4 : (Q <= 7 ? 2 : 1));
       int prq[Q];
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
@@ -214,13 +214,14 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
         }
       }
       const bool urow = t < c0;  // this thread's first row is a U row
-      for (int m0 = 0; m0 < c0; m0 += kCh) {
-        double lv[Q][kCh];
+      auto load = [&](double (&lv)[Q][kCh], int m0) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
 #pragma unroll
           for (int cc = 0; cc < kCh; ++cc) lv[q][cc] = A[base + (size_t)(m0 + cc) * lda + prq[q]];
         }
+      };
+      auto apply = [&](const double (&lv)[Q][kCh], int m0) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
           if (q > 0 || !urow) {
@@ -232,6 +233,21 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
             }
           }
         }
+      };
+      // two batches: the next one's loads are in flight while the current one is applied
+      double la[Q][kCh], lb[Q][kCh];
+      load(la, 0);
+      for (int m0 = 0;;) {  // c0 is a positive multiple of kLeaf, and kCh divides kLeaf
+        const bool more = m0 + kCh < c0;
+        if (more) load(lb, m0 + kCh);
+        apply(la, m0);
+        m0 += kCh;
+        if (!more) break;
+        const bool more2 = m0 + kCh < c0;
+        if (more2) load(la, m0 + kCh);
+        apply(lb, m0);
+        m0 += kCh;
+        if (!more2) break;
       }
     }
     // 4. the leaf's columns factored, one workgroup barrier per column
