@@ -9,17 +9,19 @@
 //                     leaves of kLeaf columns, left-looking: a leaf's columns are loaded into
 //                     registers, take the updates of the block's earlier leaves (the U rows by
 //                     a wavefront forward substitution, one wave per column, the rows below as
-//                     one streamed update),
-//                     and are factored with one workgroup barrier per column (pivot searches
-//                     reduce on the DPP paths). Row interchanges
-//                     are not applied in memory: the workgroup keeps the map from logical to
-//                     physical row in LDS, reads and writes through it, and lists the block's
-//                     interchanges at the end as (destination, source) row pairs;
+//                     one streamed update), and are factored with one workgroup barrier per
+//                     column (pivot searches reduce on the DPP paths). The kernel avoids
+//                     branches where it can: a row slot's existence is known at compile time
+//                     for all but the last slot, and picks and loads use selects. Row
+//                     interchanges are not applied in memory: the workgroup keeps the map from
+//                     logical to physical row in LDS, reads and writes through it, and lists
+//                     the block's interchanges at the end as (destination, source) row pairs;
 //   lu_laswp_kernel   those row pairs applied to every column of the matrix, all loads of a
 //                     column before its stores (rows that move form closed cycles);
 //   lu_trsm_kernel    the block's top rows of the trailing columns, L11^-1 A12 (one wave per
 //                     column, the forward substitution as a wavefront over its lanes);
-//   rocBLAS dgemm     the trailing update on the fp64 matrix cores.
+//   rocBLAS dgemm     the trailing update on the fp64 matrix cores; a large one is split so
+//                     that the next block's panel runs beside most of it (look-ahead).
 // Every entry takes its updates in the same order and with the same fused multiply-adds as a
 // right-looking factorisation that applies each leaf's rank-kLeaf update as soon as the leaf is
 // factored, so the factors do not depend on the loop order. Pivots are chosen as LAPACK's idamax
