@@ -4,7 +4,7 @@
 //
 // A library dgetrf on a matrix of a few thousand rows spends its time in three kernel launches
 // per column (pivot search, singularity check, scale and update), microseconds each. Here the
-// factorisation is right-looking in blocks of kNB columns, with four launches per block:
+// factorisation is right-looking in blocks of kNB columns, with three launches per block:
 //   lu_panel_kernel   one workgroup factors the block's panel (rows P..n-1, kNB columns) in
 //                     leaves of kLeaf columns, left-looking: a leaf's columns are loaded into
 //                     registers, take the updates of the block's earlier leaves (the U rows by
@@ -16,10 +16,10 @@
 //                     interchanges are not applied in memory: the workgroup keeps the map from
 //                     logical to physical row in LDS, reads and writes through it, and lists
 //                     the block's interchanges at the end as (destination, source) row pairs;
-//   lu_laswp_kernel   those row pairs applied to every column of the matrix, all loads of a
-//                     column before its stores (rows that move form closed cycles);
-//   lu_trsm_kernel    the block's top rows of the trailing columns, L11^-1 A12 (one wave per
-//                     column, the forward substitution as a wavefront over its lanes);
+//   lu_swap_solve_kernel  those row pairs applied to every column of the matrix (all loads of a
+//                     column before its stores), and on the trailing columns the block's top
+//                     rows solved, L11^-1 A12 (one wave per column, the forward substitution as
+//                     a wavefront over its lanes);
 //   rocBLAS dgemm     the trailing update on the fp64 matrix cores; a large one is split so
 //                     that the next block's panel runs beside most of it (look-ahead).
 // Every entry takes its updates in the same order and with the same fused multiply-adds as a
@@ -124,7 +124,8 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
                                                        double* __restrict__ A, int lda,
                                                        int* __restrict__ ipiv,
                                                        int* __restrict__ info,
-                                                       int* __restrict__ bpairs) {
+                                                       int* __restrict__ bpairs,
+                                                       double* __restrict__ l11out) {
   __shared__ int perm[kMaxOrder];
   __shared__ double L11[kNB][kNB + 1];   // final rows of the panel's top kNB rows
   __shared__ double X[kNB][kLeaf];       // the U rows of the current leaf's columns
@@ -408,45 +409,53 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
     bpairs[2 + 2 * t] = P + perm[r];
   }
   if (t == 0) bpairs[0] = 2 * nb;
+  // L11 of a full block, column-major (l11out[s * kNB + r] = L[r][s]), for the trailing solve
+  if (nb == kNB) {
+    for (int e = t; e < kNB * kNB; e += kPT) l11out[e] = L11[e % kNB][e / kNB];
+  }
 }
 
-// the block's row pairs on every column: two columns per workgroup, one pair per thread; a
-// column's loads all land in LDS before any of its stores
-__global__ __launch_bounds__(2 * kMaxPairs) void lu_laswp_kernel(int n, double* __restrict__ A,
-                                                                 int lda,
-                                                                 const int* __restrict__ bpairs) {
-  __shared__ double vals[2][kMaxPairs];
-  const int cnt = bpairs[0];
-  const int half = threadIdx.x / kMaxPairs, i = threadIdx.x % kMaxPairs;
-  const int j = blockIdx.x * 2 + half;
-  const bool ok = j < n && i < cnt;
-  double* cp = A + (size_t)(j < n ? j : 0) * lda;
-  if (ok) vals[half][i] = cp[bpairs[2 + 2 * i]];
-  __syncthreads();
-  if (ok) cp[bpairs[1 + 2 * i]] = vals[half][i];
-}
-
-// A12 <- L11^-1 A12 for the block rows P..P+kNB-1 and the trailing columns pend..n-1 (L11 unit
-// lower, kNB x kNB): one wave per column, lane i holding row i. The forward substitution runs as
-// a wavefront: at step s, row s is final (it has taken its terms 0..s-1) and every later row takes
-// its term s, so each row adds its terms in index order, as a sequential solve does.
-__global__ __launch_bounds__(256) void lu_trsm_kernel(int n, int P, double* __restrict__ A,
-                                                      int lda) {
-  static_assert(kNB == 64, "one lane per block row");
+// The block's row pairs on every column, one wave per column, all loads of a column before its
+// stores (rows that move form closed cycles). On a trailing column (j >= P + kNB) the block rows
+// are then solved with L11 (A12 <- L11^-1 A12, L11 unit lower, read from l11 in column-major
+// order): pair l < kNB has destination P + l, so lane l already holds that row's new value, and
+// the forward substitution runs as a wavefront over the lanes (at step s, row s is final and
+// every later row takes its term s, so each row adds its terms in index order, as a sequential
+// solve does).
+__global__ __launch_bounds__(256) void lu_swap_solve_kernel(int n, int P, double* __restrict__ A,
+                                                            int lda,
+                                                            const int* __restrict__ bpairs,
+                                                            const double* __restrict__ l11) {
+  static_assert(kNB == 64 && kMaxPairs == 2 * 64, "a lane per block row, two pairs per lane");
   const int lane = threadIdx.x & 63;
-  const int col = P + kNB + blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (col >= n) return;  // the whole wave
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= n) return;  // the whole wave
+  const int cnt = bpairs[0];
+  double* cp = A + (size_t)j * lda;
+  const bool ok0 = lane < cnt, ok1 = lane + 64 < cnt;
+  const int d0 = ok0 ? bpairs[1 + 2 * lane] : 0, s0 = ok0 ? bpairs[2 + 2 * lane] : 0;
+  const int d1 = ok1 ? bpairs[1 + 2 * (lane + 64)] : 0, s1 = ok1 ? bpairs[2 + 2 * (lane + 64)] : 0;
+  const double v0 = ok0 ? cp[s0] : 0.0;
+  const double v1 = ok1 ? cp[s1] : 0.0;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // every load of the column has returned
+  if (j < P + kNB) {
+    if (ok0) cp[d0] = v0;
+    if (ok1) cp[d1] = v1;
+    return;
+  }
+  // a trailing column (its block has kNB columns, so cnt = 2 kNB): the block rows are written
+  // once, after the solve; pivot pairs into the block rows carry values lanes already hold
+  if (ok1 && d1 - P >= kNB) cp[d1] = v1;
   double lc[kNB];  // L[lane][s]
 #pragma unroll
-  for (int s = 0; s < kNB; ++s) lc[s] = A[(size_t)(P + s) * lda + P + lane];
-  double* cp = A + (size_t)col * lda + P;
-  double v = cp[lane];
+  for (int s = 0; s < kNB; ++s) lc[s] = l11[s * kNB + lane];
+  double v = v0;
 #pragma unroll
   for (int s = 0; s < kNB - 1; ++s) {
     const double xs = lane_f64(v, s);
     if (lane > s) v = __builtin_fma(-lc[s], xs, v);
   }
-  cp[lane] = v;
+  cp[P + lane] = v;
 }
 
 __global__ void lu_zero_info_kernel(int* info) { *info = 0; }
@@ -454,14 +463,14 @@ __global__ void lu_zero_info_kernel(int* info) { *info = 0; }
 // the panel kernel for q rows per thread, q <= Q
 template <int Q>
 void launch_panel(int q, hipStream_t s, int n, int P, int nb, double* A, int lda, int* ipiv,
-                  int* info, int* bpairs) {
+                  int* info, int* bpairs, double* l11) {
   if constexpr (Q > 1) {
     if (q < Q) {
-      launch_panel<Q - 1>(q, s, n, P, nb, A, lda, ipiv, info, bpairs);
+      launch_panel<Q - 1>(q, s, n, P, nb, A, lda, ipiv, info, bpairs, l11);
       return;
     }
   }
-  lu_panel_kernel<Q><<<1, kPT, 0, s>>>(n, P, nb, A, lda, ipiv, info, bpairs);
+  lu_panel_kernel<Q><<<1, kPT, 0, s>>>(n, P, nb, A, lda, ipiv, info, bpairs, l11);
 }
 
 }  // namespace
@@ -499,9 +508,11 @@ int lu_factor_dev(Device* d, hipStream_t s, int n, double* A, int lda, int* ipiv
   O2D_TRY(blas_of(d, s, &h));
   if (rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success)
     return fail(O2D_ERR_HIP, "rocblas_set_pointer_mode failed");
-  void* pb;
-  O2D_TRY(slot(d, kLuPairs, sizeof(int) * (1 + 2 * kMaxPairs), s, &pb));
-  int* bpairs = static_cast<int*>(pb);
+  void* pb;  // the block's L11 (kNB x kNB doubles), then its row pairs
+  O2D_TRY(slot(d, kLuPairs, sizeof(double) * kNB * kNB + sizeof(int) * (1 + 2 * kMaxPairs), s,
+               &pb));
+  double* l11 = static_cast<double*>(pb);
+  int* bpairs = reinterpret_cast<int*>(l11 + kNB * kNB);
   lu_zero_info_kernel<<<1, 1, 0, s>>>(info);
   O2D_LAUNCH_CHECK();
   // look-ahead: a large trailing update of block k is split into the next block's columns, on
@@ -518,17 +529,15 @@ int lu_factor_dev(Device* d, hipStream_t s, int n, double* A, int lda, int* ipiv
   for (int P = 0; P < n; P += kNB) {
     const int nb = std::min(kNB, n - P);
     const int q = (n - P + kPT - 1) / kPT;
-    launch_panel<kMaxQ>(q, s, n, P, nb, A, lda, ipiv, info, bpairs);
+    launch_panel<kMaxQ>(q, s, n, P, nb, A, lda, ipiv, info, bpairs, l11);
     O2D_LAUNCH_CHECK();
     if (rest_pending) O2D_HIP(hipStreamWaitEvent(s, e_rest, 0));
     rest_pending = false;
-    lu_laswp_kernel<<<(unsigned)((n + 1) / 2), 2 * kMaxPairs, 0, s>>>(n, A, lda, bpairs);
+    lu_swap_solve_kernel<<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, P, A, lda, bpairs, l11);
     O2D_LAUNCH_CHECK();
     const int pend = P + nb;
     const int nr = n - pend;
     if (nr > 0) {  // nb = kNB here
-      lu_trsm_kernel<<<(unsigned)((nr + 3) / 4), 256, 0, s>>>(n, P, A, lda);
-      O2D_LAUNCH_CHECK();
       const int nn = nr - kNB >= kLookAhead ? kNB : nr;  // columns updated on s
       const double* L21 = A + (size_t)P * lda + pend;
       if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, nr, nn, kNB,
