@@ -204,7 +204,8 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
       //    entries of kCh columns for all of a thread's rows are loaded before they are used.
       //    Every load reads a row of the panel (rows past m read row m - 1) so no load needs a
       //    branch; a row past m only feeds its own padding. The U rows (r < c0, in wave 0)
-      //    skip the update.
+      //    take the update as well and are set back from X after it, so the loop has no
+      //    branch either (their lanes share a wave with rows that need it).
       constexpr int kCh = Q <= 2 ? 8 : (Q == 3 ? 4 : (Q <= 7 ? 2 : 1));
       int prq[Q];
 #pragma unroll
@@ -216,41 +217,51 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
           for (int c = 0; c < kLeaf; ++c) a[q][c] = X[r][c];
         }
       }
-      const bool urow = t < c0;  // this thread's first row is a U row
       auto load = [&](double (&lv)[Q][kCh], int m0) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
 #pragma unroll
           for (int cc = 0; cc < kCh; ++cc) lv[q][cc] = A[base + (size_t)(m0 + cc) * lda + prq[q]];
         }
+        __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the batch applied next
       };
       auto apply = [&](const double (&lv)[Q][kCh], int m0) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          if (q > 0 || !urow) {
 #pragma unroll
-            for (int cc = 0; cc < kCh; ++cc) {
+          for (int cc = 0; cc < kCh; ++cc) {
 #pragma unroll
-              for (int c = 0; c < kLeaf; ++c)
-                a[q][c] = __builtin_fma(-lv[q][cc], X[m0 + cc][c], a[q][c]);
-            }
+            for (int c = 0; c < kLeaf; ++c)
+              a[q][c] = __builtin_fma(-lv[q][cc], X[m0 + cc][c], a[q][c]);
           }
         }
       };
-      // two batches: the next one's loads are in flight while the current one is applied
+      // two batches: the next one's loads are in flight while the current one is applied.
+      // The last batch loads itself again instead of branching around the loads, so the
+      // loads of a batch are always followed by those of the next and every wait for a
+      // batch waits for that batch only
+      // The loop is unrolled over the most batches a leaf can have (c0 <= kNB - kLeaf), so no
+      // loop header merges the batches' outstanding loads (not for Q > 7, whose registers
+      // would spill).
+      constexpr int kMaxBatch = (kNB - kLeaf) / kCh;
+      constexpr int kUnroll = Q <= 7 ? kMaxBatch / 2 + 1 : 1;
       double la[Q][kCh], lb[Q][kCh];
       load(la, 0);
-      for (int m0 = 0;;) {  // c0 is a positive multiple of kLeaf, and kCh divides kLeaf
+#pragma unroll kUnroll
+      for (int b = 0; b < kMaxBatch; b += 2) {  // c0 is a positive multiple of kLeaf
+        const int m0 = b * kCh;                 // and kCh divides kLeaf
         const bool more = m0 + kCh < c0;
-        if (more) load(lb, m0 + kCh);
+        load(lb, more ? m0 + kCh : m0);
         apply(la, m0);
-        m0 += kCh;
         if (!more) break;
-        const bool more2 = m0 + kCh < c0;
-        if (more2) load(la, m0 + kCh);
-        apply(lb, m0);
-        m0 += kCh;
+        const bool more2 = m0 + 2 * kCh < c0;
+        load(la, more2 ? m0 + 2 * kCh : m0 + kCh);
+        apply(lb, m0 + kCh);
         if (!more2) break;
+      }
+      if (t < c0) {
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c) a[0][c] = X[t][c];
       }
     }
     // 4. the leaf's columns factored, one workgroup barrier per column
@@ -370,18 +381,23 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
     }
     __syncthreads();
     // 5. the leaf's columns back to their physical rows; its rows are now final: their L
-    //    entries (earlier leaves' from memory, this leaf's from registers) to L11. The loads
-    //    go first: loads and stores share one in-order counter, so a load issued after the
-    //    stores would wait for them as well
-    if constexpr (kPrefetch) {
-      if (k + 1 < nleaf) load_leaf(an, c0 + kLeaf, nb - c0 - kLeaf < kLeaf ? nb - c0 - kLeaf : kLeaf);
-    }
-    double lprev = 0.0;
+    //    entries (earlier leaves' from memory, this leaf's from registers) to L11. Loads and
+    //    stores share one in-order counter, so the order is: the L entries, the stores, then
+    //    the next leaf's loads, and the wait for the L entries waits for nothing after the
+    //    stores. Neither load has a branch around it (a thread with no L entry reads row
+    //    perm[c0], column 0; the next leaf's columns past the panel read its last column),
+    //    so the count of loads in flight is known where the L entries are used.
+    double lprev;
     int ei = -1, em = 0;
-    if (t < l * c0) {
-      ei = t / c0;
-      em = t % c0;
-      lprev = A[base + (size_t)em * lda + perm[c0 + ei]];
+    {
+      const bool mine = t < l * c0;
+      const int tt = mine ? t : 0, cd = c0 > 0 ? c0 : 1;
+      const int xi = tt / cd, xm = tt % cd;
+      lprev = A[base + (size_t)xm * lda + perm[c0 + xi]];
+      if (mine) {
+        ei = xi;
+        em = xm;
+      }
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -391,6 +407,20 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
 #pragma unroll
         for (int c = 0; c < kLeaf; ++c)
           if (c < l) A[base + (size_t)(c0 + c) * lda + pr] = a[q][c];
+      }
+    }
+    if constexpr (kPrefetch) {
+      // the next leaf in logical row order; on the last leaf the loads are not used. Rows past
+      // m read row m - 1 and columns past nb column nb - 1: those entries of the next leaf
+      // are never searched, stored or used for another row
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int pr = perm[exists(q) ? t + q * kPT : m - 1];
+#pragma unroll
+        for (int c = 0; c < kLeaf; ++c) {
+          const int col = c0 + kLeaf + c < nb ? c0 + kLeaf + c : nb - 1;
+          an[q][c] = A[base + (size_t)col * lda + pr];
+        }
       }
     }
     if (t >= c0 && t < c0 + l) {
