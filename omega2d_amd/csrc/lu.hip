@@ -154,24 +154,14 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
         dst[q][c] = (exists(q) && c < l) ? A[base + (size_t)(c0 + c) * lda + pr] : 0.0;
     }
   };
-  constexpr bool kPrefetch = Q <= 4;
-  double an[kPrefetch ? Q : 1][kLeaf];  // the next leaf, loaded before this leaf's stores
   const int nleaf = (nb + kLeaf - 1) / kLeaf;
+  // 1. the leaf's columns, rows in the current logical order: the first here, each later one
+  //    at the end of the leaf before it, once that leaf's values are stored
+  double a[Q][kLeaf];
+  load_leaf(a, 0, nb < kLeaf ? nb : kLeaf);
   for (int k = 0; k < nleaf; ++k) {
     const int c0 = kLeaf * k;          // the leaf's first column and diagonal row (relative)
     const int l = nb - c0 < kLeaf ? nb - c0 : kLeaf;
-    // 1. the leaf's columns, rows in the current logical order (loaded at the end of the
-    //    previous leaf when the rows are few enough to hold two leaves in registers)
-    double a[Q][kLeaf];
-    if (kPrefetch && k > 0) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-#pragma unroll
-        for (int c = 0; c < kLeaf; ++c) a[q][c] = an[q][c];
-      }
-    } else {
-      load_leaf(a, c0, l);
-    }
     if (k > 0) {
       // 2. the U rows 0..c0-1 of the leaf's columns: wave w takes column w, lane i its row i,
       //    and the forward substitution with the unit-lower L11 runs as a wavefront over the
@@ -409,23 +399,21 @@ __global__ __launch_bounds__(kPT) void lu_panel_kernel(int n, int P, int nb,
           if (c < l) A[base + (size_t)(c0 + c) * lda + pr] = a[q][c];
       }
     }
-    if constexpr (kPrefetch) {
-      // the next leaf in logical row order; on the last leaf the loads are not used. Rows past
-      // m read row m - 1 and columns past nb column nb - 1: those entries of the next leaf
-      // are never searched, stored or used for another row
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int pr = perm[exists(q) ? t + q * kPT : m - 1];
-#pragma unroll
-        for (int c = 0; c < kLeaf; ++c) {
-          const int col = c0 + kLeaf + c < nb ? c0 + kLeaf + c : nb - 1;
-          an[q][c] = A[base + (size_t)col * lda + pr];
-        }
-      }
-    }
     if (t >= c0 && t < c0 + l) {
 #pragma unroll
       for (int c = 0; c < kLeaf; ++c) L11[t][c0 + c] = a[0][c];
+    }
+    // the next leaf in logical row order, into the registers just stored (on the last leaf the
+    // loads are not used). Rows past m read row m - 1 and columns past nb column nb - 1: those
+    // entries are never searched, stored or used for another row
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int pr = perm[exists(q) ? t + q * kPT : m - 1];
+#pragma unroll
+      for (int c = 0; c < kLeaf; ++c) {
+        const int col = c0 + kLeaf + c < nb ? c0 + kLeaf + c : nb - 1;
+        a[q][c] = A[base + (size_t)col * lda + pr];
+      }
     }
     if (ei >= 0) L11[c0 + ei][em] = lprev;
     if (t < l) ipiv[P + c0 + t] = P + spiv[c0 + t] + 1;
