@@ -284,6 +284,19 @@ __device__ void block_partial(long long c0, long long c1, double v, Partial* out
   if (t == 0) out[blockIdx.x] = Partial{s0[0], s1[0], sv[0], 0.0};
 }
 
+// the same sums when only wave 0 holds non-zero values (the other threads' values are zeros,
+// which leave every partial sum of the tree unchanged): the tree over wave 0's lanes in the
+// same pairing (lane t adds lane t + w for w = 32, 16, ..., 1), in registers
+__device__ void wave0_partial(long long c0, long long c1, double v, Partial* out) {
+  if (threadIdx.x >= 64) return;
+  for (int w = 32; w > 0; w >>= 1) {
+    c0 += __shfl_down(c0, w, 64);
+    c1 += __shfl_down(c1, w, 64);
+    v = v + __shfl_down(v, w, 64);
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = Partial{c0, c1, v, 0.0};
+}
+
 // ---- the split scan. A workgroup takes kGroupTargets targets (one per lane) and its kWaves waves
 // split the panels: wave w scans the w-th of kWaves consecutive ranges, staged through its own
 // LDS tile. The sequential scan is a state machine whose result depends on panel order, so each
@@ -388,7 +401,7 @@ __device__ __forceinline__ void chunk_panel(const PanelRec& p, double x, double 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kNearBlock) void near_kernel(int64_t np, const PanelRec* __restrict__ pan,
+__global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const PanelRec* __restrict__ pan,
                                                       int64_t nt, double* __restrict__ tx,
                                                       double* __restrict__ ty,
                                                       double* __restrict__ ts,
@@ -614,7 +627,7 @@ __global__ __launch_bounds__(kNearBlock) void near_kernel(int64_t np, const Pane
     ty[i] = y;
     if (MODE == kClear && ts && tr) ts[i] = s;
   }
-  block_partial<kNearBlock>(c0, (valid && untested) ? 1 : 0, removed, part);
+  wave0_partial(c0, (valid && untested) ? 1 : 0, removed, part);  // targets live in wave 0
 }
 
 // fixed-order sum of the block partials: counts[0..1], *value; counts[2] = bad panels
