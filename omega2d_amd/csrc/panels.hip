@@ -39,9 +39,10 @@ constexpr int kPTile = 256;  // panels per LDS stage (panels on points)
 constexpr int kPanTile = O2D_PANTILE;  // panels per workgroup (points on panels)
 
 
-// Both panel kernels are built for five waves per SIMD (96 VGPRs, a few spilled outside the
-// panel loop): C3 0.96 -> 0.94 ms (panels on points), 0.97 -> 0.94 ms (points on panels);
-// six waves per SIMD spilled more and were no faster.
+// Both panel kernels are built for five waves per SIMD (96 VGPRs; panels on points spills two
+// values around its tile staging, points on panels reloads eight per particle from scratch):
+// C3 0.96 -> 0.94 ms (panels on points), 0.97 -> 0.94 ms (points on panels); six waves per
+// SIMD spilled more and were no faster.
 template <bool SRC>
 __global__ __launch_bounds__(kBlock, 5) void panels_on_points_kernel(
     int64_t np, const double* __restrict__ nx, const double* __restrict__ ny,
