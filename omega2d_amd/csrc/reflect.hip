@@ -257,36 +257,18 @@ __device__ __forceinline__ void scan_panel(const PanelRec& p, double x, double y
   st.nh += 1;
 }
 
-// fixed-order block sums of (count0, count1, value) into partial[blockIdx.x]
+// a block's (count0, count1, value) sums
 struct Partial {
   long long c0, c1;
   double v;
   double pad;
 };
 
-template <int NB>
-__device__ void block_partial(long long c0, long long c1, double v, Partial* out) {
-  __shared__ long long s0[NB], s1[NB];
-  __shared__ double sv[NB];
-  const int t = threadIdx.x;
-  s0[t] = c0;
-  s1[t] = c1;
-  sv[t] = v;
-  __syncthreads();
-  for (int w = NB / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      s0[t] += s0[t + w];
-      s1[t] += s1[t + w];
-      sv[t] = sv[t] + sv[t + w];
-    }
-    __syncthreads();
-  }
-  if (t == 0) out[blockIdx.x] = Partial{s0[0], s1[0], sv[0], 0.0};
-}
-
-// the same sums when only wave 0 holds non-zero values (the other threads' values are zeros,
-// which leave every partial sum of the tree unchanged): the tree over wave 0's lanes in the
-// same pairing (lane t adds lane t + w for w = 32, 16, ..., 1), in registers
+// fixed-order block sums of (count0, count1, value) into partial[blockIdx.x] when only wave 0
+// holds non-zero values: a tree over wave 0's lanes (lane t adds lane t + w for w = 32, 16, ...,
+// 1), in registers. It pairs the values as a tree over all the block's threads would, since the
+// other threads' zeros leave every partial sum of that tree unchanged (up to the sign of a
+// zero sum).
 __device__ void wave0_partial(long long c0, long long c1, double v, Partial* out) {
   if (threadIdx.x >= 64) return;
   for (int w = 32; w > 0; w >>= 1) {
