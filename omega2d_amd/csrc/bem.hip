@@ -297,13 +297,23 @@ int bem_factor_dev(Device* d, hipStream_t s, int64_t n, const double* A, int64_t
   return 0;
 }
 
+// the n x n identity, column-major (the right-hand sides whose solutions are A^-1)
+__global__ __launch_bounds__(kBlock) void identity_kernel(int64_t n, double* __restrict__ a) {
+  const int64_t nn = n * n;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nn;
+       e += (int64_t)gridDim.x * kBlock)
+    a[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
+}
+
 // Solves A x = b with the kept factors. A matrix that is solved against again and again (a
 // body that does not move keeps its matrix for the whole run) is worth inverting: the fourth
-// solve with the same factors builds A^-1 from them (rocsolver getri, about half the cost of the
-// factorisation at order 2049) and every solve from then on is one dgemv, an HBM-bound read of
-// n^2 doubles instead of two latency-bound triangular solves (1.1 ms at order 2049). A body in
-// relative motion is refactored every step and solved once per Runge-Kutta stage, at most three
-// times, so its factors never pay for an inverse.
+// solve with the same factors builds A^-1 from them, as the solution of A X = I (rocsolver getrs
+// with n right-hand sides: the row interchanges, then two triangular solves with n columns on
+// the fp64 matrix cores; rocsolver getri took 4.8 ms at order 2049 and 22.7 ms at 4097), and
+// every solve from then on is one dgemv, an HBM-bound read of n^2 doubles instead of two
+// latency-bound triangular solves (1.1 ms at order 2049). A body in relative motion is
+// refactored every step and solved once per Runge-Kutta stage, at most three times, so its
+// factors never pay for an inverse.
 constexpr int kSolvesBeforeInverse = 3;
 
 int bem_solve_dev(Device* d, hipStream_t s, int64_t n, const double* b, double* x) {
@@ -314,11 +324,15 @@ int bem_solve_dev(Device* d, hipStream_t s, int64_t n, const double* b, double* 
   if (!d->inv_ready && d->lu_solves >= kSolvesBeforeInverse) {
     void* inv;
     O2D_TRY(slot(d, kInv, sizeof(double) * n * n, s, &inv));
-    O2D_HIP(hipMemcpyAsync(inv, d->buf[kLU], sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-    if (rocsolver_dgetri(h, (rocblas_int)n, static_cast<double*>(inv), (rocblas_int)n,
-                         static_cast<rocblas_int*>(d->buf[kPivots]),
-                         static_cast<rocblas_int*>(d->buf[kInfo])) != rocblas_status_success)
-      return fail(O2D_ERR_HIP, "rocsolver_dgetri failed");
+    const int64_t nn = n * n;
+    const unsigned grid = (unsigned)std::min<int64_t>((nn + kBlock - 1) / kBlock, 65536);
+    identity_kernel<<<grid, kBlock, 0, s>>>(n, static_cast<double*>(inv));
+    O2D_LAUNCH_CHECK();
+    if (rocsolver_dgetrs(h, rocblas_operation_none, (rocblas_int)n, (rocblas_int)n,
+                         static_cast<double*>(d->buf[kLU]), (rocblas_int)n,
+                         static_cast<const rocblas_int*>(d->buf[kPivots]),
+                         static_cast<double*>(inv), (rocblas_int)n) != rocblas_status_success)
+      return fail(O2D_ERR_HIP, "rocsolver_dgetrs failed");
     d->inv_ready = true;
   }
   ++d->lu_solves;
