@@ -305,16 +305,16 @@ __global__ __launch_bounds__(kBlock) void identity_kernel(int64_t n, double* __r
     a[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
 }
 
-// Solves A x = b with the kept factors. A matrix that is solved against again and again (a
-// body that does not move keeps its matrix for the whole run) is worth inverting: the fourth
-// solve with the same factors builds A^-1 from them, as the solution of A X = I (rocsolver getrs
-// with n right-hand sides: the row interchanges, then two triangular solves with n columns on
-// the fp64 matrix cores; rocsolver getri took 4.8 ms at order 2049 and 22.7 ms at 4097), and
-// every solve from then on is one dgemv, an HBM-bound read of n^2 doubles instead of two
-// latency-bound triangular solves (1.1 ms at order 2049). A body in relative motion is
-// refactored every step and solved once per Runge-Kutta stage, at most three times, so its
-// factors never pay for an inverse.
-constexpr int kSolvesBeforeInverse = 3;
+// Solves A x = b with the kept factors. A matrix solved against more than once with the same
+// factors is worth inverting: the second solve builds A^-1 from them, as the solution of A X = I
+// (rocsolver getrs with n right-hand sides: the row interchanges, then two triangular solves
+// with n columns on the fp64 matrix cores; 1.2 ms at order 2049 and 5.3 ms at 4097, where
+// rocsolver getri took 4.8 and 22.7 ms), about the cost of one solve by two latency-bound
+// triangular solves (1.1 and 2.8 ms). Every solve from then on is one dgemv, an HBM-bound read
+// of n^2 doubles (0.09 and 0.17 ms). A body that does not move keeps its inverse for the whole
+// run; a body in relative motion, refactored every step and solved once per Runge-Kutta stage,
+// builds it at its second stage.
+constexpr int kSolvesBeforeInverse = 1;
 
 int bem_solve_dev(Device* d, hipStream_t s, int64_t n, const double* b, double* x) {
   if (!d->lu_ready || d->lu_n != n)

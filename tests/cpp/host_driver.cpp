@@ -150,7 +150,7 @@ void run_bem(Reader& in, Writer& out, const std::string& mode) {
 
 // Wall time of whole advection steps (src/Convection.h:249-598: every stage solves the BEM,
 // sums panels and particles on the wake, moves it and clears the inner layer), after three
-// untimed steps (the matrix is built and factored once; the fourth solve with the same factors
+// untimed steps (the matrix is built and factored once; the second solve with the same factors
 // builds the inverse). header: npan, nnodes, nwake, nunk, core, order, nsteps, precision; then
 // as for run_bem. Output: milliseconds per step, then the wake positions.
 void run_bem_time(Reader& in, Writer& out) {

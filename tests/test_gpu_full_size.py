@@ -140,7 +140,7 @@ def test_c3_full_systems_against_lapack(nunk):
         assert rel_l2(Ah, ref) <= 1e-12 and np.max(np.abs(Ah - ref)) <= 1e-13
     dv.bem_factor(A)
     fro = np.linalg.norm(Ah)
-    for k in range(5):  # the fresh factors, then the inverse path (from the fourth solve)
+    for k in range(5):  # the fresh factors, then the inverse path (from the second solve)
         b = rng.normal(size=n)
         x = _np(dv.bem_solve(_cu(b)))
         # two backward-stable fp64 solves agree to about cond(A) eps; cond(A) = 9.3e3 here, so

@@ -87,7 +87,7 @@ def test_bem_factor_solve_matches_lapack(nunk):
     xr = np.linalg.solve(A, b)
     assert rel_l2(x, xr) <= 1e-12
     # solve again with the same factors (the reference re-uses its solver until A changes): the
-    # fourth solve builds the explicit inverse, later ones are matrix-vector products
+    # second solve builds the explicit inverse, later ones are matrix-vector products
     for k in range(5):
         bk = rng.normal(size=A.shape[0])
         assert rel_l2(o2d.bem_solve(bk), np.linalg.solve(A, bk)) <= 1e-12
@@ -111,7 +111,7 @@ def test_bem_repeated_device_solves_in_place():
     dA = torch.from_numpy(np.asfortranarray(A).ravel(order="F").copy()).cuda()
     dv.bem_factor(dA.view(A.shape[0], A.shape[0]))
     rng = np.random.default_rng(3)
-    for k in range(6):  # triangular solves, then the inverse (built on the fourth)
+    for k in range(6):  # triangular solves, then the inverse (built on the second)
         b = rng.normal(size=A.shape[0])
         db = torch.from_numpy(b).cuda()
         dv.bem_solve(db, db)
@@ -125,7 +125,7 @@ def test_lu_of_general_matrices_matches_lapack(n):
     rng = np.random.default_rng(n)
     A = rng.normal(size=(n, n))
     o2d.bem_factor(A)
-    for k in range(5):  # the fresh factors, then the inverse built from them (fourth solve)
+    for k in range(5):  # the fresh factors, then the inverse built from them (second solve)
         b = rng.normal(size=n)
         x = o2d.bem_solve(b)
         assert rel_l2(x, np.linalg.solve(A, b)) <= 1e-9, (n, k)
