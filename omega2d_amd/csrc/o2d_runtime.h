@@ -29,7 +29,7 @@ enum Slot : int {
   kLU, kPivots, kInfo,
   kRkPtrs, kRkStage,
   kGradX, kGradY, kGradS, kGradR,
-  kNearPart, kNodeNorm, kPanRec, kCutTable, kNearCounts,
+  kNearPart, kNodeNorm, kPanRec, kCutTable, kNearCounts, kNearTiles,
   kInv, kSolveTmp,
   kTreeKeys, kTreeOrder, kTreeTOrder, kTreeSrc, kTreeCells, kTreeMom,
   kPan2X, kPan2Y, kPan2Idx, kPanGeom2,

@@ -382,6 +382,116 @@ __device__ __forceinline__ void chunk_panel(const PanelRec& p, double x, double 
   }
 }
 
+// ---- tile pruning. The scan takes panels in tiles of kWaveTile consecutive panels, each with
+// the bounding box of its end nodes. Before a pass every active target gets an upper bound U
+// on its nearest distance (squared): the distance to end node 0 of every kProbeStride-th panel,
+// computed as the scan computes a node distance, which is at least the distance the scan
+// computes for that panel up to a few ulps. A tile whose box lies farther (squared) from the
+// box of the group's active targets than next_up(next_up(max U)) (1 + 2^-40) holds no panel
+// within two ulps of any of those targets' smallest distance M (the relative margin covers the
+// rounding of both bounds); such panels never join a nearest set and never enter the split
+// scan's lists, so the tile is skipped. The kept tiles are dealt to the waves in turn, so each
+// wave's panels still ascend, as the replay of its lists needs.
+constexpr int kProbeStride = 16;
+
+// box[4 t ...] = (x min, x max, y min, y max) of the end nodes of tile t (non-finite nodes
+// are left out by fmin/fmax: a panel with one never enters a nearest set)
+__global__ __launch_bounds__(kBlock) void tile_box_kernel(int64_t np,
+                                                          const PanelRec* __restrict__ pan,
+                                                          int64_t ntile, double* __restrict__ box) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ntile) return;
+  const int64_t k0 = t * kWaveTile, k1 = k0 + kWaveTile < np ? k0 + kWaveTile : np;
+  double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+  for (int64_t k = k0; k < k1; ++k) {
+    const PanelRec& p = pan[k];
+    x0 = fmin(x0, fmin(p.x0, p.x1));
+    x1 = fmax(x1, fmax(p.x0, p.x1));
+    y0 = fmin(y0, fmin(p.y0, p.y1));
+    y1 = fmax(y1, fmax(p.y0, p.y1));
+  }
+  box[4 * t] = x0;
+  box[4 * t + 1] = x1;
+  box[4 * t + 2] = y0;
+  box[4 * t + 3] = y1;
+}
+
+// ---- target order. Targets are scanned in groups of 64 in Morton order of their positions, so
+// that a group is compact and its box excludes most tiles. The order changes no target's result.
+constexpr int kBoxBlocks = 256;
+
+// part[4 b ...] = (x min, x max, y min, y max) of targets b, b + grid, ... (non-finite values
+// only spoil the order, never a result)
+__global__ __launch_bounds__(kBlock) void target_box_kernel(int64_t n, const double* __restrict__ x,
+                                                            const double* __restrict__ y,
+                                                            double* __restrict__ part) {
+  __shared__ double sm[4][kBlock];
+  double a = DBL_MAX, b = -DBL_MAX, c = DBL_MAX, e = -DBL_MAX;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    a = fmin(a, x[i]);
+    b = fmax(b, x[i]);
+    c = fmin(c, y[i]);
+    e = fmax(e, y[i]);
+  }
+  const int t = threadIdx.x;
+  sm[0][t] = a;
+  sm[1][t] = b;
+  sm[2][t] = c;
+  sm[3][t] = e;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      sm[0][t] = fmin(sm[0][t], sm[0][t + w]);
+      sm[1][t] = fmax(sm[1][t], sm[1][t + w]);
+      sm[2][t] = fmin(sm[2][t], sm[2][t + w]);
+      sm[3][t] = fmax(sm[3][t], sm[3][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < 4) part[4 * blockIdx.x + t] = sm[t][0];
+}
+
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {  // bits of v to the even positions
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// the box from the partials, then each target's Morton key on a 65536 x 65536 grid over it
+__global__ __launch_bounds__(kBlock) void target_key_kernel(int64_t n, const double* __restrict__ x,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ part,
+                                                            int nb, uint32_t* __restrict__ key,
+                                                            uint32_t* __restrict__ val) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double a = DBL_MAX, b = -DBL_MAX, c = DBL_MAX, e = -DBL_MAX;
+  for (int k = 0; k < nb; ++k) {
+    a = fmin(a, part[4 * k]);
+    b = fmax(b, part[4 * k + 1]);
+    c = fmin(c, part[4 * k + 2]);
+    e = fmax(e, part[4 * k + 3]);
+  }
+  double qx = (x[i] - a) / (b - a) * 65535.0, qy = (y[i] - c) / (e - c) * 65535.0;
+  qx = qx > 0.0 ? (qx < 65535.0 ? qx : 65535.0) : 0.0;  // NaN to 0
+  qy = qy > 0.0 ? (qy < 65535.0 ? qy : 65535.0) : 0.0;
+  key[i] = spread16((uint32_t)qx) | (spread16((uint32_t)qy) << 1);
+  val[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ double wave_min_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const PanelRec* __restrict__ pan,
                                                       int64_t nt, double* __restrict__ tx,
@@ -389,7 +499,10 @@ __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const P
                                                       double* __restrict__ ts,
                                                       const double* __restrict__ tr,
                                                       double* __restrict__ dist, ClearParams cp,
-                                                      Partial* __restrict__ part) {
+                                                      Partial* __restrict__ part,
+                                                      const uint32_t* __restrict__ torder,
+                                                      const double* __restrict__ tbox,
+                                                      int64_t ntile) {
   __shared__ double wtile[kWaves][kWaveTile * kRecDoubles];  // each wave's panel tile
   __shared__ double gx[kGroupTargets], gy[kGroupTargets];   // the targets of this pass
   __shared__ int gact[kGroupTargets];
@@ -397,16 +510,16 @@ __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const P
   __shared__ int c0n[kWaves][kGroupTargets], c1n[kWaves][kGroupTargets];
   __shared__ int cfs[kWaves][kGroupTargets];                 // f2 | sel << 1
   __shared__ int ch[kWaves][2][kHits][kGroupTargets];
+  __shared__ double ub[kWaves][kGroupTargets];               // the waves' probe bounds
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t i = (int64_t)blockIdx.x * kGroupTargets + lane;
+  // the group's targets are entries gi of the order torder (input order when null)
+  const int64_t gi = (int64_t)blockIdx.x * kGroupTargets + lane;
   double* mytile = wtile[wave];
   const PanelRec* lp = reinterpret_cast<const PanelRec*>(mytile);
-  const int64_t per = (np + kWaves - 1) / kWaves;  // wave w scans panels [k0, k1)
-  const int64_t k0 = wave * per < np ? wave * per : np;
-  const int64_t k1 = k0 + per < np ? k0 + per : np;
   // the target's state lives in wave 0
-  const bool valid = wave == 0 && i < nt;
+  const bool valid = wave == 0 && gi < nt;
+  const int64_t i = valid ? (torder ? (int64_t)torder[gi] : gi) : 0;
   double x = valid ? tx[i] : 0.0, y = valid ? ty[i] : 0.0;
   double s = (valid && ts && tr) ? ts[i] : 0.0;
   const double rad = (valid && tr) ? tr[i] : cp.ips;
@@ -430,13 +543,59 @@ __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const P
     Chunk c;
     chunk_start(c, m0);
     int* hits = &ch[wave][0][0][lane];
-    if (__ballot(aw) != 0 && k0 < k1) {
-      // the wave's share in tiles of kWaveTile panels through its own LDS tile; the next
-      // tile's loads are in flight while the current one is scanned
+    // the bound U from this wave's share of the probe panels, and the box of the active targets
+    double u = DBL_MAX;
+    if (aw) {
+      for (int64_t k = (int64_t)wave * kProbeStride; k < np; k += kWaves * kProbeStride) {
+        const double ax = xw - pan[k].x0, ay = yw - pan[k].y0;
+        const double r2 = ax * ax + ay * ay;
+        u = r2 < u ? r2 : u;
+      }
+    }
+    ub[wave][lane] = u;
+    const double bx0 = wave_min_d(aw ? xw : DBL_MAX), bx1 = wave_max_d(aw ? xw : -DBL_MAX);
+    const double by0 = wave_min_d(aw ? yw : DBL_MAX), by1 = wave_max_d(aw ? yw : -DBL_MAX);
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) u = ub[w][lane] < u ? ub[w][lane] : u;
+    const double umax = wave_max_d(aw ? u : 0.0);
+    const double thr_u = next_up(next_up(umax)) * (1.0 + 0x1p-40) + 0x1p-1000;
+    // the kept tiles of 64 at base, one per lane (called with the whole wave active)
+    auto chunk_mask = [&](int64_t base) -> unsigned long long {
+      const int64_t t = base + lane;
+      bool keep = false;
+      if (t < ntile) {
+        const double* b = tbox + 4 * t;
+        const double dx = fmax(0.0, fmax(b[0] - bx1, bx0 - b[1]));
+        const double dy = fmax(0.0, fmax(b[2] - by1, by0 - b[3]));
+        keep = !((dx * dx + dy * dy) * (1.0 - 0x1p-40) > thr_u);
+      }
+      return __ballot(keep);
+    };
+    // this wave's next kept tile (kept tiles in index order, dealt to the waves in turn), or -1
+    int64_t cbase = 0, rank = 0;
+    unsigned long long cmask = ntile > 0 ? chunk_mask(0) : 0ull;
+    auto next_tile = [&]() -> int64_t {
+      for (;;) {
+        while (cmask == 0ull) {
+          cbase += 64;
+          if (cbase >= ntile) return -1;
+          cmask = chunk_mask(cbase);
+        }
+        const int b = __builtin_ctzll(cmask);
+        cmask &= cmask - 1ull;
+        const int64_t r = rank++;
+        if ((int)(r % kWaves) == wave) return cbase + b;
+      }
+    };
+    const int64_t tfirst = next_tile();
+    if (__ballot(aw) != 0 && tfirst >= 0) {
+      // the wave's tiles through its own LDS tile; the next tile's loads are in flight while
+      // the current one is scanned
       const double* src = reinterpret_cast<const double*>(pan);
       double reg[kTileLoads];
-      int64_t t0 = k0;
-      int n = (int)(k1 - t0 < kWaveTile ? k1 - t0 : kWaveTile);
+      int64_t t0 = tfirst * kWaveTile;
+      int n = (int)(np - t0 < kWaveTile ? np - t0 : kWaveTile);
 #pragma unroll
       for (int j = 0; j < kTileLoads; ++j) {
         const int e = lane + 64 * j;
@@ -447,8 +606,9 @@ __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const P
 #pragma unroll
         for (int j = 0; j < kTileLoads; ++j) mytile[lane + 64 * j] = reg[j];
         wave_sync();
-        const int64_t t1 = t0 + n;
-        const int n1 = (int)(k1 - t1 < kWaveTile ? k1 - t1 : kWaveTile);
+        const int64_t tn = next_tile();
+        const int64_t t1 = tn >= 0 ? tn * kWaveTile : np;
+        const int n1 = tn >= 0 ? (int)(np - t1 < kWaveTile ? np - t1 : kWaveTile) : 0;
         if (n1 > 0) {
 #pragma unroll
           for (int j = 0; j < kTileLoads; ++j) {
@@ -720,6 +880,9 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
   zero_counter_kernel<<<1, 64, 0, s>>>(bad);
   O2D_LAUNCH_CHECK();
   PanelRec* rec = nullptr;
+  double* tbox = nullptr;
+  int64_t ntile = 0;
+  const uint32_t* torder = nullptr;
   if (np > 0) {
     double *nnx = nullptr, *nny = nullptr;
     if (mode != kNearest) {
@@ -759,6 +922,37 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
                                                    mode != kNearest ? pny : nullptr, nnx, nny,
                                                    rec, bad);
     O2D_LAUNCH_CHECK();
+    ntile = (np + kWaveTile - 1) / kWaveTile;
+    void* tb;
+    O2D_TRY(slot(d, kNearTiles, sizeof(double) * 4 * (size_t)ntile, s, &tb));
+    tbox = static_cast<double*>(tb);
+    tile_box_kernel<<<blocks(ntile), kBlock, 0, s>>>(np, rec, ntile, tbox);
+    O2D_LAUNCH_CHECK();
+    if (nt > kGroupTargets && ntile > 1) {
+      // the targets' Morton order (the sort buffers of the node normals are free again)
+      void *kA, *vA, *kB, *vB, *bp;
+      O2D_TRY(slot(d, kKeysA, nt * sizeof(uint32_t), s, &kA));
+      O2D_TRY(slot(d, kOrderA, nt * sizeof(uint32_t), s, &vA));
+      O2D_TRY(slot(d, kKeysB, nt * sizeof(uint32_t), s, &kB));
+      O2D_TRY(slot(d, kOrderB, nt * sizeof(uint32_t), s, &vB));
+      O2D_TRY(slot(d, kBBox, sizeof(double) * 4 * kBoxBlocks, s, &bp));
+      target_box_kernel<<<kBoxBlocks, kBlock, 0, s>>>(nt, tx, ty, static_cast<double*>(bp));
+      O2D_LAUNCH_CHECK();
+      target_key_kernel<<<blocks(nt), kBlock, 0, s>>>(nt, tx, ty, static_cast<const double*>(bp),
+                                                      kBoxBlocks, static_cast<uint32_t*>(kA),
+                                                      static_cast<uint32_t*>(vA));
+      O2D_LAUNCH_CHECK();
+      size_t tbytes = 0;
+      O2D_HIP(hipcub::DeviceRadixSort::SortPairs(
+          nullptr, tbytes, static_cast<uint32_t*>(kA), static_cast<uint32_t*>(kB),
+          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nt, 0, 32, s));
+      void* tmp;
+      O2D_TRY(slot(d, kSortTemp, tbytes, s, &tmp));
+      O2D_HIP(hipcub::DeviceRadixSort::SortPairs(
+          tmp, tbytes, static_cast<uint32_t*>(kA), static_cast<uint32_t*>(kB),
+          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nt, 0, 32, s));
+      torder = static_cast<const uint32_t*>(vB);
+    }
   }
   ClearParams cp{};
   cp.method = method;
@@ -780,11 +974,11 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
     if (run) {
       KernelTimer kt(O2D_KT_NEAR, s);
       if (mode == kNearest)
-        near_kernel<kNearest><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+        near_kernel<kNearest><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part, torder, tbox, ntile);
       else if (mode == kReflect)
-        near_kernel<kReflect><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+        near_kernel<kReflect><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part, torder, tbox, ntile);
       else
-        near_kernel<kClear><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part);
+        near_kernel<kClear><<<nblk, kNearBlock, 0, s>>>(np, rec, nt, tx, ty, ts, tr, dist, cp, part, torder, tbox, ntile);
       kt.stop();
       O2D_LAUNCH_CHECK();
     }
