@@ -395,25 +395,34 @@ __device__ __forceinline__ void chunk_panel(const PanelRec& p, double x, double 
 constexpr int kProbeStride = 16;
 
 // box[4 t ...] = (x min, x max, y min, y max) of the end nodes of tile t (non-finite nodes
-// are left out by fmin/fmax: a panel with one never enters a nearest set)
+// are left out by fmin/fmax: a panel with one never enters a nearest set). One thread per
+// panel; a tile is 32 consecutive threads of a wave, reduced with shuffles.
+static_assert(kWaveTile == 32, "a tile is half a wave");
 __global__ __launch_bounds__(kBlock) void tile_box_kernel(int64_t np,
                                                           const PanelRec* __restrict__ pan,
                                                           int64_t ntile, double* __restrict__ box) {
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= ntile) return;
-  const int64_t k0 = t * kWaveTile, k1 = k0 + kWaveTile < np ? k0 + kWaveTile : np;
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // k < 32 ntile (the grid)
   double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
-  for (int64_t k = k0; k < k1; ++k) {
+  if (k < np) {
     const PanelRec& p = pan[k];
-    x0 = fmin(x0, fmin(p.x0, p.x1));
-    x1 = fmax(x1, fmax(p.x0, p.x1));
-    y0 = fmin(y0, fmin(p.y0, p.y1));
-    y1 = fmax(y1, fmax(p.y0, p.y1));
+    x0 = fmin(p.x0, p.x1);
+    x1 = fmax(p.x0, p.x1);
+    y0 = fmin(p.y0, p.y1);
+    y1 = fmax(p.y0, p.y1);
   }
-  box[4 * t] = x0;
-  box[4 * t + 1] = x1;
-  box[4 * t + 2] = y0;
-  box[4 * t + 3] = y1;
+  for (int o = 16; o > 0; o >>= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, o, 64));
+    x1 = fmax(x1, __shfl_xor(x1, o, 64));
+    y0 = fmin(y0, __shfl_xor(y0, o, 64));
+    y1 = fmax(y1, __shfl_xor(y1, o, 64));
+  }
+  const int64_t t = k / kWaveTile;
+  if (k % kWaveTile == 0 && t < ntile) {
+    box[4 * t] = x0;
+    box[4 * t + 1] = x1;
+    box[4 * t + 2] = y0;
+    box[4 * t + 3] = y1;
+  }
 }
 
 // ---- target order. Targets are scanned in groups of 64 in Morton order of their positions, so
@@ -461,24 +470,37 @@ __device__ __forceinline__ uint32_t spread16(uint32_t v) {  // bits of v to the 
   return v;
 }
 
-// the box from the partials, then each target's Morton key on a 65536 x 65536 grid over it
+// the box from the partials (once per block), then each target's Morton key on a 256 x 256 grid
+// over it (16 bits: groups of 64 targets stay compact and the sort takes two digit passes)
+constexpr int kKeyBits = 16;
+static_assert(kBoxBlocks <= kBlock, "one partial per thread of a block");
 __global__ __launch_bounds__(kBlock) void target_key_kernel(int64_t n, const double* __restrict__ x,
                                                             const double* __restrict__ y,
                                                             const double* __restrict__ part,
                                                             int nb, uint32_t* __restrict__ key,
                                                             uint32_t* __restrict__ val) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  double a = DBL_MAX, b = -DBL_MAX, c = DBL_MAX, e = -DBL_MAX;
-  for (int k = 0; k < nb; ++k) {
-    a = fmin(a, part[4 * k]);
-    b = fmax(b, part[4 * k + 1]);
-    c = fmin(c, part[4 * k + 2]);
-    e = fmax(e, part[4 * k + 3]);
+  __shared__ double sm[4][kBlock];
+  const int t = threadIdx.x;
+  sm[0][t] = t < nb ? part[4 * t] : DBL_MAX;
+  sm[1][t] = t < nb ? part[4 * t + 1] : -DBL_MAX;
+  sm[2][t] = t < nb ? part[4 * t + 2] : DBL_MAX;
+  sm[3][t] = t < nb ? part[4 * t + 3] : -DBL_MAX;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      sm[0][t] = fmin(sm[0][t], sm[0][t + w]);
+      sm[1][t] = fmax(sm[1][t], sm[1][t + w]);
+      sm[2][t] = fmin(sm[2][t], sm[2][t + w]);
+      sm[3][t] = fmax(sm[3][t], sm[3][t + w]);
+    }
+    __syncthreads();
   }
-  double qx = (x[i] - a) / (b - a) * 65535.0, qy = (y[i] - c) / (e - c) * 65535.0;
-  qx = qx > 0.0 ? (qx < 65535.0 ? qx : 65535.0) : 0.0;  // NaN to 0
-  qy = qy > 0.0 ? (qy < 65535.0 ? qy : 65535.0) : 0.0;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + t;
+  if (i >= n) return;
+  const double a = sm[0][0], b = sm[1][0], c = sm[2][0], e = sm[3][0];
+  double qx = (x[i] - a) / (b - a) * 255.0, qy = (y[i] - c) / (e - c) * 255.0;
+  qx = qx > 0.0 ? (qx < 255.0 ? qx : 255.0) : 0.0;  // NaN to 0
+  qy = qy > 0.0 ? (qy < 255.0 ? qy : 255.0) : 0.0;
   key[i] = spread16((uint32_t)qx) | (spread16((uint32_t)qy) << 1);
   val[i] = (uint32_t)i;
 }
@@ -926,7 +948,7 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
     void* tb;
     O2D_TRY(slot(d, kNearTiles, sizeof(double) * 4 * (size_t)ntile, s, &tb));
     tbox = static_cast<double*>(tb);
-    tile_box_kernel<<<blocks(ntile), kBlock, 0, s>>>(np, rec, ntile, tbox);
+    tile_box_kernel<<<blocks(ntile * kWaveTile), kBlock, 0, s>>>(np, rec, ntile, tbox);
     O2D_LAUNCH_CHECK();
     if (nt > kGroupTargets && ntile > 1) {
       // the targets' Morton order (the sort buffers of the node normals are free again)
@@ -945,12 +967,12 @@ int near_dev(Device* d, hipStream_t s, int mode, int method, double cutoff_mult,
       size_t tbytes = 0;
       O2D_HIP(hipcub::DeviceRadixSort::SortPairs(
           nullptr, tbytes, static_cast<uint32_t*>(kA), static_cast<uint32_t*>(kB),
-          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nt, 0, 32, s));
+          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nt, 0, kKeyBits, s));
       void* tmp;
       O2D_TRY(slot(d, kSortTemp, tbytes, s, &tmp));
       O2D_HIP(hipcub::DeviceRadixSort::SortPairs(
           tmp, tbytes, static_cast<uint32_t*>(kA), static_cast<uint32_t*>(kB),
-          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nt, 0, 32, s));
+          static_cast<uint32_t*>(vA), static_cast<uint32_t*>(vB), (int)nt, 0, kKeyBits, s));
       torder = static_cast<const uint32_t*>(vB);
     }
   }
