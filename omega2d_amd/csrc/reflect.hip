@@ -4,10 +4,12 @@
 // (clear_inner_panp2, :487-727; method 1 pushes targets out to the cutoff layer, method 0 trims
 // strength with the cut tables of :333-432).
 //
-// Every target scans all panels in index order, as the reference does, because the set of
+// Every target takes the panels in index order, as the reference does, because the set of
 // nearest elements depends on that order: a distance below nextafter(min, 0) replaces the set,
-// one below nextafter(min, +max) joins it. The scan is one thread per target over panels staged
-// through LDS. All arithmetic is IEEE double with no fused multiply-add, in the reference's
+// one below nextafter(min, +max) joins it. Groups of 64 targets (in Morton order) split the
+// panels over four waves that keep exact summaries, and skip tiles of panels that lie beyond a
+// bound on every group target's nearest distance (both below, with the argument that neither
+// changes a result). All arithmetic is IEEE double with no fused multiply-add, in the reference's
 // operation order, so results are bit-identical to a plain C loop. Two shortcuts keep the
 // common case free of divisions without changing any result:
 //   * t = dot/|b|^2 lies in (0,1) exactly when 0 < dot < |b|^2, unless the quotient could
