@@ -388,11 +388,12 @@ __device__ __forceinline__ void chunk_panel(const PanelRec& p, double x, double 
 // the bounding box of its end nodes. Before a pass every active target gets an upper bound U
 // on its nearest distance (squared): the distance to end node 0 of every kProbeStride-th panel,
 // computed as the scan computes a node distance, which is at least the distance the scan
-// computes for that panel up to a few ulps. A tile whose box lies farther (squared) from the
-// box of the group's active targets than next_up(next_up(max U)) (1 + 2^-40) holds no panel
-// within two ulps of any of those targets' smallest distance M (the relative margin covers the
-// rounding of both bounds); such panels never join a nearest set and never enter the split
-// scan's lists, so the tile is skipped. The kept tiles are dealt to the waves in turn, so each
+// computes for that panel up to a few ulps. A tile whose box lies farther (squared) from every
+// active target than next_up(next_up(U)) (1 + 2^-40) of that target holds no panel within two
+// ulps of any of those targets' smallest distance M (the relative margin covers the rounding of
+// both bounds); such panels never join a nearest set and never enter the split scan's lists,
+// so the tile is skipped. The box of the group's targets and the largest U sift the tiles
+// first, one tile per lane. The kept tiles are dealt to the waves in turn, so each
 // wave's panels still ascend, as the replay of its lists needs.
 constexpr int kProbeStride = 16;
 
@@ -582,9 +583,12 @@ __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const P
     __syncthreads();
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) u = ub[w][lane] < u ? ub[w][lane] : u;
+    const double thr_l = next_up(next_up(u)) * (1.0 + 0x1p-40) + 0x1p-1000;  // this target's
     const double umax = wave_max_d(aw ? u : 0.0);
-    const double thr_u = next_up(next_up(umax)) * (1.0 + 0x1p-40) + 0x1p-1000;
-    // the kept tiles of 64 at base, one per lane (called with the whole wave active)
+    const double thr_u = next_up(next_up(umax)) * (1.0 + 0x1p-40) + 0x1p-1000;  // the group's
+    // the kept tiles of 64 at base (called with the whole wave active): first against the box
+    // of the active targets and the group's bound, one tile per lane; then each of those
+    // against every active target and its own bound
     auto chunk_mask = [&](int64_t base) -> unsigned long long {
       const int64_t t = base + lane;
       bool keep = false;
@@ -594,7 +598,16 @@ __global__ __launch_bounds__(kNearBlock, 5) void near_kernel(int64_t np, const P
         const double dy = fmax(0.0, fmax(b[2] - by1, by0 - b[3]));
         keep = !((dx * dx + dy * dy) * (1.0 - 0x1p-40) > thr_u);
       }
-      return __ballot(keep);
+      unsigned long long out = 0ull;
+      for (unsigned long long r = __ballot(keep); r != 0ull; r &= r - 1ull) {
+        const int j = __builtin_ctzll(r);
+        const double* b = tbox + 4 * (base + j);
+        const double dx = fmax(0.0, fmax(b[0] - xw, xw - b[1]));
+        const double dy = fmax(0.0, fmax(b[2] - yw, yw - b[3]));
+        const bool need = aw && !((dx * dx + dy * dy) * (1.0 - 0x1p-40) > thr_l);
+        if (__ballot(need) != 0ull) out |= 1ull << j;
+      }
+      return out;
     };
     // this wave's next kept tile (kept tiles in index order, dealt to the waves in turn), or -1
     int64_t cbase = 0, rank = 0;
