@@ -33,7 +33,7 @@ enum Slot : int {
   kInv, kSolveTmp,
   kTreeKeys, kTreeOrder, kTreeTOrder, kTreeSrc, kTreeCells, kTreeMom,
   kPan2X, kPan2Y, kPan2Idx, kPanGeom2,
-  kLuPairs,
+  kLuPairs, kPanClus,
   kNumSlots
 };
 
@@ -91,6 +91,11 @@ struct Workspace {
 int take(Device* d, Workspace* ws, Slot which, size_t bytes, hipStream_t s, void** out);
 
 int check_env(const o2d_env* env);
+
+// The Morton order of n points (x, y device arrays): *order (n entries, in the kOrderB slot, so
+// valid until that slot is next used) lists the point indices along the curve. morton.hip.
+int morton_order16(Device* d, hipStream_t s, int64_t n, const double* x, const double* y,
+                   const uint32_t** order);
 
 // Optional per-kernel timing: KernelTimer t(cls, s); ... launch ...; t.stop();
 // records a HIP event pair on s around the launch when timing is enabled.

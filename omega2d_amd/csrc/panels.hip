@@ -87,6 +87,157 @@ __global__ __launch_bounds__(kBlock, 5) void panels_on_points_kernel(
   }
 }
 
+// ---- clustered far field of panels on points. The panels are cut into clusters of kClus
+// consecutive panels (contiguous along a body). About a cluster's centre c, with W_k the
+// complex factor of panel k (panel_pre: v + i u = W L) and a_k = z0_k - c, b_k = z1_k - c,
+//   sum_k W_k L_k(z) = sum_{n>=1} A_n (z - c)^-n,   A_n = sum_k W_k (b_k^n - a_k^n) / n,
+// for |z - c| > R = max |end node - c|. b^n - a^n = (z1 - z0) S_n with S_1 = 1 and
+// S_{n+1} = b S_n + a^n (S_n = sum_{j<n} b^j a^(n-1-j)), free of the cancellation of two close
+// powers. |S_n| <= n R^(n-1), so the terms past kTerms sum to at most
+// (R / |z - c|)^kTerms / (1 - R / |z - c|) of sum_k |W_k| |z1_k - z0_k| / |z - c|, the scale of
+// the first term: below 1e-17 of it at |z - c| >= kFarRatio R, where a target takes the
+// expansion (the per-panel series stops at the same relative size). Nearer targets sum the
+// cluster's panels one by one as the direct kernel does.
+constexpr int kClus = 32;
+constexpr int kTerms = 22;
+constexpr double kFarRatio = 6.0;
+constexpr int kHead = 4;  // per cluster: centre x, y, (kFarRatio R)^2, unused
+
+// head[kHead c ...] and mom[2 kTerms c + 2 (n - 1) + {0, 1}] = A_n of cluster c: one workgroup
+// of 64 threads per cluster, a thread per panel, the moments summed over panels in order
+__global__ __launch_bounds__(64) void cluster_kernel(int64_t np, const double* __restrict__ nx,
+                                                     const double* __restrict__ ny,
+                                                     const uint32_t* __restrict__ idx,
+                                                     const double* __restrict__ vs,
+                                                     const double* __restrict__ ss,
+                                                     double* __restrict__ head,
+                                                     double* __restrict__ mom) {
+  __shared__ double ends[kClus][4];
+  __shared__ double part[kClus][kTerms][2];
+  __shared__ double cen[2];
+  const int t = threadIdx.x;
+  const int64_t c = blockIdx.x, k0 = c * kClus;
+  const int nk = (int)(np - k0 < kClus ? np - k0 : kClus);
+  PanelPre q{};
+  if (t < nk) {
+    const uint32_t a = idx[2 * (k0 + t)], b = idx[2 * (k0 + t) + 1];
+    q = panel_pre(nx[a], ny[a], nx[b], ny[b], vs[k0 + t], ss ? ss[k0 + t] : 0.0);
+    ends[t][0] = q.x0;
+    ends[t][1] = q.y0;
+    ends[t][2] = q.x1;
+    ends[t][3] = q.y1;
+  }
+  __syncthreads();
+  if (t == 0) {  // the centre of the end nodes' box, and R^2
+    double x0 = ends[0][0], x1 = x0, y0 = ends[0][1], y1 = y0;
+    for (int k = 0; k < nk; ++k) {
+      x0 = fmin(x0, fmin(ends[k][0], ends[k][2]));
+      x1 = fmax(x1, fmax(ends[k][0], ends[k][2]));
+      y0 = fmin(y0, fmin(ends[k][1], ends[k][3]));
+      y1 = fmax(y1, fmax(ends[k][1], ends[k][3]));
+    }
+    const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+    double r2 = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      for (int e = 0; e < 2; ++e) {
+        const double dx = ends[k][2 * e] - cx, dy = ends[k][2 * e + 1] - cy;
+        r2 = fmax(r2, dx * dx + dy * dy);
+      }
+    }
+    cen[0] = cx;
+    cen[1] = cy;
+    head[kHead * c] = cx;
+    head[kHead * c + 1] = cy;
+    // (kFarRatio R)^2, rounded up by far more than its rounding error
+    head[kHead * c + 2] = kFarRatio * kFarRatio * r2 * (1.0 + 0x1p-30);
+    head[kHead * c + 3] = 0.0;
+  }
+  __syncthreads();
+  if (t < kClus) {
+    if (t < nk) {
+      const double ax = q.x0 - cen[0], ay = q.y0 - cen[1];
+      const double bx = q.x1 - cen[0], by = q.y1 - cen[1];
+      // W (z1 - z0), then S_n and a^n
+      const double dzx = q.x1 - q.x0, dzy = q.y1 - q.y0;
+      const double fx = q.wr * dzx - q.wi * dzy, fy = q.wr * dzy + q.wi * dzx;
+      double sx = 1.0, sy = 0.0, px = ax, py = ay;  // S_1, a^1
+      for (int n = 1; n <= kTerms; ++n) {
+        const double inv = 1.0 / (double)n;
+        part[t][n - 1][0] = (fx * sx - fy * sy) * inv;
+        part[t][n - 1][1] = (fx * sy + fy * sx) * inv;
+        const double nsx = bx * sx - by * sy + px, nsy = bx * sy + by * sx + py;  // b S + a^n
+        sx = nsx;
+        sy = nsy;
+        const double npx = px * ax - py * ay;  // a^(n+1)
+        py = px * ay + py * ax;
+        px = npx;
+      }
+    } else {
+      for (int n = 0; n < kTerms; ++n) part[t][n][0] = part[t][n][1] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (t < kTerms) {
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < kClus; ++k) {
+      ar += part[k][t][0];
+      ai += part[k][t][1];
+    }
+    mom[2 * (kTerms * c + t)] = ar;
+    mom[2 * (kTerms * c + t) + 1] = ai;
+  }
+}
+
+// panels on points with the clustered far field: one lane per target (targets taken in the
+// order torder, so that a wave is compact and its lanes agree on near and far), every cluster
+template <bool SRC>
+__global__ __launch_bounds__(kBlock, 5) void panels_clustered_kernel(
+    int64_t np, const double* __restrict__ nx, const double* __restrict__ ny,
+    const uint32_t* __restrict__ idx, const double* __restrict__ vs, const double* __restrict__ ss,
+    int64_t nclus, const double* __restrict__ head, const double* __restrict__ mom, int64_t nt,
+    const double* __restrict__ tx, const double* __restrict__ ty,
+    const uint32_t* __restrict__ torder, double* __restrict__ ou, double* __restrict__ ov) {
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t gg = g < nt ? g : nt - 1;
+  const int64_t i = torder ? (int64_t)torder[gg] : gg;
+  const double px = tx[i], py = ty[i];
+  double au = 0.0, av = 0.0;
+  for (int64_t c = 0; c < nclus; ++c) {
+    const double* h = head + kHead * c;
+    const double zx = px - h[0], zy = py - h[1];
+    const double z2 = __builtin_fma(zx, zx, zy * zy);
+    if (z2 >= h[2]) {
+      // w = 1 / (z - c); sum_n A_n w^n = w (A_1 + w (A_2 + ... + w A_kTerms))
+      const double iz = rcp64(z2);
+      const double wx = zx * iz, wy = -zy * iz;
+      const double* A = mom + 2 * kTerms * c;
+      double sx = A[2 * (kTerms - 1)], sy = A[2 * (kTerms - 1) + 1];
+#pragma unroll
+      for (int n = kTerms - 2; n >= 0; --n) {
+        const double nx_ = __builtin_fma(sx, wx, __builtin_fma(-sy, wy, A[2 * n]));
+        sy = __builtin_fma(sx, wy, __builtin_fma(sy, wx, A[2 * n + 1]));
+        sx = nx_;
+      }
+      av += __builtin_fma(sx, wx, -sy * wy);  // v + i u = W L
+      au += __builtin_fma(sx, wy, sy * wx);
+    } else {
+      const int64_t k1 = (c + 1) * kClus < np ? (c + 1) * kClus : np;
+      for (int64_t k = c * kClus; k < k1; ++k) {
+        const uint32_t a = idx[2 * k], b = idx[2 * k + 1];
+        const PanelPre q = panel_pre(nx[a], ny[a], nx[b], ny[b], vs[k], SRC ? ss[k] : 0.0);
+        double lr, li;
+        panel_log(q, px, py, lr, li);
+        au += __builtin_fma(q.wr, li, q.wi * lr);
+        av += __builtin_fma(q.wr, lr, -q.wi * li);
+      }
+    }
+  }
+  if (g < nt) {
+    ou[i] += au;
+    ov[i] += av;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void reduce_chunks2_kernel(int64_t nchunk, int64_t n,
                                                                 const double* __restrict__ pu,
                                                                 const double* __restrict__ pv,
@@ -185,6 +336,7 @@ __global__ __launch_bounds__(kBlock) void points_on_panels_finish_kernel(
 }
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+constexpr int64_t kClusMin = 4 * kClus;  // panels from which panels on points takes clusters
 
 }  // namespace
 
@@ -194,6 +346,28 @@ int panels_on_points_dev(Device* d, hipStream_t s, int64_t nnodes, const double*
                          double* tu, double* tv) {
   (void)nnodes;
   if (np == 0 || nt == 0) return 0;
+  if (np >= kClusMin) {
+    // the clustered far field, with the targets in Morton order
+    const int64_t nclus = (np + kClus - 1) / kClus;
+    void* hp;
+    O2D_TRY(slot(d, kPanClus, sizeof(double) * (kHead + 2 * kTerms) * nclus, s, &hp));
+    double* head = static_cast<double*>(hp);
+    double* mom = head + kHead * nclus;
+    cluster_kernel<<<(unsigned)nclus, 64, 0, s>>>(np, nx, ny, idx, vs, ss, head, mom);
+    O2D_LAUNCH_CHECK();
+    const uint32_t* torder = nullptr;
+    if (nt > kBlock) O2D_TRY(morton_order16(d, s, nt, tx, ty, &torder));
+    KernelTimer kt(O2D_KT_PANELS, s);
+    if (ss)
+      panels_clustered_kernel<true><<<grid1(nt), kBlock, 0, s>>>(
+          np, nx, ny, idx, vs, ss, nclus, head, mom, nt, tx, ty, torder, tu, tv);
+    else
+      panels_clustered_kernel<false><<<grid1(nt), kBlock, 0, s>>>(
+          np, nx, ny, idx, vs, ss, nclus, head, mom, nt, tx, ty, torder, tu, tv);
+    kt.stop();
+    O2D_LAUNCH_CHECK();
+    return 0;
+  }
   const int64_t twgs = (nt + kBlock - 1) / kBlock;
   // 32 workgroups per CU: at C3 size (200,000 targets, 2,048 panels) that is 11 panel chunks,
   // and 0.96 ms against 1.07 ms with 8 per CU
