@@ -189,20 +189,25 @@ __global__ __launch_bounds__(64) void cluster_kernel(int64_t np, const double* _
 }
 
 // panels on points with the clustered far field: one lane per target (targets taken in the
-// order torder, so that a wave is compact and its lanes agree on near and far), every cluster
+// order torder, so that a wave is compact and its lanes agree on near and far), the clusters
+// [blockIdx.y cchunk, (blockIdx.y + 1) cchunk); with several chunks, chunk y's sums go to
+// pu/pv[y nt + position in torder] for reduce_sorted_kernel, else they are added to ou/ov
 template <bool SRC>
 __global__ __launch_bounds__(kBlock, 5) void panels_clustered_kernel(
     int64_t np, const double* __restrict__ nx, const double* __restrict__ ny,
     const uint32_t* __restrict__ idx, const double* __restrict__ vs, const double* __restrict__ ss,
     int64_t nclus, const double* __restrict__ head, const double* __restrict__ mom, int64_t nt,
     const double* __restrict__ tx, const double* __restrict__ ty,
-    const uint32_t* __restrict__ torder, double* __restrict__ ou, double* __restrict__ ov) {
+    const uint32_t* __restrict__ torder, int64_t cchunk, double* __restrict__ ou,
+    double* __restrict__ ov, int direct) {
   const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t gg = g < nt ? g : nt - 1;
   const int64_t i = torder ? (int64_t)torder[gg] : gg;
   const double px = tx[i], py = ty[i];
   double au = 0.0, av = 0.0;
-  for (int64_t c = 0; c < nclus; ++c) {
+  const int64_t cb = (int64_t)blockIdx.y * cchunk;
+  const int64_t ce = cb + cchunk < nclus ? cb + cchunk : nclus;
+  for (int64_t c = cb; c < ce; ++c) {
     const double* h = head + kHead * c;
     const double zx = px - h[0], zy = py - h[1];
     const double z2 = __builtin_fma(zx, zx, zy * zy);
@@ -232,10 +237,33 @@ __global__ __launch_bounds__(kBlock, 5) void panels_clustered_kernel(
       }
     }
   }
-  if (g < nt) {
+  if (g >= nt) return;
+  if (direct) {
     ou[i] += au;
     ov[i] += av;
+  } else {
+    ou[(int64_t)blockIdx.y * nt + g] = au;
+    ov[(int64_t)blockIdx.y * nt + g] = av;
   }
+}
+
+// ou[torder[g]] += sum over chunks of pu[c nt + g], in chunk order (the same for ov)
+__global__ __launch_bounds__(kBlock) void reduce_sorted_kernel(int64_t nchunk, int64_t nt,
+                                                               const uint32_t* __restrict__ torder,
+                                                               const double* __restrict__ pu,
+                                                               const double* __restrict__ pv,
+                                                               double* __restrict__ ou,
+                                                               double* __restrict__ ov) {
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (g >= nt) return;
+  double su = 0.0, sv = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    su += pu[c * nt + g];
+    sv += pv[c * nt + g];
+  }
+  const int64_t i = torder ? (int64_t)torder[g] : g;
+  ou[i] += su;
+  ov[i] += sv;
 }
 
 __global__ __launch_bounds__(kBlock) void reduce_chunks2_kernel(int64_t nchunk, int64_t n,
@@ -357,15 +385,34 @@ int panels_on_points_dev(Device* d, hipStream_t s, int64_t nnodes, const double*
     O2D_LAUNCH_CHECK();
     const uint32_t* torder = nullptr;
     if (nt > kBlock) O2D_TRY(morton_order16(d, s, nt, tx, ty, &torder));
+    // chunks of clusters for 16 workgroups per CU (C3: 782 target tiles, 6 chunks)
+    const int64_t twgs = (nt + kBlock - 1) / kBlock;
+    const int64_t want = 16 * (int64_t)d->num_cu;
+    int64_t nchunk = twgs >= want ? 1 : std::min<int64_t>((want + twgs - 1) / twgs, nclus);
+    const int64_t cchunk = (nclus + nchunk - 1) / nchunk;
+    nchunk = (nclus + cchunk - 1) / cchunk;
+    double *ou = tu, *ov = tv;
+    const int direct = nchunk == 1;
+    if (!direct) {
+      void* p;
+      O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * nchunk * nt, s, &p));
+      ou = static_cast<double*>(p);
+      ov = ou + nchunk * nt;
+    }
+    const dim3 grid((unsigned)twgs, (unsigned)nchunk);
     KernelTimer kt(O2D_KT_PANELS, s);
     if (ss)
-      panels_clustered_kernel<true><<<grid1(nt), kBlock, 0, s>>>(
-          np, nx, ny, idx, vs, ss, nclus, head, mom, nt, tx, ty, torder, tu, tv);
+      panels_clustered_kernel<true><<<grid, kBlock, 0, s>>>(
+          np, nx, ny, idx, vs, ss, nclus, head, mom, nt, tx, ty, torder, cchunk, ou, ov, direct);
     else
-      panels_clustered_kernel<false><<<grid1(nt), kBlock, 0, s>>>(
-          np, nx, ny, idx, vs, ss, nclus, head, mom, nt, tx, ty, torder, tu, tv);
+      panels_clustered_kernel<false><<<grid, kBlock, 0, s>>>(
+          np, nx, ny, idx, vs, ss, nclus, head, mom, nt, tx, ty, torder, cchunk, ou, ov, direct);
     kt.stop();
     O2D_LAUNCH_CHECK();
+    if (!direct) {
+      reduce_sorted_kernel<<<grid1(nt), kBlock, 0, s>>>(nchunk, nt, torder, ou, ov, tu, tv);
+      O2D_LAUNCH_CHECK();
+    }
     return 0;
   }
   const int64_t twgs = (nt + kBlock - 1) / kBlock;
