@@ -15,6 +15,7 @@
 // two endpoint vectors, which is the same angle already wrapped (one atan2 instead of two).
 // Targets far from a panel use the convergent series of the same function (panel_math.h).
 #include <algorithm>
+#include <cfloat>
 
 #include "nbody.h"
 #include "o2d_math.h"
@@ -104,7 +105,8 @@ constexpr double kFarRatio = 6.0;
 constexpr int kHead = 4;  // per cluster: centre x, y, (kFarRatio R)^2, unused
 
 // head[kHead c ...] and mom[2 kTerms c + 2 (n - 1) + {0, 1}] = A_n of cluster c: one workgroup
-// of 64 threads per cluster, a thread per panel, the moments summed over panels in order
+// of 64 threads per cluster, a thread per panel, the moments summed over panels in order (with
+// mom null, the heads alone)
 __global__ __launch_bounds__(64) void cluster_kernel(int64_t np, const double* __restrict__ nx,
                                                      const double* __restrict__ ny,
                                                      const uint32_t* __restrict__ idx,
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(64) void cluster_kernel(int64_t np, const double* _
   PanelPre q{};
   if (t < nk) {
     const uint32_t a = idx[2 * (k0 + t)], b = idx[2 * (k0 + t) + 1];
-    q = panel_pre(nx[a], ny[a], nx[b], ny[b], vs[k0 + t], ss ? ss[k0 + t] : 0.0);
+    q = panel_pre(nx[a], ny[a], nx[b], ny[b], vs ? vs[k0 + t] : 1.0, ss ? ss[k0 + t] : 0.0);
     ends[t][0] = q.x0;
     ends[t][1] = q.y0;
     ends[t][2] = q.x1;
@@ -153,6 +155,7 @@ __global__ __launch_bounds__(64) void cluster_kernel(int64_t np, const double* _
     head[kHead * c + 3] = 0.0;
   }
   __syncthreads();
+  if (!mom) return;  // the heads alone (points on panels)
   if (t < kClus) {
     if (t < nk) {
       const double ax = q.x0 - cen[0], ay = q.y0 - cen[1];
@@ -292,7 +295,13 @@ __global__ __launch_bounds__(kBlock, 5) void points_on_panels_kernel(
     int64_t ns, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ ss, int64_t np, const double* __restrict__ nx,
     const double* __restrict__ ny, const uint32_t* __restrict__ idx, int64_t chunk,
-    double* __restrict__ part) {
+    double* __restrict__ part, const double* __restrict__ head,
+    const double* __restrict__ gbox) {
+  // with head, only the particles nearer than kFarRatio radii to the centre of the cluster of
+  // this workgroup's panels (rhs_far_kernel takes the others). The particles then come in
+  // Morton order with the box of every 64 (gbox), chunks are whole groups, and a wave skips a
+  // group whose box lies beyond those radii (by a margin far above the rounding of both tests,
+  // so that every particle it skips is one the far kernel takes)
   __shared__ double red[kBlock / 64][2 * kPanTile];
   __shared__ PanelPre pp[kPanTile];
   const int tid = threadIdx.x;
@@ -309,8 +318,26 @@ __global__ __launch_bounds__(kBlock, 5) void points_on_panels_kernel(
   for (int k = 0; k < kPanTile; ++k) ar[k] = ai[k] = 0.0;
   const int64_t jb = (int64_t)blockIdx.y * chunk;
   const int64_t je = jb + chunk < ns ? jb + chunk : ns;
+  static_assert(kClus % kPanTile == 0, "a panel tile lies in one cluster");
+  double hx = 0.0, hy = 0.0, hr2 = 0.0;
+  if (head) {
+    const double* h = head + kHead * (p0 / kClus);
+    hx = h[0];
+    hy = h[1];
+    hr2 = h[2];
+  }
   for (int64_t j = jb + tid; j < je; j += kBlock) {
+    if (head) {
+      const double* b = gbox + 4 * (j / 64);  // the same for the whole wave
+      const double bx = fmax(0.0, fmax(b[0] - hx, hx - b[1]));
+      const double by = fmax(0.0, fmax(b[2] - hy, hy - b[3]));
+      if ((bx * bx + by * by) * (1.0 - 0x1p-40) >= hr2) continue;
+    }
     const double qx = sx[j], qy = sy[j], qs = ss[j];
+    if (head) {
+      const double dx = qx - hx, dy = qy - hy;
+      if (__builtin_fma(dx, dx, dy * dy) >= hr2) continue;
+    }
 #pragma unroll
     for (int k = 0; k < kPanTile; ++k) {
       // read the panel from LDS at each use rather than keeping all eight in registers
@@ -344,6 +371,145 @@ __global__ __launch_bounds__(kBlock, 5) void points_on_panels_kernel(
   }
 }
 
+// the particles in the order sorder (xs, ys, qs) and the box of every 64 of them, gbox[4 g ...] =
+// (x min, x max, y min, y max); one thread per sorted position, a group per wave
+__global__ __launch_bounds__(kBlock) void gather_groups_kernel(int64_t ns,
+                                                               const double* __restrict__ sx,
+                                                               const double* __restrict__ sy,
+                                                               const double* __restrict__ ss,
+                                                               const uint32_t* __restrict__ sorder,
+                                                               double* __restrict__ xs,
+                                                               double* __restrict__ ys,
+                                                               double* __restrict__ qs,
+                                                               double* __restrict__ gbox) {
+  const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+  if (g < ns) {
+    const int64_t j = sorder ? (int64_t)sorder[g] : g;
+    const double x = sx[j], y = sy[j];
+    xs[g] = x;
+    ys[g] = y;
+    qs[g] = ss[j];
+    x0 = x1 = x;
+    y0 = y1 = y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, o, 64));
+    x1 = fmax(x1, __shfl_xor(x1, o, 64));
+    y0 = fmin(y0, __shfl_xor(y0, o, 64));
+    y1 = fmax(y1, __shfl_xor(y1, o, 64));
+  }
+  if (g % 64 == 0 && g < ns) {
+    gbox[4 * (g / 64)] = x0;
+    gbox[4 * (g / 64) + 1] = x1;
+    gbox[4 * (g / 64) + 2] = y0;
+    gbox[4 * (g / 64) + 3] = y1;
+  }
+}
+
+// The far particles of each cluster (points on panels): per cluster c and particle chunk y,
+// G_n = sum_j s_j (z_j - c)^-n over the particles at kFarRatio radii or more, n = 1..kTerms,
+// to gpart[((c nchunk + y) kTerms + n - 1) 2 + {0, 1}]. A panel's sum over those particles is
+// then sum_j s_j L_p(z_j) = sum_n G_n (b_p^n - a_p^n) / n (rhs_combine_kernel), the expansion of
+// the panels-on-points moments read the other way, with the same remainder bound.
+__global__ __launch_bounds__(kBlock) void rhs_far_kernel(int64_t ns, const double* __restrict__ sx,
+                                                         const double* __restrict__ sy,
+                                                         const double* __restrict__ ss,
+                                                         const double* __restrict__ head,
+                                                         int64_t chunk, int64_t nchunk,
+                                                         double* __restrict__ gpart) {
+  __shared__ double red[kBlock / 64][2 * kTerms];
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x;
+  const double hx = head[kHead * c], hy = head[kHead * c + 1], hr2 = head[kHead * c + 2];
+  double gr[kTerms], gi[kTerms];
+#pragma unroll
+  for (int n = 0; n < kTerms; ++n) gr[n] = gi[n] = 0.0;
+  const int64_t jb = (int64_t)blockIdx.y * chunk;
+  const int64_t je = jb + chunk < ns ? jb + chunk : ns;
+  for (int64_t j = jb + tid; j < je; j += kBlock) {
+    const double dx = sx[j] - hx, dy = sy[j] - hy;
+    const double z2 = __builtin_fma(dx, dx, dy * dy);
+    if (z2 >= hr2) {
+      const double iz = rcp64(z2);
+      const double wx = dx * iz, wy = -dy * iz;  // w = 1 / (z - c)
+      double pr = ss[j] * wx, pi = ss[j] * wy;   // s w^n
+#pragma unroll
+      for (int n = 0; n < kTerms; ++n) {
+        gr[n] += pr;
+        gi[n] += pi;
+        const double npr = __builtin_fma(pr, wx, -pi * wy);
+        pi = __builtin_fma(pr, wy, pi * wx);
+        pr = npr;
+      }
+    }
+  }
+  const int wv = tid / 64, ln = tid % 64;
+#pragma unroll
+  for (int n = 0; n < kTerms; ++n) {
+    const double a = wave_sum(gr[n]), b = wave_sum(gi[n]);
+    if (ln == 0) {
+      red[wv][2 * n] = a;
+      red[wv][2 * n + 1] = b;
+    }
+  }
+  __syncthreads();
+  if (tid < 2 * kTerms) {
+    double acc = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) acc += red[w][tid];
+    gpart[((c * nchunk + blockIdx.y) * kTerms) * 2 + tid] = acc;
+  }
+}
+
+// gsum[(c kTerms + n) 2 + {0, 1}] = sum over the chunks of gpart, in chunk order
+__global__ __launch_bounds__(kBlock) void rhs_gsum_kernel(int64_t nclus, int64_t nchunk,
+                                                          const double* __restrict__ gpart,
+                                                          double* __restrict__ gsum) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (c kTerms + n) 2 + part
+  if (e >= nclus * kTerms * 2) return;
+  const int64_t c = e / (2 * kTerms), r = e % (2 * kTerms);
+  double acc = 0.0;
+  for (int64_t y = 0; y < nchunk; ++y) acc += gpart[(c * nchunk + y) * kTerms * 2 + r];
+  gsum[e] = acc;
+}
+
+// the far particles' share of each panel's sum, as one more chunk row of part: row[{0,1} np + p]
+// (the same form as points_on_panels_kernel's rows: Im and Re of conj(e) sum_j s_j L_p(z_j))
+__global__ __launch_bounds__(kBlock) void rhs_combine_kernel(int64_t np,
+                                                             const double* __restrict__ nx,
+                                                             const double* __restrict__ ny,
+                                                             const uint32_t* __restrict__ idx,
+                                                             const double* __restrict__ head,
+                                                             const double* __restrict__ gsum,
+                                                             double* __restrict__ row) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= np) return;
+  const int64_t c = p / kClus;
+  const uint32_t a = idx[2 * p], b = idx[2 * p + 1];
+  const PanelPre q = panel_pre(nx[a], ny[a], nx[b], ny[b], 1.0, 0.0);  // W = conj(e)
+  const double cx = head[kHead * c], cy = head[kHead * c + 1];
+  const double ax = q.x0 - cx, ay = q.y0 - cy, bx = q.x1 - cx, by = q.y1 - cy;
+  const double dzx = q.x1 - q.x0, dzy = q.y1 - q.y0;
+  double tr = 0.0, ti = 0.0;
+  double sx = 1.0, sy = 0.0, px = ax, py = ay;  // S_1, a^1
+  for (int n = 1; n <= kTerms; ++n) {
+    const double gr = gsum[(c * kTerms + n - 1) * 2], gi = gsum[(c * kTerms + n - 1) * 2 + 1];
+    // coefficient (z1 - z0) S_n / n
+    const double inv = 1.0 / (double)n;
+    const double kr = (dzx * sx - dzy * sy) * inv, ki = (dzx * sy + dzy * sx) * inv;
+    tr += kr * gr - ki * gi;
+    ti += kr * gi + ki * gr;
+    const double nsx = bx * sx - by * sy + px, nsy = bx * sy + by * sx + py;  // b S + a^n
+    sx = nsx;
+    sy = nsy;
+    const double npx = px * ax - py * ay;
+    py = px * ay + py * ax;
+    px = npx;
+  }
+  row[p] = __builtin_fma(q.wr, ti, q.wi * tr);
+  row[np + p] = __builtin_fma(q.wr, tr, -q.wi * ti);
+}
+
 // pu[p] -= (1/L_p) * sum over chunks (src/Influence.h:726-759)
 __global__ __launch_bounds__(kBlock) void points_on_panels_finish_kernel(
     int64_t nchunk, int64_t np, const double* __restrict__ nx, const double* __restrict__ ny,
@@ -364,7 +530,8 @@ __global__ __launch_bounds__(kBlock) void points_on_panels_finish_kernel(
 }
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
-constexpr int64_t kClusMin = 4 * kClus;  // panels from which panels on points takes clusters
+constexpr int64_t kClusMin = 4 * kClus;  // panels from which the panel sums take clusters
+constexpr int64_t kClusMinSources = 1024;  // particles from which points on panels takes them
 
 }  // namespace
 
@@ -457,17 +624,72 @@ int points_on_panels_dev(Device* d, hipStream_t s, int64_t ns, const double* sx,
   int64_t nchunk = pwgs >= want ? 1 : (want + pwgs - 1) / pwgs;
   const int64_t maxchunk = (ns + kBlock - 1) / kBlock;  // at least one particle per lane
   if (nchunk > maxchunk) nchunk = maxchunk;
-  const int64_t chunk = (ns + nchunk - 1) / nchunk;
+  int64_t chunk = (ns + nchunk - 1) / nchunk;
+  // clusters when there are enough panels and particles: the near particles of each cluster
+  // (in Morton order, so that a wave's lanes agree) by the direct kernel, the far ones by the
+  // cluster expansion, as one more row of partial sums
+  const bool clus = np >= kClusMin && ns >= kClusMinSources;
+  const int64_t nclus = (np + kClus - 1) / kClus;
+  double* head = nullptr;
+  double* gbox = nullptr;
+  const double *nsx = sx, *nsy = sy, *nss = ss;  // the near kernel's particles
+  if (clus) {
+    void* hp;
+    O2D_TRY(slot(d, kPanClus, sizeof(double) * kHead * nclus, s, &hp));
+    head = static_cast<double*>(hp);
+    cluster_kernel<<<(unsigned)nclus, 64, 0, s>>>(np, nx, ny, idx, nullptr, nullptr, head,
+                                                  nullptr);
+    O2D_LAUNCH_CHECK();
+    const uint32_t* sorder = nullptr;
+    O2D_TRY(morton_order16(d, s, ns, sx, sy, &sorder));
+    const int64_t ngroups = (ns + 63) / 64;
+    void* sp;
+    O2D_TRY(slot(d, kPanClusS, sizeof(double) * (3 * ns + 4 * ngroups), s, &sp));
+    double* xs = static_cast<double*>(sp);
+    gbox = xs + 3 * ns;
+    gather_groups_kernel<<<grid1(ns), kBlock, 0, s>>>(ns, sx, sy, ss, sorder, xs, xs + ns,
+                                                      xs + 2 * ns, gbox);
+    O2D_LAUNCH_CHECK();
+    nsx = xs;
+    nsy = xs + ns;
+    nss = xs + 2 * ns;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;  // whole groups of 64 (and of the block)
+  }
   nchunk = (ns + chunk - 1) / chunk;
+  const int64_t rows = nchunk + (clus ? 1 : 0);
   void* p;
-  O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * nchunk * np, s, &p));
+  O2D_TRY(slot(d, kPartials, sizeof(double) * 2 * rows * np, s, &p));
   double* part = static_cast<double*>(p);
+  // far chunks for 16 workgroups per CU
+  int64_t fchunk = 0, nfchunk = 0;
+  double* gpart = nullptr;
+  if (clus) {
+    const int64_t fwant = 16 * (int64_t)d->num_cu;
+    nfchunk = nclus >= fwant ? 1 : (fwant + nclus - 1) / nclus;
+    if (nfchunk > maxchunk) nfchunk = maxchunk;
+    fchunk = (ns + nfchunk - 1) / nfchunk;
+    nfchunk = (ns + fchunk - 1) / fchunk;
+    void* gp;
+    O2D_TRY(slot(d, kPanClusG, sizeof(double) * 2 * kTerms * nclus * (nfchunk + 1), s, &gp));
+    gpart = static_cast<double*>(gp);
+  }
   KernelTimer kt(O2D_KT_PANEL_RHS, s);
   points_on_panels_kernel<<<dim3((unsigned)pwgs, (unsigned)nchunk), kBlock, 0, s>>>(
-      ns, sx, sy, ss, np, nx, ny, idx, chunk, part);
-  kt.stop();
+      ns, nsx, nsy, nss, np, nx, ny, idx, chunk, part, head, gbox);
   O2D_LAUNCH_CHECK();
-  points_on_panels_finish_kernel<<<grid1(np), kBlock, 0, s>>>(nchunk, np, nx, ny, idx, part, pu, pv);
+  if (clus) {
+    rhs_far_kernel<<<dim3((unsigned)nclus, (unsigned)nfchunk), kBlock, 0, s>>>(
+        ns, sx, sy, ss, head, fchunk, nfchunk, gpart);
+    O2D_LAUNCH_CHECK();
+    double* gsum = gpart + 2 * kTerms * nclus * nfchunk;
+    rhs_gsum_kernel<<<grid1(2 * kTerms * nclus), kBlock, 0, s>>>(nclus, nfchunk, gpart, gsum);
+    O2D_LAUNCH_CHECK();
+    rhs_combine_kernel<<<grid1(np), kBlock, 0, s>>>(np, nx, ny, idx, head, gsum,
+                                                    part + 2 * nchunk * np);
+    O2D_LAUNCH_CHECK();
+  }
+  kt.stop();
+  points_on_panels_finish_kernel<<<grid1(np), kBlock, 0, s>>>(rows, np, nx, ny, idx, part, pu, pv);
   O2D_LAUNCH_CHECK();
   return 0;
 }
