@@ -1,5 +1,6 @@
 // panels.hip - constant-strength straight panels acting on points, and point vortices acting on
-// panels (the BEM right-hand side). fp64 throughout.
+// panels (the BEM right-hand side). fp64 throughout. From 128 panels both sums take far
+// clusters of 32 panels by an expansion about the cluster's centre (below).
 //
 // panels_on_points (src/Influence.h:422-614, kernels src/Kernels.h:314-405): one lane per
 // target point, 256-panel tiles staged in LDS, fp64 sums.
