@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock, 5) void panels_clustered_kernel(
     const double* h = head + kHead * c;
     const double zx = px - h[0], zy = py - h[1];
     const double z2 = __builtin_fma(zx, zx, zy * zy);
-    if (z2 >= h[2]) {
+    if (z2 >= h[2] && z2 > 0.0) {  // far (a point at the centre of a cluster of no extent is near)
       // w = 1 / (z - c); sum_n A_n w^n = w (A_1 + w (A_2 + ... + w A_kTerms))
       const double iz = rcp64(z2);
       const double wx = zx * iz, wy = -zy * iz;
@@ -337,7 +337,8 @@ __global__ __launch_bounds__(kBlock, 5) void points_on_panels_kernel(
     const double qx = sx[j], qy = sy[j], qs = ss[j];
     if (head) {
       const double dx = qx - hx, dy = qy - hy;
-      if (__builtin_fma(dx, dx, dy * dy) >= hr2) continue;
+      const double z2 = __builtin_fma(dx, dx, dy * dy);
+      if (z2 >= hr2 && z2 > 0.0) continue;  // far: rhs_far_kernel's test
     }
 #pragma unroll
     for (int k = 0; k < kPanTile; ++k) {
@@ -393,6 +394,10 @@ __global__ __launch_bounds__(kBlock) void gather_groups_kernel(int64_t ns,
     qs[g] = ss[j];
     x0 = x1 = x;
     y0 = y1 = y;
+    if (!(x == x) || !(y == y)) {  // a NaN position: the group is never skipped, so the NaN
+      x0 = y0 = -INFINITY;           // reaches the sums as in the direct kernel
+      x1 = y1 = INFINITY;
+    }
   }
   for (int o = 32; o > 0; o >>= 1) {
     x0 = fmin(x0, __shfl_xor(x0, o, 64));
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void rhs_far_kernel(int64_t ns, const doubl
   for (int64_t j = jb + tid; j < je; j += kBlock) {
     const double dx = sx[j] - hx, dy = sy[j] - hy;
     const double z2 = __builtin_fma(dx, dx, dy * dy);
-    if (z2 >= hr2) {
+    if (z2 >= hr2 && z2 > 0.0) {  // far: the complement of points_on_panels_kernel's test
       const double iz = rcp64(z2);
       const double wx = dx * iz, wy = -dy * iz;  // w = 1 / (z - c)
       double pr = ss[j] * wx, pi = ss[j] * wy;   // s w^n

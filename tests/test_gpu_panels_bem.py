@@ -39,6 +39,45 @@ def test_points_on_panels(npan, ns):
         assert rel_l2(g, rr) <= 1e-12
 
 
+def test_clustered_panel_sums_over_two_bodies():
+    """From 128 panels the panel sums expand far clusters of 32 panels; here clusters straddle
+    the two bodies (200 + 129 panels), targets crowd both surfaces and one lies on a cluster's
+    end node, and a NaN particle must reach every panel sum as in the per-panel kernels."""
+    ax, ay, ai = circle_panels(200)
+    bx, by, bi = circle_panels(129, diam=0.6, cx=1.6, cy=0.3)
+    nx = np.concatenate([ax, bx]).astype(np.float64)
+    ny = np.concatenate([ay, by]).astype(np.float64)
+    idx = np.concatenate([ai, bi + len(ax)]).astype(np.uint32)
+    npan = len(idx) // 2
+    rng = np.random.default_rng(11)
+    vs, ss = rng.normal(size=npan), rng.normal(size=npan)
+    n = 6000
+    t = rng.uniform(0, 2 * np.pi, n)
+    r = np.where(rng.uniform(size=n) < 0.5, 0.5 * rng.uniform(1.0, 1.3, n), rng.uniform(0.2, 3.0, n))
+    tx, ty = r * np.cos(t), r * np.sin(t)
+    tx[:n // 3] = 1.6 + 0.3 * np.cos(t[:n // 3]) * rng.uniform(1.0, 1.5, n // 3)
+    ty[:n // 3] = 0.3 + 0.3 * np.sin(t[:n // 3]) * rng.uniform(1.0, 1.5, n // 3)
+    tx[5], ty[5] = nx[idx[2 * 40]], ny[idx[2 * 40]]  # on an end node
+    got = o2d.panels_on_points(nx, ny, idx, vs, ss, tx, ty)
+    ref = O.panels_on_points(nx, ny, idx, vs, ss, tx, ty)
+    ok = np.isfinite(ref[0]) & np.isfinite(ref[1])
+    for g, rr in zip(got, ref):
+        assert np.array_equal(np.isfinite(g), np.isfinite(rr))
+        assert rel_l2(g[ok], rr[ok]) <= 1e-12
+    qs = rng.normal(size=n)
+    keep = np.hypot(tx - 0.0, ty) > 0.52  # particles off the first body
+    keep &= np.hypot(tx - 1.6, ty - 0.3) > 0.32
+    sx, sy, sq = tx[keep], ty[keep], qs[keep]
+    got = o2d.points_on_panels(sx, sy, sq, nx, ny, idx)
+    ref = O.points_on_panels(sx, sy, sq, nx, ny, idx)
+    for g, rr in zip(got, ref):
+        assert rel_l2(g, rr) <= 1e-12
+    sx = sx.copy()
+    sx[len(sx) // 2] = np.nan
+    got = o2d.points_on_panels(sx, sy, sq, nx, ny, idx)
+    assert np.all(np.isnan(got[0])) and np.all(np.isnan(got[1]))
+
+
 @pytest.mark.parametrize("nunk", [1, 2])
 @pytest.mark.parametrize("augmented", [False, True])
 @pytest.mark.parametrize("npan", [16, 257])
