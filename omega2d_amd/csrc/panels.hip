@@ -327,7 +327,37 @@ __global__ __launch_bounds__(kBlock, 5) void points_on_panels_kernel(
     hy = h[1];
     hr2 = h[2];
   }
-  for (int64_t j = jb + tid; j < je; j += kBlock) {
+  // with head, the whole chunk is skipped when the box of its groups lies beyond those radii
+  bool skip = false;
+  if (head) {
+    __shared__ double cb[4][kBlock];
+    double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+    for (int64_t q = jb / 64 + tid; q < (je + 63) / 64; q += kBlock) {
+      const double* b = gbox + 4 * q;
+      x0 = fmin(x0, b[0]);
+      x1 = fmax(x1, b[1]);
+      y0 = fmin(y0, b[2]);
+      y1 = fmax(y1, b[3]);
+    }
+    cb[0][tid] = x0;
+    cb[1][tid] = x1;
+    cb[2][tid] = y0;
+    cb[3][tid] = y1;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (tid < w) {
+        cb[0][tid] = fmin(cb[0][tid], cb[0][tid + w]);
+        cb[1][tid] = fmax(cb[1][tid], cb[1][tid + w]);
+        cb[2][tid] = fmin(cb[2][tid], cb[2][tid + w]);
+        cb[3][tid] = fmax(cb[3][tid], cb[3][tid + w]);
+      }
+      __syncthreads();
+    }
+    const double bx = fmax(0.0, fmax(cb[0][0] - hx, hx - cb[1][0]));
+    const double by = fmax(0.0, fmax(cb[2][0] - hy, hy - cb[3][0]));
+    skip = (bx * bx + by * by) * (1.0 - 0x1p-40) >= hr2;
+  }
+  for (int64_t j = jb + tid; j < je && !skip; j += kBlock) {
     if (head) {
       const double* b = gbox + 4 * (j / 64);  // the same for the whole wave
       const double bx = fmax(0.0, fmax(b[0] - hx, hx - b[1]));
