@@ -397,8 +397,9 @@ __device__ __forceinline__ void chunk_panel(const PanelRec& p, double x, double 
 // wave's panels still ascend, as the replay of its lists needs.
 constexpr int kProbeStride = 16;
 
-// box[4 t ...] = (x min, x max, y min, y max) of the end nodes of tile t (non-finite nodes
-// are left out by fmin/fmax: a panel with one never enters a nearest set). One thread per
+// box[4 t ...] = (x min, x max, y min, y max) of the end nodes of tile t. fmin/fmax leave NaN
+// coordinates out, which is safe: the scan's distance to a panel with a NaN node is NaN or
+// the distance to its other node, which the box holds. One thread per
 // panel; a tile is 32 consecutive threads of a wave, reduced with shuffles.
 static_assert(kWaveTile == 32, "a tile is half a wave");
 __global__ __launch_bounds__(kBlock) void tile_box_kernel(int64_t np,
