@@ -9,7 +9,8 @@ C2tree  the 1M blob workload with the treecode (summation_t barneshut): time, eq
 C3  circle of 2048 panels + 200k wake, V2 core: BEM assembly (vortex-only + augmented 2049^2,
     and vortex+source + augmented 4097^2) with HBM GB/s, LU factor + solve, RHS (points on
     panels), panels on particles, particles on particles (split and plain), and the
-    particle/panel proximity scans run after every move (clearing pass, nearest distances)
+    particle/panel proximity scans run after every move (clearing pass, nearest distances),
+    and the Gaussian vorticity of the wake on itself
 C3step  whole RK2 steps of C3 through the C++ host mirror (tests/cpp/host_driver.cpp, built
     by python -m omega2d_amd.build), host arrays over PCIe
 C4  250k particles -> 4096 probes, Gaussian core: forward and adjoint (fp64), ratio
@@ -24,6 +25,7 @@ treewalk  the C2 treecode evaluation (theta 0.5, order 20) twice on the particle
 near      the C3 proximity scans: three clearing passes (method 1) and three nearest-distance
           scans of the 200k wake against the 2,048 panels
 panels    the C3 panel sums: three points-on-panels (RHS) and three panels-on-points launches
+vort      three Gaussian vorticity sums of the C3 wake on itself
 fp64c2    two fp64 evaluations of the C2 particles on themselves (Gaussian core)
 
 Timing: torch.cuda.synchronize() around each measured piece; kernel times from the library's
@@ -197,6 +199,12 @@ def c3():
         ms, _ = kernel_ms(KT_NEAR, fn, reps=3)
         res[name] = {"kernel_ms": round(ms, 4),
                      "equivalent_Gint_s": round(float(nw) * npan / (ms * 1e-3) / 1e9, 2)}
+    # Gaussian vorticity of the wake on itself (src/InfluenceVort.h): the kernel alone, and the
+    # whole call with the Morton orders and the source tiles' boxes
+    ms, _ = kernel_ms(KT_VORT, lambda: dv.points_vorticity(x, y, s, r, x, y), reps=3)
+    t = timed(lambda: dv.points_vorticity(x, y, s, r, x, y), reps=3)
+    res["vorticity"] = {"kernel_ms": round(ms, 4), "call_ms": round(t * 1e3, 4),
+                        "equivalent_Gint_s": round(float(nw) * nw / (ms * 1e-3) / 1e9, 2)}
     us, vs_ = vel(x, y, s, r, x, y, r, "v2", "split")
     u6, v6 = vel(x, y, s, r, x, y, r, "v2", "fp64")
     res["split_err_vs_fp64"] = math.sqrt((rel(us, u6) ** 2 * float((u6 ** 2).sum())
@@ -471,6 +479,14 @@ def panels():
     return res
 
 
+def vort():
+    """Profiler driver: three Gaussian vorticity sums of the C3 wake on itself."""
+    w = wl.c3_circle_wake()
+    x, y, s, r = (T(w[k]) for k in ("x", "y", "s", "r"))
+    ms, _ = kernel_ms(KT_VORT, lambda: dv.points_vorticity(x, y, s, r, x, y), reps=3)
+    return {"config": "vort", "particles": x.numel(), "kernel_ms": round(ms, 4)}
+
+
 def fp64c2():
     """Profiler driver: two fp64 evaluations of the C2 particles on themselves (Gaussian)."""
     w = wl.c2_blobs()
@@ -488,7 +504,7 @@ def main():
     args = ap.parse_args()
     fns = {"c1": c1, "c2host": c2host, "c2tree": c2tree, "c3": c3, "c3step": c3step,
            "c4": c4, "c5": c5, "lu": lu, "assemble": assemble, "treewalk": treewalk,
-           "near": near, "panels": panels, "fp64c2": fp64c2}
+           "near": near, "panels": panels, "vort": vort, "fp64c2": fp64c2}
     out = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()

@@ -33,7 +33,7 @@ enum Slot : int {
   kInv, kSolveTmp,
   kTreeKeys, kTreeOrder, kTreeTOrder, kTreeSrc, kTreeCells, kTreeMom,
   kPan2X, kPan2Y, kPan2Idx, kPanGeom2,
-  kLuPairs, kPanClus, kPanClusG, kPanClusS,
+  kLuPairs, kPanClus, kPanClusG, kPanClusS, kVortSrc,
   kNumSlots
 };
 

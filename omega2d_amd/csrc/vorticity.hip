@@ -4,7 +4,16 @@
 // in fp64. Sources are staged through LDS as (x, y, 1/r^2, 2 s/r^2); each lane owns two
 // targets. A pair with d^2/r^2 > 746 contributes exactly +-0 (exp underflows to 0 in double), so
 // its exponential is skipped without changing the sum. Few targets split the source range into
-// chunks whose fp64 partials are summed in chunk order.
+// chunks whose fp64 partials are summed in chunk order. With many sources and targets both go
+// in Morton order: a tile of 256 sources carries the box of its positions and its smallest
+// 1/r^2, a workgroup's 512 targets are compact, and a tile whose every pair lies beyond the
+// underflow bound (by a margin far above the rounding of both tests) is skipped whole; it adds
+// only +-0 terms. The Morton order changes the order of the fp64 additions, so sums move by
+// rounding only. A NaN position or radius adds NaN, as in the reference (its tile and its
+// targets' group are never skipped); a non-finite strength or a zero radius beyond the bound
+// adds nothing, where the reference's inf*0 gives NaN.
+#include <cfloat>
+
 #include "nbody.h"
 #include "o2d_runtime.h"
 
@@ -22,11 +31,68 @@ constexpr int kBlock = 256;
 constexpr int kTile = 256;
 constexpr int kTpl = 2;
 constexpr double kUnderflow = 746.0;  // exp(-x) == 0 in double for every x above this
+constexpr int64_t kPruneSources = 4096, kPruneTargets = 1024;  // from here the sort pays
 
 struct VSrc {
   double x, y, rr, c;  // position, 1/r^2, 2 s / r^2
 };
 
+// the staged record of a source (as vort_kernel stages it)
+__device__ __forceinline__ VSrc vsrc_of(double x, double y, double s, double r) {
+  const double r2 = r * r;
+  const double rr = 1.0 / r2;
+  return VSrc{x, y, rr, 2.0 * s * rr};
+}
+
+// the sources in the order sorder as staged records, and per tile of kTile the box of their
+// positions and their smallest 1/r^2: tbox[5 t ...] = (x min, x max, y min, y max, rr min).
+// One workgroup per tile. A NaN position or radius makes the tile's box unbounded.
+__global__ __launch_bounds__(kBlock) void vort_gather_kernel(int64_t ns, const double* __restrict__ sx,
+                                                             const double* __restrict__ sy,
+                                                             const double* __restrict__ ss,
+                                                             const double* __restrict__ sr,
+                                                             const uint32_t* __restrict__ sorder,
+                                                             VSrc* __restrict__ vs,
+                                                             double* __restrict__ tbox) {
+  __shared__ double sm[5][kBlock];
+  const int t = threadIdx.x;
+  const int64_t g = (int64_t)blockIdx.x * kBlock + t;
+  double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX, r0 = DBL_MAX;
+  if (g < ns) {
+    const int64_t j = sorder ? (int64_t)sorder[g] : g;
+    const VSrc q = vsrc_of(sx[j], sy[j], ss[j], sr[j]);
+    vs[g] = q;
+    x0 = x1 = q.x;
+    y0 = y1 = q.y;
+    r0 = q.rr;
+    if (!(q.x == q.x) || !(q.y == q.y) || !(q.rr == q.rr)) {
+      x0 = y0 = -INFINITY;
+      x1 = y1 = INFINITY;
+      r0 = 0.0;
+    }
+  }
+  sm[0][t] = x0;
+  sm[1][t] = x1;
+  sm[2][t] = y0;
+  sm[3][t] = y1;
+  sm[4][t] = r0;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      sm[0][t] = fmin(sm[0][t], sm[0][t + w]);
+      sm[1][t] = fmax(sm[1][t], sm[1][t + w]);
+      sm[2][t] = fmin(sm[2][t], sm[2][t + w]);
+      sm[3][t] = fmax(sm[3][t], sm[3][t + w]);
+      sm[4][t] = fmin(sm[4][t], sm[4][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < 5) tbox[5 * blockIdx.x + t] = sm[t][0];
+}
+
+// vs/tbox null: the sources in input order from sx..sr; else the staged records in Morton
+// order with their tile boxes. torder null: the targets in input order; else the target at
+// position p is torder[p] and chunk partials are kept by position (vort_reduce_kernel maps them)
 __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* __restrict__ sx,
                                                       const double* __restrict__ sy,
                                                       const double* __restrict__ ss,
@@ -34,28 +100,68 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
                                                       const double* __restrict__ tx,
                                                       const double* __restrict__ ty,
                                                       int64_t chunk, double* __restrict__ out,
-                                                      int direct) {
+                                                      int direct, const VSrc* __restrict__ vs,
+                                                      const double* __restrict__ tbox,
+                                                      const uint32_t* __restrict__ torder) {
   __shared__ VSrc tile[kTile];
+  __shared__ double wb[4][kBlock];
   const int64_t j0 = (int64_t)blockIdx.y * chunk;
   const int64_t j1 = (j0 + chunk < ns) ? j0 + chunk : ns;
   double px[kTpl], py[kTpl], acc[kTpl];
   int64_t ti[kTpl];
+  double bx0 = DBL_MAX, bx1 = -DBL_MAX, by0 = DBL_MAX, by1 = -DBL_MAX;
 #pragma unroll
   for (int t = 0; t < kTpl; ++t) {
     ti[t] = ((int64_t)blockIdx.x * kTpl + t) * kBlock + threadIdx.x;
     const bool ok = ti[t] < nt;
-    px[t] = ok ? tx[ti[t]] : 0.0;
-    py[t] = ok ? ty[ti[t]] : 0.0;
+    const int64_t i = ok ? (torder ? (int64_t)torder[ti[t]] : ti[t]) : 0;
+    px[t] = ok ? tx[i] : 0.0;
+    py[t] = ok ? ty[i] : 0.0;
     acc[t] = 0.0;
+    if (ok) {
+      bx0 = fmin(bx0, px[t]);
+      bx1 = fmax(bx1, px[t]);
+      by0 = fmin(by0, py[t]);
+      by1 = fmax(by1, py[t]);
+      if (!(px[t] == px[t]) || !(py[t] == py[t])) {
+        bx0 = by0 = -INFINITY;
+        bx1 = by1 = INFINITY;
+      }
+    }
+  }
+  if (tbox) {  // the box of the workgroup's targets
+    const int t = threadIdx.x;
+    wb[0][t] = bx0;
+    wb[1][t] = bx1;
+    wb[2][t] = by0;
+    wb[3][t] = by1;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (t < w) {
+        wb[0][t] = fmin(wb[0][t], wb[0][t + w]);
+        wb[1][t] = fmax(wb[1][t], wb[1][t + w]);
+        wb[2][t] = fmin(wb[2][t], wb[2][t + w]);
+        wb[3][t] = fmax(wb[3][t], wb[3][t + w]);
+      }
+      __syncthreads();
+    }
+    bx0 = wb[0][0];
+    bx1 = wb[1][0];
+    by0 = wb[2][0];
+    by1 = wb[3][0];
   }
   for (int64_t base = j0; base < j1; base += kTile) {
     const int n = (int)((j1 - base) < kTile ? (j1 - base) : kTile);
+    if (tbox) {  // the same for the whole workgroup: tiles are whole (chunks are whole tiles)
+      const double* b = tbox + 5 * (base / kTile);
+      const double dx = fmax(0.0, fmax(b[0] - bx1, bx0 - b[1]));
+      const double dy = fmax(0.0, fmax(b[2] - by1, by0 - b[3]));
+      if ((dx * dx + dy * dy) * b[4] * (1.0 - 0x1p-40) > kUnderflow) continue;
+    }
     __syncthreads();
     if ((int)threadIdx.x < n) {
       const int64_t j = base + threadIdx.x;
-      const double r2 = sr[j] * sr[j];
-      const double rr = 1.0 / r2;
-      tile[threadIdx.x] = VSrc{sx[j], sy[j], rr, 2.0 * ss[j] * rr};
+      tile[threadIdx.x] = vs ? vs[j] : vsrc_of(sx[j], sy[j], ss[j], sr[j]);
     }
     __syncthreads();
     for (int k = 0; k < n; ++k) {
@@ -64,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
       for (int t = 0; t < kTpl; ++t) {
         const double dx = px[t] - q.x, dy = py[t] - q.y;
         const double x = (dx * dx + dy * dy) * q.rr;
-        if (x < kUnderflow) acc[t] += q.c * exp(-x);
+        if (!(x >= kUnderflow)) acc[t] += q.c * exp(-x);  // a NaN exponent still adds NaN
       }
     }
   }
@@ -72,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
   for (int t = 0; t < kTpl; ++t) {
     if (ti[t] >= nt) continue;
     if (direct)
-      out[ti[t]] += acc[t];
+      out[torder ? (int64_t)torder[ti[t]] : ti[t]] += acc[t];
     else
       out[(int64_t)blockIdx.y * nt + ti[t]] = acc[t];
   }
@@ -80,12 +186,13 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
 
 __global__ __launch_bounds__(kBlock) void vort_reduce_kernel(int64_t nchunk, int64_t nt,
                                                              const double* __restrict__ part,
+                                                             const uint32_t* __restrict__ torder,
                                                              double* __restrict__ tw) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= nt) return;
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nt) return;
   double s = 0.0;
-  for (int64_t c = 0; c < nchunk; ++c) s += part[c * nt + i];
-  tw[i] += s;
+  for (int64_t c = 0; c < nchunk; ++c) s += part[c * nt + p];
+  tw[torder ? (int64_t)torder[p] : p] += s;
 }
 
 }  // namespace
@@ -110,16 +217,31 @@ int vorticity_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const 
     O2D_TRY(slot(d, kPartials, sizeof(double) * nchunk * nt, s, &p));
     out = static_cast<double*>(p);
   }
+  // with many sources and targets, both in Morton order and source tiles skipped whole
+  VSrc* vs = nullptr;
+  double* tbox = nullptr;
+  const uint32_t* torder = nullptr;
+  if (ns >= kPruneSources && nt >= kPruneTargets) {
+    const uint32_t* sorder = nullptr;
+    O2D_TRY(morton_order16(d, s, ns, sx, sy, &sorder));
+    void* vp;
+    O2D_TRY(slot(d, kVortSrc, sizeof(VSrc) * ns + sizeof(double) * 5 * ntiles, s, &vp));
+    vs = static_cast<VSrc*>(vp);
+    tbox = reinterpret_cast<double*>(vs + ns);
+    vort_gather_kernel<<<(unsigned)ntiles, kBlock, 0, s>>>(ns, sx, sy, ss, sr, sorder, vs, tbox);
+    O2D_LAUNCH_CHECK();
+    O2D_TRY(morton_order16(d, s, nt, tx, ty, &torder));  // reuses the order slot: vs holds the sources
+  }
   {
     KernelTimer kt(O2D_KT_VORT, s);
     vort_kernel<<<dim3((unsigned)twgs, (unsigned)nchunk), kBlock, 0, s>>>(
-        ns, sx, sy, ss, sr, nt, tx, ty, chunk, out, direct);
+        ns, sx, sy, ss, sr, nt, tx, ty, chunk, out, direct, vs, tbox, torder);
     kt.stop();
     O2D_LAUNCH_CHECK();
   }
   if (!direct) {
     vort_reduce_kernel<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(nchunk, nt, out,
-                                                                                 tw);
+                                                                                 torder, tw);
     O2D_LAUNCH_CHECK();
   }
   return 0;

@@ -50,3 +50,52 @@ def test_vorticity_accumulates_and_device_entry():
     assert np.array_equal(tw.cpu().numpy(), o2d.points_vorticity(sx, sy, ss, sr, tx, ty))
     empty = np.zeros(0)
     assert np.all(o2d.points_vorticity(empty, empty, empty, empty, tx, ty) == 0.0)
+
+
+def _wake(rng, n, r):
+    """particles along a few thin sheets, as a wake sheds them, with small blobs"""
+    k = rng.integers(0, 4, n)
+    t = rng.uniform(0, 6, n)
+    x = t + rng.normal(0, 0.01, n)
+    y = 0.3 * k + 0.05 * np.sin(3 * t) + rng.normal(0, 0.01, n)
+    return x, y, rng.normal(0, 1e-3, n), np.full(n, r)
+
+
+def test_vorticity_pruned_tiles_per_target():
+    """Sources and targets in Morton order with whole source tiles skipped beyond the underflow
+    distance: every target within 1e-12 of the oracle's sum scaled by its largest term, a
+    target group far from every source exactly 0, and the result independent of the input
+    order of sources and targets up to rounding."""
+    rng = np.random.default_rng(11)
+    ns, nt = 24000, 5000
+    sx, sy, ss, sr = _wake(rng, ns, 0.004)
+    tx, ty = np.concatenate([sx[:nt - 600], rng.uniform(20, 21, 600)]), \
+        np.concatenate([sy[:nt - 600], rng.uniform(20, 21, 600)])
+    got = o2d.points_vorticity(sx, sy, ss, sr, tx, ty)
+    ref = O.points_vorticity(sx, sy, ss, sr, tx, ty)
+    scale = 2.0 * np.max(np.abs(ss)) / 0.004 ** 2
+    assert np.max(np.abs(got - ref)) <= 1e-12 * scale
+    assert np.all(got[nt - 600:] == 0.0)
+    ps, pt = rng.permutation(ns), rng.permutation(nt)
+    got2 = o2d.points_vorticity(sx[ps], sy[ps], ss[ps], sr[ps], tx[pt], ty[pt])
+    assert np.max(np.abs(got2 - got[pt])) <= 1e-13 * scale
+
+
+def test_vorticity_pruned_nan_positions():
+    """A NaN target position gives NaN at that target only; a NaN source position or radius
+    gives NaN at every target, as the reference's exp(NaN) does; the pruned path keeps both."""
+    rng = np.random.default_rng(12)
+    ns, nt = 9000, 3000
+    sx, sy, ss, sr = _wake(rng, ns, 0.004)
+    tx, ty = sx[:nt].copy(), sy[:nt].copy()
+    ty[1234] = np.nan
+    got = o2d.points_vorticity(sx, sy, ss, sr, tx, ty)
+    ref = O.points_vorticity(sx, sy, ss, sr, tx, ty)
+    assert np.array_equal(np.isnan(got), np.isnan(ref)) and np.sum(np.isnan(got)) == 1
+    ok = ~np.isnan(ref)
+    assert np.max(np.abs(got[ok] - ref[ok])) <= 1e-12 * 2.0 * np.max(np.abs(ss)) / 0.004 ** 2
+    for arr in (sx, sr):
+        bad = arr.copy()
+        bad[777] = np.nan
+        a = (bad, sy, ss, sr) if arr is sx else (sx, sy, ss, bad)
+        assert np.all(np.isnan(o2d.points_vorticity(*a, tx, ty)))
