@@ -32,6 +32,11 @@ constexpr int kTile = 256;
 constexpr int kTpl = 2;
 constexpr double kUnderflow = 746.0;  // exp(-x) == 0 in double for every x above this
 constexpr int64_t kPruneSources = 4096, kPruneTargets = 1024;  // from here the sort pays
+// pruned launch: source tiles per chunk, and the cap on the chunk partials' bytes
+#ifndef O2D_VORT_PRUNED_TILES
+#define O2D_VORT_PRUNED_TILES 2
+#endif
+constexpr int64_t kPartialBytes = int64_t(1) << 30;
 
 struct VSrc {
   double x, y, rr, c;  // position, 1/r^2, 2 s / r^2
@@ -90,9 +95,62 @@ __global__ __launch_bounds__(kBlock) void vort_gather_kernel(int64_t ns, const d
   if (t < 5) tbox[5 * blockIdx.x + t] = sm[t][0];
 }
 
+// the box of each kBlock consecutive targets in the order torder (the first or the second
+// targets of one vort_kernel workgroup's lanes): gbox[4 g ...] = (x min, x max, y min, y max).
+// One workgroup per group; a NaN position makes the group's box unbounded.
+__global__ __launch_bounds__(kBlock) void vort_target_box_kernel(int64_t nt,
+                                                                 const double* __restrict__ tx,
+                                                                 const double* __restrict__ ty,
+                                                                 const uint32_t* __restrict__ torder,
+                                                                 double* __restrict__ gbox) {
+  __shared__ double sm[4][kBlock];
+  const int t = threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * kBlock + t;
+  double x0 = DBL_MAX, x1 = -DBL_MAX, y0 = DBL_MAX, y1 = -DBL_MAX;
+  if (p < nt) {
+    const int64_t i = torder[p];
+    x0 = x1 = tx[i];
+    y0 = y1 = ty[i];
+    if (!(x0 == x0) || !(y0 == y0)) {
+      x0 = y0 = -INFINITY;
+      x1 = y1 = INFINITY;
+    }
+  }
+  sm[0][t] = x0;
+  sm[1][t] = x1;
+  sm[2][t] = y0;
+  sm[3][t] = y1;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      sm[0][t] = fmin(sm[0][t], sm[0][t + w]);
+      sm[1][t] = fmax(sm[1][t], sm[1][t + w]);
+      sm[2][t] = fmin(sm[2][t], sm[2][t + w]);
+      sm[3][t] = fmax(sm[3][t], sm[3][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < 4) gbox[4 * blockIdx.x + t] = sm[t][0];
+}
+
+// true unless every pair of source tile box b (with its smallest 1/r^2) and target group box g
+// lies beyond the underflow bound, by a relative margin of 2^-40
+__device__ __forceinline__ bool tile_near(const double* b, const double* g) {
+  const double dx = fmax(0.0, fmax(b[0] - g[1], g[0] - b[1]));
+  const double dy = fmax(0.0, fmax(b[2] - g[3], g[2] - b[3]));
+  return !((dx * dx + dy * dy) * b[4] * (1.0 - 0x1p-40) > kUnderflow);
+}
+
 // vs/tbox null: the sources in input order from sx..sr; else the staged records in Morton
-// order with their tile boxes. torder null: the targets in input order; else the target at
-// position p is torder[p] and chunk partials are kept by position (vort_reduce_kernel maps them)
+// order with their tile boxes, gbox the boxes of the target groups of kBlock (a workgroup holds
+// two: its lanes' first and second targets) and torder the targets' order: the target at
+// position p is torder[p] and chunk partials are kept by position (vort_reduce_kernel maps
+// them). A tile is skipped for a group it cannot reach, for both groups at once when neither,
+// and a workgroup whose chunk reaches neither group writes no partials and ends. The grid is one
+// dimensional: block b takes (target workgroup, chunk) number (b * perm) % (twgs * nchunk), row
+// major by chunk, so that the few (workgroup, chunk) pairs near the Morton diagonal that hold
+// work are spread over the whole dispatch. flag[chunk * twgs + workgroup] says whether its
+// partials were written.
 __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* __restrict__ sx,
                                                       const double* __restrict__ sy,
                                                       const double* __restrict__ ss,
@@ -100,63 +158,53 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
                                                       const double* __restrict__ tx,
                                                       const double* __restrict__ ty,
                                                       int64_t chunk, double* __restrict__ out,
-                                                      int direct, const VSrc* __restrict__ vs,
+                                                      uint8_t* __restrict__ flag, int64_t twgs,
+                                                      int64_t perm, int direct,
+                                                      const VSrc* __restrict__ vs,
                                                       const double* __restrict__ tbox,
+                                                      const double* __restrict__ gbox,
                                                       const uint32_t* __restrict__ torder) {
+  static_assert(kTpl == 2, "a workgroup holds two target groups");
   __shared__ VSrc tile[kTile];
-  __shared__ double wb[4][kBlock];
-  const int64_t j0 = (int64_t)blockIdx.y * chunk;
+  const int64_t cell = ((int64_t)blockIdx.x * perm) % ((int64_t)gridDim.x);
+  const int64_t bx = cell % twgs, by = cell / twgs;
+  const int64_t j0 = by * chunk;
   const int64_t j1 = (j0 + chunk < ns) ? j0 + chunk : ns;
   double px[kTpl], py[kTpl], acc[kTpl];
   int64_t ti[kTpl];
-  double bx0 = DBL_MAX, bx1 = -DBL_MAX, by0 = DBL_MAX, by1 = -DBL_MAX;
+#pragma unroll
+  for (int t = 0; t < kTpl; ++t) ti[t] = (bx * kTpl + t) * kBlock + threadIdx.x;
+  // the workgroup's two target groups (the second may hold no target)
+  const double* g0 = gbox ? gbox + 4 * (2 * bx) : nullptr;
+  const double* g1 = (gbox && (2 * bx + 1) * kBlock < nt) ? g0 + 4 : nullptr;
+  if (tbox) {  // uniform over the workgroup: chunks are whole tiles
+    bool any = false;
+    for (int64_t base = j0; base < j1 && !any; base += kTile) {
+      const double* b = tbox + 5 * (base / kTile);
+      any = tile_near(b, g0) || (g1 && tile_near(b, g1));
+    }
+    if (!any) {
+      if (!direct && threadIdx.x == 0) flag[by * twgs + bx] = 0;
+      return;
+    }
+  }
+  if (!direct && threadIdx.x == 0) flag[by * twgs + bx] = 1;
 #pragma unroll
   for (int t = 0; t < kTpl; ++t) {
-    ti[t] = ((int64_t)blockIdx.x * kTpl + t) * kBlock + threadIdx.x;
     const bool ok = ti[t] < nt;
     const int64_t i = ok ? (torder ? (int64_t)torder[ti[t]] : ti[t]) : 0;
     px[t] = ok ? tx[i] : 0.0;
     py[t] = ok ? ty[i] : 0.0;
     acc[t] = 0.0;
-    if (ok) {
-      bx0 = fmin(bx0, px[t]);
-      bx1 = fmax(bx1, px[t]);
-      by0 = fmin(by0, py[t]);
-      by1 = fmax(by1, py[t]);
-      if (!(px[t] == px[t]) || !(py[t] == py[t])) {
-        bx0 = by0 = -INFINITY;
-        bx1 = by1 = INFINITY;
-      }
-    }
-  }
-  if (tbox) {  // the box of the workgroup's targets
-    const int t = threadIdx.x;
-    wb[0][t] = bx0;
-    wb[1][t] = bx1;
-    wb[2][t] = by0;
-    wb[3][t] = by1;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-      if (t < w) {
-        wb[0][t] = fmin(wb[0][t], wb[0][t + w]);
-        wb[1][t] = fmax(wb[1][t], wb[1][t + w]);
-        wb[2][t] = fmin(wb[2][t], wb[2][t + w]);
-        wb[3][t] = fmax(wb[3][t], wb[3][t + w]);
-      }
-      __syncthreads();
-    }
-    bx0 = wb[0][0];
-    bx1 = wb[1][0];
-    by0 = wb[2][0];
-    by1 = wb[3][0];
   }
   for (int64_t base = j0; base < j1; base += kTile) {
     const int n = (int)((j1 - base) < kTile ? (j1 - base) : kTile);
-    if (tbox) {  // the same for the whole workgroup: tiles are whole (chunks are whole tiles)
+    bool k0 = true, k1 = true;
+    if (tbox) {
       const double* b = tbox + 5 * (base / kTile);
-      const double dx = fmax(0.0, fmax(b[0] - bx1, bx0 - b[1]));
-      const double dy = fmax(0.0, fmax(b[2] - by1, by0 - b[3]));
-      if ((dx * dx + dy * dy) * b[4] * (1.0 - 0x1p-40) > kUnderflow) continue;
+      k0 = tile_near(b, g0);
+      k1 = g1 && tile_near(b, g1);
+      if (!k0 && !k1) continue;
     }
     __syncthreads();
     if ((int)threadIdx.x < n) {
@@ -164,14 +212,29 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
       tile[threadIdx.x] = vs ? vs[j] : vsrc_of(sx[j], sy[j], ss[j], sr[j]);
     }
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-      const VSrc q = tile[k];
+    if (k0 && k1) {
+      for (int k = 0; k < n; ++k) {
+        const VSrc q = tile[k];
 #pragma unroll
-      for (int t = 0; t < kTpl; ++t) {
-        const double dx = px[t] - q.x, dy = py[t] - q.y;
-        const double x = (dx * dx + dy * dy) * q.rr;
-        if (!(x >= kUnderflow)) acc[t] += q.c * exp(-x);  // a NaN exponent still adds NaN
+        for (int t = 0; t < kTpl; ++t) {
+          const double dx = px[t] - q.x, dy = py[t] - q.y;
+          const double x = (dx * dx + dy * dy) * q.rr;
+          if (!(x >= kUnderflow)) acc[t] += q.c * exp(-x);  // a NaN exponent still adds NaN
+        }
       }
+    } else {  // one group: the other's pairs with this tile all add nothing
+      const double qx = k0 ? px[0] : px[1], qy = k0 ? py[0] : py[1];
+      double a = k0 ? acc[0] : acc[1];
+      for (int k = 0; k < n; ++k) {
+        const VSrc q = tile[k];
+        const double dx = qx - q.x, dy = qy - q.y;
+        const double x = (dx * dx + dy * dy) * q.rr;
+        if (!(x >= kUnderflow)) a += q.c * exp(-x);
+      }
+      if (k0)
+        acc[0] = a;
+      else
+        acc[1] = a;
     }
   }
 #pragma unroll
@@ -180,18 +243,22 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
     if (direct)
       out[torder ? (int64_t)torder[ti[t]] : ti[t]] += acc[t];
     else
-      out[(int64_t)blockIdx.y * nt + ti[t]] = acc[t];
+      out[by * nt + ti[t]] = acc[t];
   }
 }
 
 __global__ __launch_bounds__(kBlock) void vort_reduce_kernel(int64_t nchunk, int64_t nt,
                                                              const double* __restrict__ part,
+                                                             const uint8_t* __restrict__ flag,
+                                                             int64_t twgs,
                                                              const uint32_t* __restrict__ torder,
                                                              double* __restrict__ tw) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= nt) return;
+  const int64_t wg = p / (kBlock * kTpl);
   double s = 0.0;
-  for (int64_t c = 0; c < nchunk; ++c) s += part[c * nt + p];
+  for (int64_t c = 0; c < nchunk; ++c)
+    if (flag[c * twgs + wg]) s += part[c * nt + p];
   tw[torder ? (int64_t)torder[p] : p] += s;
 }
 
@@ -203,45 +270,78 @@ int vorticity_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const 
   if (ns == 0 || nt == 0) return 0;
   const int64_t twgs = (nt + kBlock * kTpl - 1) / (kBlock * kTpl);
   const int64_t ntiles = (ns + kTile - 1) / kTile;
-  const int64_t want = O2D_WANT_VORT * (int64_t)d->num_cu;
-  int64_t nchunk = twgs >= want ? 1 : (want + twgs - 1) / twgs;
-  if (nchunk > ntiles) nchunk = ntiles;
-  if (nchunk > 65535) nchunk = 65535;
-  const int64_t per = (ntiles + nchunk - 1) / nchunk;
+  // with many sources and targets, both in Morton order and source tiles skipped whole
+  const bool prune = ns >= kPruneSources && nt >= kPruneTargets;
+  int64_t nchunk;
+  if (prune) {  // short chunks: the work sits in few (workgroup, chunk) pairs
+    nchunk = (ntiles + O2D_VORT_PRUNED_TILES - 1) / O2D_VORT_PRUNED_TILES;
+    const int64_t cap = kPartialBytes / ((int64_t)sizeof(double) * nt);
+    if (nchunk > cap) nchunk = cap > 1 ? cap : 1;
+  } else {  // enough workgroups to keep every CU busy
+    const int64_t want = O2D_WANT_VORT * (int64_t)d->num_cu;
+    nchunk = twgs >= want ? 1 : (want + twgs - 1) / twgs;
+    if (nchunk > ntiles) nchunk = ntiles;
+  }
+  int64_t per = (ntiles + nchunk - 1) / nchunk;
+  while (per < ntiles && twgs * ((ntiles + per - 1) / per) > INT32_MAX) per *= 2;  // 1-D grid
   nchunk = (ntiles + per - 1) / per;
   const int64_t chunk = per * kTile;
+  const int64_t cells = twgs * nchunk;
+  // a multiplier coprime to the cell count spreads the cells over the dispatch order
+  int64_t perm = 1;
+  if (prune && cells > 1) {
+    perm = (int64_t)(0.6180339887 * (double)cells) | 1;
+    auto gcd = [](int64_t a, int64_t b) {
+      while (b) {
+        const int64_t t = a % b;
+        a = b;
+        b = t;
+      }
+      return a;
+    };
+    while (gcd(perm, cells) != 1) perm += 2;
+    perm %= cells;
+  }
   double* out = tw;
+  uint8_t* flag = nullptr;
   const int direct = nchunk == 1;
   if (!direct) {
     void* p;
-    O2D_TRY(slot(d, kPartials, sizeof(double) * nchunk * nt, s, &p));
+    O2D_TRY(slot(d, kPartials, sizeof(double) * nchunk * nt + cells, s, &p));
     out = static_cast<double*>(p);
+    flag = reinterpret_cast<uint8_t*>(out + nchunk * nt);
   }
-  // with many sources and targets, both in Morton order and source tiles skipped whole
   VSrc* vs = nullptr;
   double* tbox = nullptr;
+  double* gbox = nullptr;
   const uint32_t* torder = nullptr;
-  if (ns >= kPruneSources && nt >= kPruneTargets) {
+  if (prune) {
     const uint32_t* sorder = nullptr;
     O2D_TRY(morton_order16(d, s, ns, sx, sy, &sorder));
     void* vp;
-    O2D_TRY(slot(d, kVortSrc, sizeof(VSrc) * ns + sizeof(double) * 5 * ntiles, s, &vp));
+    const int64_t ngrp = (nt + kBlock - 1) / kBlock;
+    O2D_TRY(slot(d, kVortSrc, sizeof(VSrc) * ns + sizeof(double) * (5 * ntiles + 4 * ngrp), s,
+                 &vp));
     vs = static_cast<VSrc*>(vp);
     tbox = reinterpret_cast<double*>(vs + ns);
+    gbox = tbox + 5 * ntiles;
     vort_gather_kernel<<<(unsigned)ntiles, kBlock, 0, s>>>(ns, sx, sy, ss, sr, sorder, vs, tbox);
     O2D_LAUNCH_CHECK();
     O2D_TRY(morton_order16(d, s, nt, tx, ty, &torder));  // reuses the order slot: vs holds the sources
+    vort_target_box_kernel<<<(unsigned)ngrp, kBlock, 0, s>>>(nt, tx, ty, torder, gbox);
+    O2D_LAUNCH_CHECK();
   }
   {
     KernelTimer kt(O2D_KT_VORT, s);
-    vort_kernel<<<dim3((unsigned)twgs, (unsigned)nchunk), kBlock, 0, s>>>(
-        ns, sx, sy, ss, sr, nt, tx, ty, chunk, out, direct, vs, tbox, torder);
+    vort_kernel<<<(unsigned)cells, kBlock, 0, s>>>(ns, sx, sy, ss, sr, nt, tx, ty, chunk, out,
+                                                   flag, twgs, perm, direct, vs, tbox, gbox,
+                                                   torder);
     kt.stop();
     O2D_LAUNCH_CHECK();
   }
   if (!direct) {
-    vort_reduce_kernel<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(nchunk, nt, out,
-                                                                                 torder, tw);
+    vort_reduce_kernel<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        nchunk, nt, out, flag, twgs, torder, tw);
     O2D_LAUNCH_CHECK();
   }
   return 0;

@@ -11,7 +11,7 @@ from helpers import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("ns,nt", [(1, 1), (257, 513), (3000, 7), (20000, 3001)])
+@pytest.mark.parametrize("ns,nt", [(1, 1), (257, 513), (3000, 7), (9000, 1100), (20000, 3001)])
 def test_vorticity_matches_oracle(ns, nt):
     rng = np.random.default_rng(ns + nt)
     sx, sy = rng.uniform(-1, 1, ns), rng.uniform(-1, 1, ns)
