@@ -149,8 +149,8 @@ __device__ __forceinline__ bool tile_near(const double* b, const double* g) {
 // and a workgroup whose chunk reaches neither group writes no partials and ends. The grid is one
 // dimensional: block b takes (target workgroup, chunk) number (b * perm) % (twgs * nchunk), row
 // major by chunk, so that the few (workgroup, chunk) pairs near the Morton diagonal that hold
-// work are spread over the whole dispatch. flag[chunk * twgs + workgroup] says whether its
-// partials were written.
+// work are spread over the whole dispatch. flag[workgroup * nchunk8 + chunk] says whether its
+// partials were written (nchunk8: the chunk count rounded up to 8; the padding is never read).
 __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* __restrict__ sx,
                                                       const double* __restrict__ sy,
                                                       const double* __restrict__ ss,
@@ -159,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
                                                       const double* __restrict__ ty,
                                                       int64_t chunk, double* __restrict__ out,
                                                       uint8_t* __restrict__ flag, int64_t twgs,
+                                                      int64_t nchunk8,
                                                       int64_t perm, int direct,
                                                       const VSrc* __restrict__ vs,
                                                       const double* __restrict__ tbox,
@@ -184,11 +185,11 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
       any = tile_near(b, g0) || (g1 && tile_near(b, g1));
     }
     if (!any) {
-      if (!direct && threadIdx.x == 0) flag[by * twgs + bx] = 0;
+      if (!direct && threadIdx.x == 0) flag[bx * nchunk8 + by] = 0;
       return;
     }
   }
-  if (!direct && threadIdx.x == 0) flag[by * twgs + bx] = 1;
+  if (!direct && threadIdx.x == 0) flag[bx * nchunk8 + by] = 1;
 #pragma unroll
   for (int t = 0; t < kTpl; ++t) {
     const bool ok = ti[t] < nt;
@@ -250,15 +251,24 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
 __global__ __launch_bounds__(kBlock) void vort_reduce_kernel(int64_t nchunk, int64_t nt,
                                                              const double* __restrict__ part,
                                                              const uint8_t* __restrict__ flag,
-                                                             int64_t twgs,
+                                                             int64_t nchunk8,
                                                              const uint32_t* __restrict__ torder,
                                                              double* __restrict__ tw) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= nt) return;
   const int64_t wg = p / (kBlock * kTpl);
+  // the workgroup's flags eight at a time (the row starts 8-byte aligned)
+  const uint64_t* fw = reinterpret_cast<const uint64_t*>(flag + wg * nchunk8);
   double s = 0.0;
-  for (int64_t c = 0; c < nchunk; ++c)
-    if (flag[c * twgs + wg]) s += part[c * nt + p];
+  for (int64_t w = 0; w < nchunk8 / 8; ++w) {
+    const uint64_t f = fw[w];
+    if (!f) continue;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int64_t c = 8 * w + b;
+      if (c < nchunk && ((f >> (8 * b)) & 0xff)) s += part[c * nt + p];
+    }
+  }
   tw[torder ? (int64_t)torder[p] : p] += s;
 }
 
@@ -305,9 +315,10 @@ int vorticity_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const 
   double* out = tw;
   uint8_t* flag = nullptr;
   const int direct = nchunk == 1;
+  const int64_t nchunk8 = (nchunk + 7) & ~int64_t(7);
   if (!direct) {
     void* p;
-    O2D_TRY(slot(d, kPartials, sizeof(double) * nchunk * nt + cells, s, &p));
+    O2D_TRY(slot(d, kPartials, sizeof(double) * nchunk * nt + twgs * nchunk8, s, &p));
     out = static_cast<double*>(p);
     flag = reinterpret_cast<uint8_t*>(out + nchunk * nt);
   }
@@ -334,14 +345,14 @@ int vorticity_dev(Device* d, hipStream_t s, int64_t ns, const double* sx, const 
   {
     KernelTimer kt(O2D_KT_VORT, s);
     vort_kernel<<<(unsigned)cells, kBlock, 0, s>>>(ns, sx, sy, ss, sr, nt, tx, ty, chunk, out,
-                                                   flag, twgs, perm, direct, vs, tbox, gbox,
-                                                   torder);
+                                                   flag, twgs, nchunk8, perm, direct, vs, tbox,
+                                                   gbox, torder);
     kt.stop();
     O2D_LAUNCH_CHECK();
   }
   if (!direct) {
     vort_reduce_kernel<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        nchunk, nt, out, flag, twgs, torder, tw);
+        nchunk, nt, out, flag, nchunk8, torder, tw);
     O2D_LAUNCH_CHECK();
   }
   return 0;
