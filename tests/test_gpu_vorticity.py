@@ -99,3 +99,20 @@ def test_vorticity_pruned_nan_positions():
         bad[777] = np.nan
         a = (bad, sy, ss, sr) if arr is sx else (sx, sy, ss, bad)
         assert np.all(np.isnan(o2d.points_vorticity(*a, tx, ty)))
+
+
+def test_vorticity_exponential_single_pair():
+    """One source (s = 1/2, r = 1) on targets at d^2 = x over [0, 746): each sum is the one
+    term exp(-x), against numpy's exp within 3 ulp (within 3 units of 2^-1074 where the value
+    is subnormal), and exactly 0 at and beyond the underflow bound."""
+    x = np.concatenate([np.linspace(0.0, 745.99, 200000), [745.999999, 746.0, 800.0]])
+    tx = np.sqrt(x)
+    d2 = tx * tx
+    got = o2d.points_vorticity([0.0], [0.0], [0.5], [1.0], tx, np.zeros_like(tx))
+    ref = np.exp(-d2)
+    ulp = np.spacing(ref)
+    err = np.abs(got - ref)
+    normal = ref >= np.finfo(np.float64).tiny
+    assert np.all(err[normal] <= 3.0 * ulp[normal])
+    assert np.all(err[~normal] <= 3.0 * 2.0 ** -1074)
+    assert np.all(got[d2 >= 746.0] == 0.0) and got[0] == 1.0
