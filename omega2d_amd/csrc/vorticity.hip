@@ -191,7 +191,8 @@ __device__ __forceinline__ bool tile_near(const double* b, const double* g) {
 // two: its lanes' first and second targets) and torder the targets' order: the target at
 // position p is torder[p] and chunk partials are kept by position (vort_reduce_kernel maps
 // them). A tile is skipped for a group it cannot reach, for both groups at once when neither,
-// and a workgroup whose chunk reaches neither group writes no partials and ends. The grid is one
+// and, after staging, by each wave for its own 64 first or second targets that it cannot reach;
+// a workgroup whose chunk reaches neither group writes no partials and ends. The grid is one
 // dimensional: block b takes (target workgroup, chunk) number (b * perm) % (twgs * nchunk), row
 // major by chunk, so that the few (workgroup, chunk) pairs near the Morton diagonal that hold
 // work are spread over the whole dispatch. flag[workgroup * nchunk8 + chunk] says whether its
@@ -245,6 +246,29 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
     py[t] = ok ? ty[i] : 0.0;
     acc[t] = 0.0;
   }
+  // each wave's box of its 64 first and 64 second targets (a NaN position: unbounded)
+  double wb[kTpl][4];
+#pragma unroll
+  for (int t = 0; t < kTpl; ++t) {
+    const bool ok = ti[t] < nt;
+    const bool bad = ok && (!(px[t] == px[t]) || !(py[t] == py[t]));
+    double x0 = ok ? px[t] : DBL_MAX, x1 = ok ? px[t] : -DBL_MAX;
+    double y0 = ok ? py[t] : DBL_MAX, y1 = ok ? py[t] : -DBL_MAX;
+    if (bad) {
+      x0 = y0 = -INFINITY;
+      x1 = y1 = INFINITY;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      x0 = fmin(x0, __shfl_xor(x0, o));
+      x1 = fmax(x1, __shfl_xor(x1, o));
+      y0 = fmin(y0, __shfl_xor(y0, o));
+      y1 = fmax(y1, __shfl_xor(y1, o));
+    }
+    wb[t][0] = x0;
+    wb[t][1] = x1;
+    wb[t][2] = y0;
+    wb[t][3] = y1;
+  }
   for (int64_t base = j0; base < j1; base += kTile) {
     const int n = (int)((j1 - base) < kTile ? (j1 - base) : kTile);
     bool k0 = true, k1 = true;
@@ -260,6 +284,12 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
       tile[threadIdx.x] = vs ? vs[j] : vsrc_of(sx[j], sy[j], ss[j], sr[j]);
     }
     __syncthreads();
+    // the same test for this wave's targets: uniform over the wave, no barrier below
+    if (tbox) {
+      const double* b = tbox + 5 * (base / kTile);
+      k0 = k0 && tile_near(b, wb[0]);
+      k1 = k1 && tile_near(b, wb[1]);
+    }
     if (k0 && k1) {
       for (int k = 0; k < n; ++k) {
         const VSrc q = tile[k];
@@ -270,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void vort_kernel(int64_t ns, const double* 
           if (!(x >= kUnderflow)) acc[t] += q.c * exp_neg(x, e2t);  // a NaN exponent adds NaN
         }
       }
-    } else {  // one group: the other's pairs with this tile all add nothing
+    } else if (k0 || k1) {  // one group: the other's pairs with this tile all add nothing
       const double qx = k0 ? px[0] : px[1], qy = k0 ? py[0] : py[1];
       double a = k0 ? acc[0] : acc[1];
       for (int k = 0; k < n; ++k) {
