@@ -6,9 +6,10 @@
 // its exponential is skipped without changing the sum. Few targets split the source range into
 // chunks whose fp64 partials are summed in chunk order. With many sources and targets both go
 // in Morton order: a tile of 256 sources carries the box of its positions and its smallest
-// 1/r^2, a workgroup's 512 targets are compact, and a tile whose every pair lies beyond the
-// underflow bound (by a margin far above the rounding of both tests) is skipped whole; it adds
-// only +-0 terms. The Morton order changes the order of the fp64 additions, so sums move by
+// 1/r^2, each group of 256 targets (and each wave's 64) has a compact box, and a tile whose
+// every pair with a group lies beyond the underflow bound (by a margin far above the rounding
+// of both tests) is skipped for that group; it adds only +-0 terms. The exponential is a
+// table-driven exp(-x) on [0, 746). The Morton order changes the order of the fp64 additions, so sums move by
 // rounding only. A NaN position or radius adds NaN, as in the reference (its tile and its
 // targets' group are never skipped); a non-finite strength or a zero radius beyond the bound
 // adds nothing, where the reference's inf*0 gives NaN.
